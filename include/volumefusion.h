@@ -1,0 +1,232 @@
+/*
+ * volumefusion.h -- C ABI of the MI355X (gfx950) warped TSDF integration path.
+ *
+ * Drop-in for VolumeFusion's per-frame TSDF integration: the af:: entry points
+ * of include/af/TSDFWarp.h, include/af/TSDFIntegration.cuh and
+ * af::runTranslateKernel (include/af/VertexManipulation.cuh:16), on the same
+ * data layouts:
+ *   - TSDF: two float[N] arrays (tsdf, weight), z-major,
+ *     idx = z*X*Y + y*X + x (src/TSDFIntegration.cu:88);
+ *   - motion graph: Vec3f[G] positions (12 B packed), Mat4f[G] transforms
+ *     (64 B, column-major), float[G] radii (include/af/DeviceMotionGraph.cuh);
+ *   - knn buffers: Vecui<K>[N], Vecf<K>[N] (K contiguous values per voxel),
+ *     int[N] counts (include/af/DataTypes.cuh);
+ *   - depth: float[W*H] row-major in metres; intrinsics Mat3f column-major.
+ *
+ * Conventions: the caller owns every input/output buffer (device pointers);
+ * calls are asynchronous on the given HIP stream (NULL = default stream) and
+ * return a status code (VF_OK = 0).  The af:: C++ wrapper in
+ * include/af/vf_dropin.h restores the reference's blocking, exception-throwing
+ * behaviour.  K (knn slots per voxel) is a run-time argument, 1 <= K <= 8.
+ *
+ * Canonical knn: where the reference claims knn slots with atomicAdd
+ * (src/TSDFWarp.cu:61-67, a race), these kernels keep the K candidates with the
+ * smallest (distance, node index), in that order.
+ */
+#ifndef VOLUMEFUSION_AMD_H
+#define VOLUMEFUSION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vf_float3 { float x, y, z; } vf_float3; /* Eigen::Vector3f, 12 B */
+typedef struct vf_int3 { int32_t x, y, z; } vf_int3;   /* Eigen::Vector3i       */
+typedef struct vf_int2 { int32_t x, y; } vf_int2;      /* Eigen::Vector2i       */
+typedef struct vf_mat3 { float m[9]; } vf_mat3;        /* Eigen::Matrix3f, column-major */
+typedef struct vf_mat4 { float m[16]; } vf_mat4;       /* Eigen::Matrix4f, column-major */
+typedef void* vf_stream;                               /* hipStream_t */
+
+enum {
+    VF_OK = 0,
+    VF_ERR_INVALID_ARGUMENT = 1,
+    VF_ERR_UNSUPPORTED_K = 2,
+    VF_ERR_HIP = 3,
+    VF_ERR_OUT_OF_MEMORY = 4,
+    VF_ERR_WORKSPACE = 5,
+    VF_ERR_CAPACITY = 6
+};
+
+/* Text of a status code, and the message of the last failure on this thread. */
+const char* vf_status_string(int status);
+const char* vf_last_error(void);
+/* Library version string and the device architecture it was built for. */
+const char* vf_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Reference entry points                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* af::runTsdfKNNGraph, include/af/TSDFWarp.h:9-18 (src/TSDFWarp.cu:23-86).
+ * For every node (positions already translated by the caller) and every voxel
+ * of its cube (voxels_per_node, centred on worldToVoxel(node)) with
+ * |voxelToWorld(v) - node| <= 2 r: candidate.  Writes, for each voxel with at
+ * least one candidate, count = min(#candidates, K) and the K best candidates by
+ * (distance, node index).  Counts must be zero on entry, as the reference's
+ * caller makes them (src/VolumeFusion.cu:358); voxels without candidates are
+ * not written. */
+int vf_tsdf_knn_graph(int k, uint32_t* knn_idxs, float* knn_dists, int32_t* knn_counts, const vf_float3* graph,
+                      const float* graph_radii, size_t graph_size, vf_float3 voxel_size, vf_float3 volume_size,
+                      vf_int3 volume_dim, vf_int3 voxels_per_node, vf_stream stream);
+
+/* af::runComputeTsdfWarpWeightsKernel, include/af/TSDFWarp.h:20-25
+ * (src/TSDFWarp.cu:88-129): w_j = exp(-d_j^2 / (2 r^2)), normalised to sum 1. */
+int vf_compute_tsdf_warp_weights(int k, float* knn_weights, const float* graph_radii, const uint32_t* knn_idxs,
+                                 const float* knn_dists, const int32_t* knn_counts, size_t grid_size,
+                                 vf_stream stream);
+
+/* af::runGenerateTsdfVerticesKernel, include/af/TSDFWarp.h:27 (src/TSDFWarp.cu:131-156). */
+int vf_generate_tsdf_vertices(vf_float3* tsdf_vertices, vf_float3 voxel_size, vf_float3 volume_size,
+                              vf_int3 volume_dim, vf_stream stream);
+
+/* af::runWarpTsdfKernel, include/af/TSDFWarp.h:29-36 (src/TSDFWarp.cu:158-206):
+ * p' = sum_j w_j (T_j [p; 1]).xyz in place. */
+int vf_warp_tsdf(int k, vf_float3* tsdf_vertices, const vf_mat4* motion_graph, const uint32_t* knn_idxs,
+                 const float* knn_weights, const int32_t* knn_counts, vf_float3 voxel_size, vf_float3 volume_size,
+                 vf_int3 volume_dim, vf_stream stream);
+
+/* af::runCountCollisionsPerVoxelKernel, include/af/TSDFIntegration.cuh:49-56
+ * (src/TSDFIntegration.cu:252-299).  collision_counts must be zeroed by the caller. */
+int vf_count_collisions_per_voxel(int32_t* collision_counts, const vf_float3* tsdf_vertices,
+                                  const int32_t* knn_counts, vf_float3 voxel_size, vf_float3 volume_size,
+                                  vf_int3 volume_dim, vf_mat3 volume_rotate, vf_float3 volume_translate,
+                                  vf_stream stream);
+
+/* af::runIntegrateWarpedTSDFFromFrameKernel, include/af/TSDFIntegration.cuh:34-47
+ * (src/TSDFIntegration.cu:173-250).  max_weight replaces the reference's fixed
+ * cap of 15 (src/TSDFIntegration.cu:64); pass 15 for the reference value. */
+int vf_integrate_warped_tsdf(float* tsdf, float* tsdf_weights, const vf_float3* tsdf_vertices,
+                             const int32_t* collision_counts, const int32_t* knn_counts, const float* depth_frame,
+                             vf_int2 frame_dims, float delta, vf_float3 voxel_size, vf_float3 volume_size,
+                             vf_int3 volume_dim, vf_mat3 volume_rotate, vf_float3 volume_translate,
+                             vf_mat3 depth_intrinsics, float max_weight, vf_stream stream);
+
+/* af::runIntegrateTSDFFromFrameKernel, include/af/TSDFIntegration.cuh:8-18
+ * (src/TSDFIntegration.cu:69-116): plain projective integration. */
+int vf_integrate_tsdf(float* tsdf, float* tsdf_weights, const float* depth_frame, vf_int2 frame_dims, float delta,
+                      vf_int3 dims, vf_float3 voxel_size, vf_float3 volume_size, vf_mat3 volume_rotate,
+                      vf_float3 volume_translate, vf_mat3 depth_intrinsics, float max_weight, vf_stream stream);
+
+/* af::runIntegrateTSDFFromCameraFrameKernel, include/af/TSDFIntegration.cuh:20-32
+ * (src/TSDFIntegration.cu:118-171). */
+int vf_integrate_tsdf_camera_frame(float* tsdf, float* tsdf_weights, const float* depth_frame, vf_int2 frame_dims,
+                                   float delta, vf_int3 dims, vf_float3 voxel_size, vf_float3 volume_size,
+                                   vf_mat3 world2camera_rotate, vf_float3 world2camera_translate,
+                                   vf_mat3 volume_rotate, vf_float3 volume_translate, vf_mat3 depth_intrinsics,
+                                   float max_weight, vf_stream stream);
+
+/* af::runTranslateKernel, include/af/VertexManipulation.cuh:16
+ * (src/VertexManipulation.cu:73-87): mesh[i] += translation. */
+int vf_translate(vf_float3* mesh, uint32_t mesh_size, vf_float3 translation, vf_stream stream);
+
+/* Gaussian warp weight of src/TSDFWarp.cu:17 element by element,
+ * out[i] = exp(-(dist[i]^2) / (2 radius[i]^2)), unnormalised: the function
+ * every weight above is built from, exposed for diagnostics. */
+int vf_eval_warp_weight(const float* radius, const float* dist, float* out, size_t n, vf_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused per-frame warped integration                                       */
+/* ------------------------------------------------------------------------ */
+/* One call performs the whole TSDF-integration block of the reference's frame
+ * loop, src/VolumeFusion.cu:702-742 (translate nodes by -c, knn, weights,
+ * vertices, translate back, warp, count collisions, integrate, with volume
+ * rotation I and translation c), in three kernels and with no per-voxel
+ * scratch arrays.  The tsdf/weight arrays are updated in place; when
+ * graph.points_out is not NULL it receives the node positions as the
+ * reference's two translations leave them, fl(fl(p - c) + c). */
+
+typedef struct vf_volume {
+    float* tsdf;          /* float[N], z-major (DeviceTsdf::tsdfPtr)      */
+    float* weights;       /* float[N]                                     */
+    vf_int3 dim;
+    vf_float3 voxel_size; /* size / float(dim) as the caller computes it  */
+    vf_float3 size;
+    vf_float3 center;     /* c: the frame centroid (volume translation)   */
+} vf_volume;
+
+typedef struct vf_graph {
+    const vf_float3* points;  /* Vec3f[G] node positions (camera frame)    */
+    vf_float3* points_out;    /* optional, may alias points                */
+    const vf_mat4* transforms;/* Mat4f[G]                                  */
+    const float* radii;       /* float[G]                                  */
+    size_t count;             /* G                                         */
+} vf_graph;
+
+typedef struct vf_frame {
+    const float* depth;       /* float[W*H], metres                        */
+    int32_t width, height;
+    vf_mat3 intrinsics;
+} vf_frame;
+
+typedef struct vf_warp_params {
+    int k;                    /* knn slots, 1..8                           */
+    float delta;              /* truncation distance                       */
+    float max_weight;         /* weight cap (reference: 15)                */
+    vf_int3 voxels_per_node;  /* node cube, 2*(int(R/vs)+1)+1 per axis     */
+} vf_warp_params;
+
+typedef struct vf_frame_stats {
+    uint64_t supported;       /* voxels with at least one knn candidate    */
+    uint64_t in_bounds;       /* supported voxels with target in volume    */
+    uint64_t pending;         /* voxels that passed the projective tests   */
+    uint64_t collided;        /* pending voxels dropped, target count > 1  */
+    uint64_t updated;         /* voxels whose (tsdf, weight) changed       */
+    uint64_t active_bricks;   /* bricks touched by some node cube          */
+    uint64_t overflow_bricks; /* bricks whose node list spilled            */
+} vf_frame_stats;
+
+typedef struct vf_workspace vf_workspace;
+
+/* Workspace for up to `max_volumes` volumes per call, each with at most
+ * max_voxels voxels and max_nodes nodes (total over the batch).  Allocates
+ * device memory on the current device. */
+int vf_workspace_create(vf_workspace** ws, int max_volumes, uint64_t max_total_voxels, uint64_t max_total_nodes,
+                        vf_stream stream);
+int vf_workspace_destroy(vf_workspace* ws);
+/* Device bytes held by a workspace. */
+uint64_t vf_workspace_bytes(const vf_workspace* ws);
+
+/* One volume, one frame (src/VolumeFusion.cu:702-742). */
+int vf_warped_integrate(vf_workspace* ws, const vf_volume* volume, const vf_graph* graph, const vf_frame* frame,
+                        const vf_warp_params* params, vf_stream stream);
+
+/* Many independent volumes, one frame each, in the same three launches.
+ * params[i] belongs to volume i; all volumes must share params[i].k. */
+int vf_warped_integrate_batch(vf_workspace* ws, int n, const vf_volume* volumes, const vf_graph* graphs,
+                              const vf_frame* frames, const vf_warp_params* params, vf_stream stream);
+
+/* Statistics of the last call on this workspace (blocks on `stream`). */
+int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Device plumbing for hosts without a GPU framework (Python via ctypes)    */
+/* ------------------------------------------------------------------------ */
+int vf_device_count(int* count);
+int vf_set_device(int device);
+int vf_malloc(void** ptr, uint64_t bytes);
+int vf_free(void* ptr);
+int vf_memcpy_h2d(void* dst, const void* src, uint64_t bytes, vf_stream stream);
+int vf_memcpy_d2h(void* dst, const void* src, uint64_t bytes, vf_stream stream);
+int vf_memcpy_d2d(void* dst, const void* src, uint64_t bytes, vf_stream stream);
+int vf_memset(void* dst, int value, uint64_t bytes, vf_stream stream);
+int vf_fill_f32(float* dst, float value, uint64_t count, vf_stream stream);
+int vf_stream_create(vf_stream* stream);
+int vf_stream_destroy(vf_stream stream);
+int vf_stream_synchronize(vf_stream stream);
+int vf_device_synchronize(void);
+int vf_host_alloc(void** ptr, uint64_t bytes);
+int vf_host_free(void* ptr);
+/* Event timing on a stream: record start/stop, read milliseconds. */
+int vf_event_create(void** event);
+int vf_event_destroy(void* event);
+int vf_event_record(void* event, vf_stream stream);
+int vf_event_elapsed_ms(float* ms, void* start, void* stop);
+int vf_event_synchronize(void* event);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+"""ctypes front end of the C oracle (vf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP kernels are compared against.
+Imported by the tests, by the smoke check and by bench.py's CPU-baseline leg;
+never by the volumefusion_amd package.
+
+Arrays are numpy: vectors as float32[..., 3], 4x4 transforms as
+float32[G, 16] column-major (Eigen::Matrix4f storage), intrinsics as
+float32[9] column-major.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libvf_oracle.so"
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+class Vol(C.Structure):
+    _fields_ = [("dim", C.c_int32 * 3), ("vs", C.c_float * 3), ("size", C.c_float * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("supported", C.c_int64),
+        ("in_bounds", C.c_int64),
+        ("collided", C.c_int64),
+        ("updated", C.c_int64),
+        ("candidates", C.c_int64),
+        ("overflow", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class FrameOut(C.Structure):
+    _fields_ = [
+        ("cnt", C.c_void_p),
+        ("idx", C.c_void_p),
+        ("dist", C.c_void_p),
+        ("w", C.c_void_p),
+        ("verts", C.c_void_p),
+        ("col", C.c_void_p),
+        ("pix", C.c_void_p),
+    ]
+
+
+def build() -> Path:
+    """Compile the oracle with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.vfo_expw.argtypes = [C.c_float]
+        L.vfo_expw.restype = C.c_float
+        L.vfo_exp_libm.argtypes = [C.c_float]
+        L.vfo_exp_libm.restype = C.c_float
+        L.vfo_weight_tsdf.argtypes = [C.c_float, C.c_float]
+        L.vfo_weight_tsdf.restype = C.c_float
+        L.vfo_weight_mesh_host.argtypes = [C.c_float, C.c_float]
+        L.vfo_weight_mesh_host.restype = C.c_float
+        L.vfo_div_markstein.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.vfo_div_markstein.restype = C.c_float
+        L.vfo_mat3_inverse.argtypes = [_f32p, _f32p]
+        L.vfo_cvt_i32_rz.argtypes = [C.c_float]
+        L.vfo_cvt_i32_rz.restype = C.c_int32
+        L.vfo_translate.argtypes = [_f32p, C.c_size_t, _f32p]
+        L.vfo_generate_vertices.argtypes = [_f32p, C.POINTER(Vol)]
+        L.vfo_knn_graph.argtypes = [C.c_int, _u32p, _f32p, _i32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol), _i32p]
+        L.vfo_knn_graph.restype = C.c_int64
+        L.vfo_warp_weights.argtypes = [C.c_int, _f32p, _f32p, _u32p, _f32p, _i32p, C.c_size_t]
+        L.vfo_warp_tsdf.argtypes = [C.c_int, _f32p, _f32p, _u32p, _f32p, _i32p, C.c_size_t]
+        L.vfo_count_collisions.argtypes = [_i32p, _f32p, _i32p, C.POINTER(Vol), _f32p, _f32p]
+        L.vfo_integrate_warped.argtypes = [_f32p, _f32p, _f32p, _i32p, _i32p, _f32p, C.c_int, C.c_int, C.c_float,
+                                           C.POINTER(Vol), _f32p, _f32p, _f32p, C.c_float]
+        L.vfo_integrate_warped.restype = C.c_int64
+        L.vfo_integrate_plain.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float, C.POINTER(Vol), _f32p,
+                                          _f32p, _f32p, C.c_float]
+        L.vfo_integrate_plain.restype = C.c_int64
+        L.vfo_integrate_camera_frame.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float, C.POINTER(Vol),
+                                                 _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.vfo_integrate_camera_frame.restype = C.c_int64
+        L.vfo_tsdfvolume_integrate.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float, C.POINTER(Vol),
+                                               _f32p, _f32p]
+        L.vfo_tsdfvolume_integrate.restype = C.c_int64
+        L.vfo_mesh_warp.argtypes = [_f32p, _f32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_size_t, C.c_int, C.c_int,
+                                    C.c_int]
+        L.vfo_warped_frame.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol), _i32p,
+                                       _f32p, _f32p, C.c_int, C.c_int, C.c_float, _f32p, C.c_float, C.POINTER(Stats),
+                                       C.POINTER(FrameOut)]
+        L.vfo_warped_frame.restype = C.c_int
+        L.vfo_warped_frame_gather.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_size_t,
+                                              C.POINTER(Vol), _i32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float,
+                                              _f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(Stats)]
+        L.vfo_warped_frame_gather.restype = C.c_int
+        L.vfo_warped_frame_f64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol),
+                                           _f32p, _f32p, C.c_int, C.c_float, C.c_float, _i32p, _u32p, _i32p]
+        L.vfo_warped_frame_f64.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def make_vol(dim, vs, size) -> Vol:
+    v = Vol()
+    for a in range(3):
+        v.dim[a] = int(dim[a])
+        v.vs[a] = float(np.float32(vs[a]))
+        v.size[a] = float(np.float32(size[a]))
+    return v
+
+
+@dataclass
+class FrameResult:
+    stats: dict
+    cnt: np.ndarray | None = None
+    idx: np.ndarray | None = None
+    dist: np.ndarray | None = None
+    w: np.ndarray | None = None
+    verts: np.ndarray | None = None
+    col: np.ndarray | None = None
+    pix: np.ndarray | None = None
+
+
+# ---- scalar helpers -----------------------------------------------------
+
+def expw(x: float) -> float:
+    return lib().vfo_expw(float(x))
+
+
+def expw_array(x: np.ndarray) -> np.ndarray:
+    f = lib().vfo_expw
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
+
+
+def mat3_inverse(m9: np.ndarray) -> np.ndarray:
+    out = np.zeros(9, np.float32)
+    lib().vfo_mat3_inverse(_f32(m9), out)
+    return out
+
+
+# ---- reference kernels one by one ----------------------------------------
+
+def translate(pts: np.ndarray, t) -> np.ndarray:
+    p = _f32(pts).copy()
+    lib().vfo_translate(p, p.shape[0], _f32(t))
+    return p
+
+
+def generate_vertices(dim, vs, size) -> np.ndarray:
+    n = int(dim[0]) * int(dim[1]) * int(dim[2])
+    out = np.zeros((n, 3), np.float32)
+    v = make_vol(dim, vs, size)
+    lib().vfo_generate_vertices(out, C.byref(v))
+    return out
+
+
+def knn_graph(k, nodes, radii, dim, vs, size, cube):
+    n = int(dim[0]) * int(dim[1]) * int(dim[2])
+    idx = np.zeros((n, k), np.uint32)
+    dist = np.zeros((n, k), np.float32)
+    cnt = np.zeros(n, np.int32)
+    v = make_vol(dim, vs, size)
+    nodes = _f32(nodes)
+    of = lib().vfo_knn_graph(k, idx, dist, cnt, nodes, _f32(radii), nodes.shape[0], C.byref(v),
+                             np.ascontiguousarray(cube, np.int32))
+    return idx, dist, cnt, int(of)
+
+
+def warp_weights(k, radii, idx, dist, cnt) -> np.ndarray:
+    n = cnt.shape[0]
+    w = np.zeros((n, k), np.float32)
+    lib().vfo_warp_weights(k, w, _f32(radii), np.ascontiguousarray(idx, np.uint32), _f32(dist),
+                           np.ascontiguousarray(cnt, np.int32), n)
+    return w
+
+
+def warp_tsdf(k, verts, T, idx, w, cnt) -> np.ndarray:
+    v = _f32(verts).copy()
+    lib().vfo_warp_tsdf(k, v, _f32(T), np.ascontiguousarray(idx, np.uint32), _f32(w),
+                        np.ascontiguousarray(cnt, np.int32), cnt.shape[0])
+    return v
+
+
+def count_collisions(verts, cnt, dim, vs, size, R, t) -> np.ndarray:
+    n = cnt.shape[0]
+    col = np.zeros(n, np.int32)
+    v = make_vol(dim, vs, size)
+    lib().vfo_count_collisions(col, _f32(verts), np.ascontiguousarray(cnt, np.int32), C.byref(v), _f32(R), _f32(t))
+    return col
+
+
+def integrate_warped(tsdf, wt, verts, col, cnt, depth, delta, dim, vs, size, R, t, Kin, maxw):
+    ts = _f32(tsdf).copy()
+    ww = _f32(wt).copy()
+    H, W = depth.shape
+    v = make_vol(dim, vs, size)
+    u = lib().vfo_integrate_warped(ts, ww, _f32(verts), np.ascontiguousarray(col, np.int32),
+                                   np.ascontiguousarray(cnt, np.int32), _f32(depth), W, H, float(delta), C.byref(v),
+                                   _f32(R), _f32(t), _f32(Kin), float(maxw))
+    return ts, ww, int(u)
+
+
+def integrate_plain(tsdf, wt, depth, delta, dim, vs, size, R, t, Kin, maxw):
+    ts = _f32(tsdf).copy()
+    ww = _f32(wt).copy()
+    H, W = depth.shape
+    v = make_vol(dim, vs, size)
+    u = lib().vfo_integrate_plain(ts, ww, _f32(depth), W, H, float(delta), C.byref(v), _f32(R), _f32(t), _f32(Kin),
+                                  float(maxw))
+    return ts, ww, int(u)
+
+
+def integrate_camera_frame(tsdf, wt, depth, delta, dim, vs, size, Rw2c, tw2c, R, t, Kin, maxw):
+    ts = _f32(tsdf).copy()
+    ww = _f32(wt).copy()
+    H, W = depth.shape
+    v = make_vol(dim, vs, size)
+    u = lib().vfo_integrate_camera_frame(ts, ww, _f32(depth), W, H, float(delta), C.byref(v), _f32(Rw2c),
+                                         _f32(tw2c), _f32(R), _f32(t), _f32(Kin), float(maxw))
+    return ts, ww, int(u)
+
+
+def tsdfvolume_integrate(tsdf, wt, depth, delta, dim, vs, size, pose16, Kin):
+    ts = _f32(tsdf).copy()
+    ww = _f32(wt).copy()
+    H, W = depth.shape
+    v = make_vol(dim, vs, size)
+    u = lib().vfo_tsdfvolume_integrate(ts, ww, _f32(depth), W, H, float(delta), C.byref(v), _f32(pose16), _f32(Kin))
+    return ts, ww, int(u)
+
+
+def mesh_warp(pts, nodes, radii, T, k, host_order=True, squared_dist=False) -> np.ndarray:
+    pts = _f32(pts)
+    out = np.zeros_like(pts)
+    nodes = _f32(nodes)
+    lib().vfo_mesh_warp(out, pts, pts.shape[0], nodes, _f32(radii), _f32(T), nodes.shape[0], int(k),
+                        1 if host_order else 0, 1 if squared_dist else 0)
+    return out
+
+
+# ---- whole frame ----------------------------------------------------------
+
+def warped_frame(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw, want=()):
+    """One frame of src/VolumeFusion.cu:699-742.  tsdf, wt and nodes are
+    updated in place (nodes get the reference's round trip).  `want` names the
+    per-voxel outputs to return: cnt, idx, dist, w, verts, col, pix."""
+    n = int(dim[0]) * int(dim[1]) * int(dim[2])
+    out = FrameOut()
+    keep = {}
+    shapes = {"cnt": ((n,), np.int32), "idx": ((n, k), np.uint32), "dist": ((n, k), np.float32),
+              "w": ((n, k), np.float32), "verts": ((n, 3), np.float32), "col": ((n,), np.int32),
+              "pix": ((n,), np.int32)}
+    for name in want:
+        shp, dt = shapes[name]
+        arr = np.zeros(shp, dt)
+        keep[name] = arr
+        setattr(out, name, arr.ctypes.data)
+    st = Stats()
+    v = make_vol(dim, vs, size)
+    H, W = depth.shape
+    assert tsdf.dtype == np.float32 and wt.dtype == np.float32 and nodes.dtype == np.float32
+    rc = lib().vfo_warped_frame(k, tsdf, wt, nodes, _f32(radii), _f32(T), nodes.shape[0], C.byref(v),
+                                np.ascontiguousarray(cube, np.int32), _f32(c), _f32(depth), W, H, float(delta),
+                                _f32(Kin), float(maxw), C.byref(st), C.byref(out))
+    if rc != 0:
+        raise MemoryError("oracle frame allocation failed")
+    return FrameResult(stats=st.as_dict(), **keep)
+
+
+def warped_frame_gather(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw,
+                        nthreads=None, z0=0, z1=0) -> dict:
+    """Voxel-major multi-threaded restatement of the same frame (sources in
+    z in [z0, z1); z1 <= 0 = whole volume).  Updates tsdf, wt, nodes in place."""
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    st = Stats()
+    v = make_vol(dim, vs, size)
+    H, W = depth.shape
+    assert tsdf.dtype == np.float32 and wt.dtype == np.float32 and nodes.dtype == np.float32
+    rc = lib().vfo_warped_frame_gather(k, tsdf, wt, nodes, _f32(radii), _f32(T), nodes.shape[0], C.byref(v),
+                                       np.ascontiguousarray(cube, np.int32), _f32(c), _f32(depth), W, H,
+                                       float(delta), _f32(Kin), float(maxw), int(nthreads), int(z0), int(z1),
+                                       C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle gather frame failed ({rc})")
+    return st.as_dict()
+
+
+def warped_frame_f64(k, tsdf64, wt64, nodes_before, radii, T, dim, vs, size, c, depth, delta, maxw, cnt, idx, pix):
+    v = make_vol(dim, vs, size)
+    H, W = depth.shape
+    lib().vfo_warped_frame_f64(k, tsdf64, wt64, _f32(nodes_before), _f32(radii), _f32(T), nodes_before.shape[0],
+                               C.byref(v), _f32(c), _f32(depth), W, float(delta), float(maxw),
+                               np.ascontiguousarray(cnt, np.int32), np.ascontiguousarray(idx, np.uint32),
+                               np.ascontiguousarray(pix, np.int32))

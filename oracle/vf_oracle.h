@@ -1,0 +1,103 @@
+/*
+ * vf_oracle.h -- CPU restatement of VolumeFusion's warped TSDF integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP kernels
+ * are compared against.  Only the test suite, the smoke check and the CPU
+ * baseline leg of bench.py load it; the shipped library never links, loads or
+ * calls it.
+ *
+ * Every function restates one reference function (file:line in the comment
+ * beside it) with the same floating-point operation order as the reference's
+ * device code (Eigen's scalar, coefficient-based evaluation), every operation
+ * rounded on its own (no fused multiply-add contraction).  Vectors are packed
+ * float triples (Eigen::Vector3f, 12 bytes); 4x4 and 3x3 matrices are
+ * column-major float arrays (Eigen default storage).
+ */
+#ifndef VF_ORACLE_H
+#define VF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Volume geometry as the reference passes it: dims, voxel size, volume size. */
+typedef struct {
+    int32_t dim[3];
+    float vs[3];   /* voxel size = size / float(dim), src/VolumeFusion.cu:229 */
+    float size[3];
+} vfo_vol;
+
+typedef struct {
+    int64_t supported;      /* voxels with knn count > 0                       */
+    int64_t in_bounds;      /* supported voxels whose warped target is inside  */
+    int64_t collided;       /* supported voxels skipped because col[u] > 1     */
+    int64_t updated;        /* voxels whose (tsdf, weight) changed             */
+    int64_t candidates;     /* (voxel, node) pairs passing the 2r test         */
+    int64_t overflow;       /* voxels with more than K candidates              */
+} vfo_stats;
+
+/* ---- scalar helpers ---------------------------------------------------- */
+float vfo_expw(float x);
+float vfo_exp_libm(float x);
+float vfo_weight_tsdf(float radius, float dist);
+float vfo_weight_mesh_host(float radius, float dist);
+float vfo_div_markstein(float a, float b, float recip_b);
+void vfo_mat3_inverse(const float* m, float* out);
+int32_t vfo_cvt_i32_rz(float f);
+
+/* ---- reference kernels, one by one --------------------------------------- */
+void vfo_translate(float* pts, size_t n, const float* t);
+void vfo_generate_vertices(float* verts, const vfo_vol* vol);
+int64_t vfo_knn_graph(int K, uint32_t* idx, float* dist, int32_t* cnt, const float* nodes, const float* radii,
+                      size_t G, const vfo_vol* vol, const int32_t* cube);
+void vfo_warp_weights(int K, float* w, const float* radii, const uint32_t* idx, const float* dist, const int32_t* cnt,
+                      size_t N);
+void vfo_warp_tsdf(int K, float* verts, const float* T, const uint32_t* idx, const float* w, const int32_t* cnt,
+                   size_t N);
+void vfo_count_collisions(int32_t* col, const float* verts, const int32_t* cnt, const vfo_vol* vol, const float* R,
+                          const float* t);
+int64_t vfo_integrate_warped(float* tsdf, float* wt, const float* verts, const int32_t* col, const int32_t* cnt,
+                             const float* depth, int W, int H, float delta, const vfo_vol* vol, const float* R,
+                             const float* t, const float* Kin, float maxw);
+int64_t vfo_integrate_plain(float* tsdf, float* wt, const float* depth, int W, int H, float delta,
+                            const vfo_vol* vol, const float* R, const float* t, const float* Kin, float maxw);
+int64_t vfo_integrate_camera_frame(float* tsdf, float* wt, const float* depth, int W, int H, float delta,
+                                   const vfo_vol* vol, const float* Rw2c, const float* tw2c, const float* R,
+                                   const float* t, const float* Kin, float maxw);
+int64_t vfo_tsdfvolume_integrate(float* tsdf, float* wt, const float* depth, int W, int H, float delta,
+                                 const vfo_vol* vol, const float* pose, const float* Kin);
+void vfo_mesh_warp(float* out, const float* pts, size_t n, const float* nodes, const float* radii, const float* T,
+                   size_t G, int K, int host_order, int squared_dist);
+
+/* ---- the per-frame call sequence (src/VolumeFusion.cu:699-742) ----------- */
+/* Optional per-voxel outputs of one frame; any pointer may be NULL. */
+typedef struct {
+    int32_t* cnt;     /* [N]   knn counts                        */
+    uint32_t* idx;    /* [N*K] knn node indices (sorted slots)   */
+    float* dist;      /* [N*K] knn distances                     */
+    float* w;         /* [N*K] normalised blend weights          */
+    float* verts;     /* [N*3] warped vertices (camera frame)    */
+    int32_t* col;     /* [N]   collision counts                  */
+    int32_t* pix;     /* [N]   depth pixel used, -1 if no update */
+} vfo_frame_out;
+
+int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T, size_t G,
+                     const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth, int W, int H,
+                     float delta, const float* Kin, float maxw, vfo_stats* st, const vfo_frame_out* out);
+
+int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T,
+                            size_t G, const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth,
+                            int W, int H, float delta, const float* Kin, float maxw, int nthreads, int z0, int z1,
+                            vfo_stats* st);
+
+int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes, const float* radii,
+                         const float* T, size_t G, const vfo_vol* vol, const float* c, const float* depth, int W,
+                         float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,116 @@
+"""GPU parity of the fused three-kernel frame (and its batched form) against
+the oracle's frame, bit for bit, over several frames."""
+import numpy as np
+import pytest
+
+from volumefusion_amd import ops, scenes
+from volumefusion_amd.device import DeviceArray
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def dev(a):
+    return DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True):
+    n = s.n_voxels
+    t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+    tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
+    graph = ops.DeviceMotionGraph(s.nodes, s.transforms(frames[0]), s.radii)
+    ws = ops.Workspace()
+    for f in frames:
+        depth, T = s.depth(f), s.transforms(f)
+        if gather:
+            st = oracle.warped_frame_gather(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size,
+                                            s.cube, s.center, depth, s.delta, s.intrinsics(), s.max_weight,
+                                            nthreads=threads)
+        else:
+            st = oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size, s.cube,
+                                     s.center, depth, s.delta, s.intrinsics(), s.max_weight).stats
+        graph.transforms.upload(T)
+        ops.warped_integrate(ws, tsdf, graph, dev(depth), s.intrinsics(), s.k, s.delta, s.max_weight, s.cube)
+        gs = ws.last_stats()
+        if check_stats:
+            assert gs["supported"] == st["supported"]
+            assert gs["in_bounds"] == st["in_bounds"]
+            assert gs["updated"] == st["updated"]
+        np.testing.assert_array_equal(tsdf.weights.download(), w_or, err_msg=f"weights, frame {f}")
+        np.testing.assert_array_equal(tsdf.tsdf.download(), t_or, err_msg=f"tsdf, frame {f}")
+        np.testing.assert_array_equal(graph.points.download(), nodes_or, err_msg=f"nodes, frame {f}")
+    return st
+
+
+@pytest.mark.parametrize("k", [1, 4, 6, 8])
+def test_fused_small_multi_frame(vf, oracle, k):
+    s = scenes.small_scene(seed=11, k=k, nodes=70)
+    st = run_frames(vf, oracle, s, [0, 1, 2, 3])
+    assert st["updated"] > 500 and st["collided"] > 0
+
+
+def test_fused_spill_path(vf, oracle):
+    """Dense node cluster: brick lists above 128 entries take the all-nodes
+    path inside the kernel."""
+    s = scenes.small_scene(seed=12, k=8, nodes=400, radius=0.03)
+    run_frames(vf, oracle, s, [0, 1])
+
+
+def test_fused_empty_graph_leaves_volume(vf, oracle):
+    s = scenes.small_scene(seed=13, k=4, nodes=8)
+    s.nodes = s.nodes[:0]
+    s.radii = s.radii[:0]
+    tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
+    graph = ops.DeviceMotionGraph(np.zeros((0, 3), F), np.zeros((0, 16), F), np.zeros(0, F))
+    ws = ops.Workspace()
+    ops.warped_integrate(ws, tsdf, graph, dev(s.depth(0)), s.intrinsics(), 4, s.delta, 64.0, s.cube)
+    assert (tsdf.tsdf.download() == -1).all() and (tsdf.weights.download() == 0).all()
+
+
+def test_fused_config1(vf, oracle):
+    """128^3, 256 nodes, identity transforms, k=4 (the CPU-runnable config)."""
+    s = scenes.config1()
+    st = run_frames(vf, oracle, s, [0, 1])
+    assert st["updated"] > 10000
+
+
+@pytest.mark.slow
+def test_fused_config2_frames(vf, oracle):
+    """256^3, 2048 nodes with random warps, k=8: first three frames against the
+    gather oracle (itself equal to the node-major oracle)."""
+    s = scenes.config2()
+    run_frames(vf, oracle, s, [0, 1, 2], gather=True)
+
+
+def test_batched_equals_sequential(vf, oracle):
+    """Volumes of uneven size and node count in one batched call give the same
+    bits as one call per volume, and as the oracle."""
+    vols = [scenes.small_scene(seed=20 + i, k=8, nodes=30 + 25 * i, dim=(24 + 8 * i, 32, 20 + 12 * i))
+            for i in range(4)]
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii) for s in vols]
+    depths = [dev(s.depth(0)) for s in vols]
+    ws = ops.Workspace()
+    ops.warped_integrate_batch(ws, tsdfs, graphs, depths, [s.intrinsics() for s in vols], 8, 0.012, 64.0,
+                               [s.cube for s in vols])
+    for s, t, g in zip(vols, tsdfs, graphs):
+        n = s.n_voxels
+        t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+        oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, s.transforms(0), s.dim, s.voxel_size, s.size,
+                            s.cube, s.center, s.depth(0), s.delta, s.intrinsics(), s.max_weight)
+        np.testing.assert_array_equal(t.tsdf.download(), t_or)
+        np.testing.assert_array_equal(t.weights.download(), w_or)
+        np.testing.assert_array_equal(g.points.download(), nodes_or)
+
+
+def test_fused_deterministic_repeat(vf, oracle):
+    """Same inputs twice (fresh volumes): identical bits (no race in the knn)."""
+    s = scenes.small_scene(seed=14, k=8, nodes=90)
+    out = []
+    for _ in range(2):
+        tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
+        graph = ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii)
+        ops.warped_integrate(ops.Workspace(), tsdf, graph, dev(s.depth(0)), s.intrinsics(), 8, s.delta, 64.0, s.cube)
+        out.append((tsdf.tsdf.download(), tsdf.weights.download()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])

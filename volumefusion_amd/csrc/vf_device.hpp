@@ -1,0 +1,225 @@
+// vf_device.hpp -- device-side arithmetic shared by every warped-TSDF kernel.
+//
+// All translation units are compiled with -ffp-contract=off: each float
+// operation below is rounded on its own, in the order the reference's device
+// code evaluates it (Eigen's scalar coefficient-based path; sums of three
+// terms are a0 + (a1 + a2), Eigen/src/Core/Redux.h:99-113).  Fused
+// multiply-adds appear only where an algorithm here is defined with one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vf {
+
+constexpr float kInf = __builtin_huge_valf();
+
+// float -> int32, truncation toward zero, saturating, NaN -> 0.
+__device__ __forceinline__ int32_t cvt_i32_rz(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return 2147483647;
+    if (f < -2147483648.0f) return (-2147483647 - 1);
+    return (int32_t)f;
+}
+
+// e^x in binary64, rounded once to binary32: Cody & Waite (1980) two-constant
+// reduction x = k ln2 + r (fdlibm's ln2 split), degree-11 Taylor polynomial of
+// e^r by Horner's rule with fused multiply-adds, scaling by 2^k.  Inputs below
+// -87 give 0.  Every step is an exactly specified IEEE operation, so a host
+// build of the same steps gives the same bits.
+__device__ __forceinline__ float expw(float xf) {
+    if (xf != xf) return xf;
+    if (xf < -87.0f) return 0.0f;
+    if (xf > 88.5f) return kInf;
+    const double x = (double)xf;
+    const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-k, 0x1.62e42fee00000p-1, x);
+    r = __builtin_fma(-k, 0x1.a39ef35793c76p-33, r);
+    double p = 0x1.ae64567f544e4p-26;
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return (float)__builtin_ldexp(p, (int)k);
+}
+
+// Gaussian warp weight of src/TSDFWarp.cu:17, exp(-(d*d) / (2 * r*r)), no
+// cutoff.  `den` is 2*(r*r), evaluated once per node.
+__device__ __forceinline__ float warp_weight(float den, float d) {
+    const float num = -(d * d);
+    return expw(num / den);
+}
+
+// Per-volume geometry, as the reference derives it from (dims, voxel size,
+// volume size): inv = float(1.0 / double(vs)) (src/TSDFWarp.cu:12) and
+// halfs = size * 0.5f.
+struct Geom {
+    int32_t dim[3];
+    float vs[3];
+    float inv[3];
+    float halfs[3];
+};
+
+// voxelToWorld, src/TSDFWarp.cu:8-10: the voxel's min corner.
+__device__ __forceinline__ void voxel_to_world(const Geom& g, int32_t x, int32_t y, int32_t z, float p[3]) {
+    const float ax = (float)x * g.vs[0];
+    const float ay = (float)y * g.vs[1];
+    const float az = (float)z * g.vs[2];
+    p[0] = ax - g.halfs[0];
+    p[1] = ay - g.halfs[1];
+    p[2] = az - g.halfs[2];
+}
+
+// worldToVoxel, src/TSDFWarp.cu:11-15: truncation toward zero.
+__device__ __forceinline__ void world_to_voxel(const Geom& g, const float p[3], int32_t v[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float s = p[a] + g.halfs[a];
+        v[a] = cvt_i32_rz(s * g.inv[a]);
+    }
+}
+
+// Mat3f (column-major) times Vec3f, Eigen device order a0 + (a1 + a2).
+__device__ __forceinline__ void mat3_vec(const float* m, const float v[3], float o[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a0 = m[0 + i] * v[0];
+        const float a1 = m[3 + i] * v[1];
+        const float a2 = m[6 + i] * v[2];
+        o[i] = a0 + (a1 + a2);
+    }
+}
+
+// (T * p.homogeneous()).head(3) with rows[i] = (T(i,0), T(i,1), T(i,2), T(i,3)):
+// Eigen evaluates leftCols<3>() * p then adds the last column
+// (Eigen/src/Geometry/Homogeneous.h:268-276).
+__device__ __forceinline__ void affine_apply(const float4& r0, const float4& r1, const float4& r2, const float p[3],
+                                             float o[3]) {
+    {
+        const float a0 = r0.x * p[0], a1 = r0.y * p[1], a2 = r0.z * p[2];
+        const float s = a0 + (a1 + a2);
+        o[0] = s + r0.w;
+    }
+    {
+        const float a0 = r1.x * p[0], a1 = r1.y * p[1], a2 = r1.z * p[2];
+        const float s = a0 + (a1 + a2);
+        o[1] = s + r1.w;
+    }
+    {
+        const float a0 = r2.x * p[0], a1 = r2.y * p[1], a2 = r2.z * p[2];
+        const float s = a0 + (a1 + a2);
+        o[2] = s + r2.w;
+    }
+}
+
+// Target voxel of a warped vertex (src/TSDFIntegration.cu:272-279):
+// u = worldToVoxel(Rinv * (p' - t)).  Returns false when u is outside.
+__device__ __forceinline__ bool warped_target(const Geom& g, const float* Rinv, const float t[3], const float pw[3],
+                                              int32_t u[3]) {
+    float q[3], o[3];
+    q[0] = pw[0] - t[0];
+    q[1] = pw[1] - t[1];
+    q[2] = pw[2] - t[2];
+    mat3_vec(Rinv, q, o);
+    world_to_voxel(g, o, u);
+    return !(u[0] < 0 || u[1] < 0 || u[2] < 0 || u[0] >= g.dim[0] || u[1] >= g.dim[1] || u[2] >= g.dim[2]);
+}
+
+struct Camera {
+    float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)
+    int32_t W, H;
+    float delta, maxw;
+};
+
+// project3dToFrame + the tests of updateTsdf up to the read-modify-write,
+// src/TSDFIntegration.cu:16-58.  Returns the pixel index, or -1 when the
+// voxel is not updated; *tn is the new truncated sdf sample.
+__device__ __forceinline__ int64_t project_sample(const Camera& cam, const float* __restrict__ depth, const float p[3],
+                                                  float* tn) {
+    if (p[2] < 1.e-4f) return -1;
+    const float zInv = 1.0f / p[2];
+    const float ux = cam.fx * p[0];
+    const float uf = ux * zInv + cam.cx;
+    const float vy = cam.fy * p[1];
+    const float vf = vy * zInv + cam.cy;
+    const int32_t px = cvt_i32_rz(uf + 0.5f);
+    const int32_t py = cvt_i32_rz(vf + 0.5f);
+    if (px < 0 || px >= cam.W || py < 0 || py >= cam.H) return -1;
+    const int64_t pi = (int64_t)py * cam.W + px;
+    const float d = depth[pi];
+    if (__builtin_isinf(d) || d < 1.e-4f) return -1;
+    const float sdf = p[2] - d;
+    if (sdf > cam.delta) return -1;
+    *tn = (sdf < -cam.delta) ? -1.0f : sdf / cam.delta;
+    return pi;
+}
+
+// The read-modify-write of updateTsdf, src/TSDFIntegration.cu:60-64: the
+// running average uses the uncapped w_old + 1; the stored weight is capped.
+__device__ __forceinline__ void blend_sample(float to, float wo, float tn, float maxw, float* t_out, float* w_out) {
+    const float a = to * wo;
+    const float b = tn * 1.0f;
+    const float wn = wo + 1.0f;
+    *t_out = (a + b) / wn;
+    *w_out = (maxw < wn) ? maxw : wn;
+}
+
+// Canonical knn list kept sorted ascending by (dist, node order) in
+// registers.  Callers offer candidates in ascending node order, so a strict
+// `<` on the distance keeps the lower index first among equal distances.
+template <int K>
+struct KnnList {
+    float d[K];
+    uint32_t i[K];
+    int n;
+    int seen;
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            d[s] = kInf;
+            i[s] = 0xffffffffu;
+        }
+        n = 0;
+        seen = 0;
+    }
+
+    __device__ __forceinline__ void offer(float dist, uint32_t id) {
+        ++seen;
+        if (dist < d[K - 1]) {
+            bool c[K];
+#pragma unroll
+            for (int s = 0; s < K; ++s) c[s] = dist < d[s];
+#pragma unroll
+            for (int s = K - 1; s > 0; --s) {
+                const float ds = c[s - 1] ? d[s - 1] : (c[s] ? dist : d[s]);
+                const uint32_t is = c[s - 1] ? i[s - 1] : (c[s] ? id : i[s]);
+                d[s] = ds;
+                i[s] = is;
+            }
+            if (c[0]) {
+                d[0] = dist;
+                i[0] = id;
+            }
+            if (n < K) ++n;
+        } else if (n < K) {
+            // Room left and dist is +inf (or NaN): a list insertion appends it.
+#pragma unroll
+            for (int s = 0; s < K; ++s)
+                if (s == n) {
+                    d[s] = dist;
+                    i[s] = id;
+                }
+            ++n;
+        }
+    }
+};
+
+}  // namespace vf

@@ -1,0 +1,293 @@
+"""The reference's af:: entry points on the gfx950 kernels.
+
+Each function keeps the reference's name (snake_case), argument order and
+meaning, and, like the reference wrappers (which end in cudaDeviceSynchronize,
+e.g. src/TSDFWarp.cu:85), returns only when the work is done.  Errors raise
+VFError (the reference throws std::runtime_error from CUDA_CHECK).
+
+Device buffers are DeviceArray objects on the reference layouts:
+  knn idx  uint32[N, K]   (Vecui<K>[N])     knn dist / weights float32[N, K]
+  counts   int32[N]       tsdf / weights    float32[N] z-major
+  vertices float32[N, 3]  (Vec3f[N])        transforms float32[G, 16] column-major
+  depth    float32[H, W]
+Small types (Vec3f, Vec3i, Mat3f) are plain sequences; Mat3f column-major.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .device import DeviceArray, Stream, default_stream
+
+IDENTITY3 = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+
+def _s(stream: Stream | None):
+    return (stream or default_stream()).ptr
+
+
+def _done(stream: Stream | None) -> None:
+    (stream or default_stream()).synchronize()
+
+
+def _k_of(arr: DeviceArray) -> int:
+    if len(arr.shape) != 2:
+        raise ValueError("knn buffers are [N, K]")
+    return arr.shape[1]
+
+
+def run_tsdf_knn_graph(knn_idxs: DeviceArray, knn_dists: DeviceArray, knn_counts: DeviceArray, graph: DeviceArray,
+                       graph_radiuses: DeviceArray, graph_size: int, voxel_size, volume_size, volume_dim,
+                       voxels_per_node, stream: Stream | None = None) -> None:
+    """af::runTsdfKNNGraph (include/af/TSDFWarp.h:9-18)."""
+    L.call("vf_tsdf_knn_graph", _k_of(knn_idxs), knn_idxs.ptr, knn_dists.ptr, knn_counts.ptr, graph.ptr,
+           graph_radiuses.ptr, int(graph_size), L.f3(voxel_size), L.f3(volume_size), L.i3(volume_dim),
+           L.i3(voxels_per_node), _s(stream))
+    _done(stream)
+
+
+def run_compute_tsdf_warp_weights_kernel(knn_weights: DeviceArray, motion_graph_radiuses: DeviceArray,
+                                         knn_idxs: DeviceArray, knn_dists: DeviceArray, knn_counts: DeviceArray,
+                                         grid_size: int, stream: Stream | None = None) -> None:
+    """af::runComputeTsdfWarpWeightsKernel (include/af/TSDFWarp.h:20-25)."""
+    L.call("vf_compute_tsdf_warp_weights", _k_of(knn_weights), knn_weights.ptr, motion_graph_radiuses.ptr,
+           knn_idxs.ptr, knn_dists.ptr, knn_counts.ptr, int(grid_size), _s(stream))
+    _done(stream)
+
+
+def run_generate_tsdf_vertices_kernel(tsdf_vertices: DeviceArray, voxel_size, volume_size, volume_dim,
+                                      stream: Stream | None = None) -> None:
+    """af::runGenerateTsdfVerticesKernel (include/af/TSDFWarp.h:27)."""
+    L.call("vf_generate_tsdf_vertices", tsdf_vertices.ptr, L.f3(voxel_size), L.f3(volume_size), L.i3(volume_dim),
+           _s(stream))
+    _done(stream)
+
+
+def run_warp_tsdf_kernel(tsdf_vertices: DeviceArray, motion_graph: DeviceArray, knn_idxs: DeviceArray,
+                         knn_weights: DeviceArray, knn_counts: DeviceArray, voxel_size, volume_size, volume_dim,
+                         stream: Stream | None = None) -> None:
+    """af::runWarpTsdfKernel (include/af/TSDFWarp.h:29-36)."""
+    L.call("vf_warp_tsdf", _k_of(knn_idxs), tsdf_vertices.ptr, motion_graph.ptr, knn_idxs.ptr, knn_weights.ptr,
+           knn_counts.ptr, L.f3(voxel_size), L.f3(volume_size), L.i3(volume_dim), _s(stream))
+    _done(stream)
+
+
+def run_count_collisions_per_voxel_kernel(collision_counts: DeviceArray, tsdf_vertices: DeviceArray,
+                                          knn_counts: DeviceArray, voxel_size, volume_size, volume_dim,
+                                          volume_rotate, volume_translate, stream: Stream | None = None) -> None:
+    """af::runCountCollisionsPerVoxelKernel (include/af/TSDFIntegration.cuh:49-56)."""
+    L.call("vf_count_collisions_per_voxel", collision_counts.ptr, tsdf_vertices.ptr, knn_counts.ptr,
+           L.f3(voxel_size), L.f3(volume_size), L.i3(volume_dim), L.m3(volume_rotate), L.f3(volume_translate),
+           _s(stream))
+    _done(stream)
+
+
+def run_integrate_warped_tsdf_from_frame_kernel(tsdf: DeviceArray, tsdf_weights: DeviceArray,
+                                                tsdf_vertices: DeviceArray, collision_counts: DeviceArray,
+                                                knn_counts: DeviceArray, depth_frame: DeviceArray, frame_dims,
+                                                delta: float, voxel_size, volume_size, volume_dim, volume_rotate,
+                                                volume_translate, depth_intrinsics, max_weight: float = 15.0,
+                                                stream: Stream | None = None) -> None:
+    """af::runIntegrateWarpedTSDFFromFrameKernel (include/af/TSDFIntegration.cuh:34-47)."""
+    L.call("vf_integrate_warped_tsdf", tsdf.ptr, tsdf_weights.ptr, tsdf_vertices.ptr, collision_counts.ptr,
+           knn_counts.ptr, depth_frame.ptr, L.Int2(int(frame_dims[0]), int(frame_dims[1])), float(delta),
+           L.f3(voxel_size), L.f3(volume_size), L.i3(volume_dim), L.m3(volume_rotate), L.f3(volume_translate),
+           L.m3(depth_intrinsics), float(max_weight), _s(stream))
+    _done(stream)
+
+
+def run_integrate_tsdf_from_frame_kernel(tsdf: DeviceArray, tsdf_weights: DeviceArray, depth_frame: DeviceArray,
+                                         frame_dims, delta: float, dims, voxel_size, volume_size, volume_rotate,
+                                         volume_translate, depth_intrinsics, max_weight: float = 15.0,
+                                         stream: Stream | None = None) -> None:
+    """af::runIntegrateTSDFFromFrameKernel (include/af/TSDFIntegration.cuh:8-18)."""
+    L.call("vf_integrate_tsdf", tsdf.ptr, tsdf_weights.ptr, depth_frame.ptr,
+           L.Int2(int(frame_dims[0]), int(frame_dims[1])), float(delta), L.i3(dims), L.f3(voxel_size),
+           L.f3(volume_size), L.m3(volume_rotate), L.f3(volume_translate), L.m3(depth_intrinsics), float(max_weight),
+           _s(stream))
+    _done(stream)
+
+
+def run_integrate_tsdf_from_camera_frame_kernel(tsdf: DeviceArray, tsdf_weights: DeviceArray,
+                                                depth_frame: DeviceArray, frame_dims, delta: float, dims, voxel_size,
+                                                volume_size, world2camera_rotate, world2camera_translate,
+                                                volume_rotate, volume_translate, depth_intrinsics,
+                                                max_weight: float = 15.0, stream: Stream | None = None) -> None:
+    """af::runIntegrateTSDFFromCameraFrameKernel (include/af/TSDFIntegration.cuh:20-32)."""
+    L.call("vf_integrate_tsdf_camera_frame", tsdf.ptr, tsdf_weights.ptr, depth_frame.ptr,
+           L.Int2(int(frame_dims[0]), int(frame_dims[1])), float(delta), L.i3(dims), L.f3(voxel_size),
+           L.f3(volume_size), L.m3(world2camera_rotate), L.f3(world2camera_translate), L.m3(volume_rotate),
+           L.f3(volume_translate), L.m3(depth_intrinsics), float(max_weight), _s(stream))
+    _done(stream)
+
+
+def run_translate_kernel(mesh: DeviceArray, mesh_size: int, translation, stream: Stream | None = None) -> None:
+    """af::runTranslateKernel (include/af/VertexManipulation.cuh:16)."""
+    L.call("vf_translate", mesh.ptr, int(mesh_size), L.f3(translation), _s(stream))
+    _done(stream)
+
+
+# ---------------------------------------------------------------------------
+# Reference-layout device containers
+# ---------------------------------------------------------------------------
+
+class DeviceTsdf:
+    """af::DeviceTsdf<float> (include/af/DeviceTsdf.cuh:13-84): tsdf initialised
+    to -1, weights to 0, voxel size = size / dims."""
+
+    def __init__(self, vol_size, vol_dim, center=(0.0, 0.0, 0.0), rotation=IDENTITY3, delta: float = 0.01,
+                 stream: Stream | None = None):
+        self.vol_dim = tuple(int(d) for d in vol_dim)
+        self.vol_size = tuple(float(np.float32(s)) for s in vol_size)
+        self.center = tuple(float(np.float32(c)) for c in center)
+        self.rotation = tuple(float(np.float32(r)) for r in rotation)
+        self.delta = float(delta)
+        n = self.voxel_count()
+        self.tsdf = DeviceArray((n,), np.float32, stream)
+        self.weights = DeviceArray((n,), np.float32, stream)
+        self.init_default()
+
+    def voxel_count(self) -> int:
+        return self.vol_dim[0] * self.vol_dim[1] * self.vol_dim[2]
+
+    def voxel_size(self):
+        """DeviceTsdf::voxelSize(): volSize.cwiseQuotient(volDim.cast<float>()) in float32."""
+        return tuple(float(np.float32(s) / np.float32(d)) for s, d in zip(self.vol_size, self.vol_dim))
+
+    def init_default(self) -> None:
+        self.tsdf.fill(-1.0)
+        self.weights.fill(0.0)
+
+    def integrate(self, depth: DeviceArray, intrinsics, max_weight: float = 15.0,
+                  stream: Stream | None = None) -> None:
+        """DeviceTsdf::integrate (include/af/DeviceTsdf.cuh:28-32)."""
+        h, w = depth.shape
+        run_integrate_tsdf_from_frame_kernel(self.tsdf, self.weights, depth, (w, h), self.delta, self.vol_dim,
+                                             self.voxel_size(), self.vol_size, self.rotation, self.center,
+                                             intrinsics, max_weight, stream)
+
+
+class DeviceMotionGraph:
+    """af::DeviceMotionGraph (include/af/DeviceMotionGraph.cuh:10-31): node
+    positions Vec3f[G], transforms Mat4f[G] (column-major), radii float[G]."""
+
+    def __init__(self, points: np.ndarray, transforms: np.ndarray, radii: np.ndarray,
+                 stream: Stream | None = None):
+        points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        self.count = points.shape[0]
+        self.points = DeviceArray.from_numpy(points, stream)
+        self.transforms = DeviceArray.from_numpy(np.ascontiguousarray(transforms, np.float32).reshape(-1, 16), stream)
+        self.radii = DeviceArray.from_numpy(np.ascontiguousarray(radii, np.float32).reshape(-1), stream)
+
+
+# ---------------------------------------------------------------------------
+# Fused per-frame path
+# ---------------------------------------------------------------------------
+
+class Workspace:
+    """Device scratch of the fused path (vf_workspace); grows on demand."""
+
+    def __init__(self, max_volumes: int = 1, max_total_voxels: int = 0, max_total_nodes: int = 0,
+                 stream: Stream | None = None):
+        self.handle = C.c_void_p(None)
+        L.call("vf_workspace_create", C.byref(self.handle), int(max_volumes), int(max_total_voxels),
+               int(max_total_nodes), _s(stream))
+
+    def bytes(self) -> int:
+        return int(L.load().vf_workspace_bytes(self.handle))
+
+    def last_stats(self, stream: Stream | None = None) -> dict:
+        st = L.FrameStats()
+        L.call("vf_workspace_last_stats", self.handle, C.byref(st), _s(stream))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            L.load().vf_workspace_destroy(self.handle)
+            self.handle = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def voxels_per_node(motion_graph_radius: float, voxel_size) -> tuple:
+    """Node cube of src/VolumeFusion.cu:702-705: 2 * (int(R / vs) + 1) + 1."""
+    r = np.float32(motion_graph_radius)
+    out = []
+    for v in voxel_size:
+        q = np.float32(r / np.float32(v))
+        out.append(2 * (int(q) + 1) + 1)
+    return tuple(out)
+
+
+def make_volume(tsdf: DeviceTsdf) -> L.Volume:
+    v = L.Volume()
+    v.tsdf = tsdf.tsdf.ptr
+    v.weights = tsdf.weights.ptr
+    v.dim = L.i3(tsdf.vol_dim)
+    v.voxel_size = L.f3(tsdf.voxel_size())
+    v.size = L.f3(tsdf.vol_size)
+    v.center = L.f3(tsdf.center)
+    return v
+
+
+def make_graph(g: DeviceMotionGraph, write_back: bool = True) -> L.Graph:
+    gr = L.Graph()
+    gr.points = g.points.ptr
+    gr.points_out = g.points.ptr if write_back else None
+    gr.transforms = g.transforms.ptr
+    gr.radii = g.radii.ptr
+    gr.count = g.count
+    return gr
+
+
+def make_frame(depth: DeviceArray, intrinsics) -> L.Frame:
+    f = L.Frame()
+    f.depth = depth.ptr
+    f.height, f.width = int(depth.shape[0]), int(depth.shape[1])
+    f.intrinsics = L.m3(intrinsics)
+    return f
+
+
+def make_params(k: int, delta: float, max_weight: float, cube) -> L.WarpParams:
+    p = L.WarpParams()
+    p.k = int(k)
+    p.delta = float(delta)
+    p.max_weight = float(max_weight)
+    p.voxels_per_node = L.i3(cube)
+    return p
+
+
+def warped_integrate(ws: Workspace, tsdf: DeviceTsdf, graph: DeviceMotionGraph, depth: DeviceArray, intrinsics,
+                     k: int, delta: float, max_weight: float, cube, write_back: bool = True,
+                     stream: Stream | None = None, sync: bool = True) -> None:
+    """The reference's per-frame integration block (src/VolumeFusion.cu:702-742)
+    as one call: volume centre = tsdf.center, rotation I."""
+    vol = make_volume(tsdf)
+    gr = make_graph(graph, write_back)
+    fr = make_frame(depth, intrinsics)
+    p = make_params(k, delta, max_weight, cube)
+    L.call("vf_warped_integrate", ws.handle, C.byref(vol), C.byref(gr), C.byref(fr), C.byref(p), _s(stream))
+    if sync:
+        _done(stream)
+
+
+def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list, k: int, delta: float,
+                           max_weight: float, cubes, write_back: bool = True, stream: Stream | None = None,
+                           sync: bool = True) -> None:
+    """Many independent volumes, one frame each, in one set of launches."""
+    n = len(tsdfs)
+    vols = (L.Volume * n)(*[make_volume(t) for t in tsdfs])
+    grs = (L.Graph * n)(*[make_graph(g, write_back) for g in graphs])
+    frs = (L.Frame * n)(*[make_frame(d, K) for d, K in zip(depths, intrinsics_list)])
+    deltas = delta if isinstance(delta, (list, tuple)) else [delta] * n
+    maxws = max_weight if isinstance(max_weight, (list, tuple)) else [max_weight] * n
+    ps = (L.WarpParams * n)(*[make_params(k, d, m, c) for d, m, c in zip(deltas, maxws, cubes)])
+    L.call("vf_warped_integrate_batch", ws.handle, n, vols, grs, frs, ps, _s(stream))
+    if sync:
+        _done(stream)
