@@ -201,6 +201,17 @@ int vf_warped_integrate_batch(vf_workspace* ws, int n, const vf_volume* volumes,
 /* Statistics of the last call on this workspace (blocks on `stream`). */
 int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream stream);
 
+/* The same counters summed over every call since the last reset (blocks on
+ * `stream`); reset != 0 zeroes them afterwards. */
+int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream);
+
+/* Per-kernel timing: when enabled, each call records HIP events on its stream
+ * around its three kernels.  vf_workspace_kernel_ms waits for the last one,
+ * returns the summed milliseconds of k_prep_bin, k_pass_a and k_pass_b over
+ * the timed calls since enabling (or the previous query) and starts over. */
+int vf_workspace_set_timing(vf_workspace* ws, int enable);
+int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames);
+
 /* ------------------------------------------------------------------------ */
 /* Device plumbing for hosts without a GPU framework (Python via ctypes)    */
 /* ------------------------------------------------------------------------ */

@@ -117,6 +117,9 @@ _SIGS = {
     "vf_warped_integrate_batch": [vp, C.c_int, C.POINTER(Volume), C.POINTER(Graph), C.POINTER(Frame),
                                   C.POINTER(WarpParams), vp],
     "vf_workspace_last_stats": [vp, C.POINTER(FrameStats), vp],
+    "vf_workspace_totals": [vp, C.POINTER(FrameStats), C.c_int, vp],
+    "vf_workspace_set_timing": [vp, C.c_int],
+    "vf_workspace_kernel_ms": [vp, C.POINTER(f32), C.POINTER(C.c_int)],
     "vf_device_count": [C.POINTER(C.c_int)],
     "vf_set_device": [C.c_int],
     "vf_malloc": [C.POINTER(vp), u64],

@@ -75,7 +75,8 @@ struct Ctrl {
     uint32_t touched_count[2];
     uint32_t overflow_bricks;
     uint32_t pad[2];
-    unsigned long long stats[8];  // supported, in_bounds, pending, collided, updated
+    unsigned long long stats[8];   // this frame: supported, in_bounds, pending, collided, updated, active bricks
+    unsigned long long totals[8];  // since the last reset: the same counters summed over frames
 };
 
 struct KnnOut {
@@ -417,6 +418,9 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
         atomicAdd(&ctrl->stats[0], s_stat[0]);
         atomicAdd(&ctrl->stats[1], s_stat[1]);
         atomicAdd(&ctrl->stats[2], s_stat[2]);
+        atomicAdd(&ctrl->totals[0], s_stat[0]);
+        atomicAdd(&ctrl->totals[1], s_stat[1]);
+        atomicAdd(&ctrl->totals[2], s_stat[2]);
     }
 }
 
@@ -435,6 +439,7 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
     const uint32_t nthreads = gridDim.x * blockDim.x;
     if (gid == 0) {
         ctrl->stats[5] = ctrl->active_count;
+        ctrl->totals[5] += ctrl->active_count;
         ctrl->active_count = 0u;
         ctrl->work_next = 0u;
     }
@@ -475,8 +480,14 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
         updated += __shfl_down(updated, off);
     }
     if (__lane_id() == 0) {
-        if (collided) atomicAdd(&ctrl->stats[3], collided);
-        if (updated) atomicAdd(&ctrl->stats[4], updated);
+        if (collided) {
+            atomicAdd(&ctrl->stats[3], collided);
+            atomicAdd(&ctrl->totals[3], collided);
+        }
+        if (updated) {
+            atomicAdd(&ctrl->stats[4], updated);
+            atomicAdd(&ctrl->totals[4], updated);
+        }
     }
 }
 
@@ -512,6 +523,11 @@ struct vf_workspace {
     uint64_t frame = 0;  // selects the touched-tile list (frame parity)
     std::vector<vf::VolDesc> vols_h;
     std::vector<vf::VolPtr> vptr_h;
+    // optional per-kernel timing: 4 events per frame (before k_prep_bin, after
+    // each of the three kernels), recycled every kMaxTimed frames
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    int timed_frames = 0;
 };
 
 namespace vf {
@@ -692,12 +708,25 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
 
     const uint32_t parity = (uint32_t)(ws->frame & 1u);
     const uint32_t prev = parity ^ 1u;
+    hipEvent_t* ev = nullptr;
+    if (ws->timing && !knn_only) {
+        constexpr int kMaxTimed = 4096;
+        if (ws->events.empty()) {
+            ws->events.resize(4 * kMaxTimed);
+            for (auto& e : ws->events) VF_HIP(hipEventCreate(&e));
+        }
+        if (ws->timed_frames >= kMaxTimed) return fail(VF_ERR_CAPACITY, "more than %d timed frames", kMaxTimed);
+        ev = &ws->events[4 * ws->timed_frames];
+        ws->timed_frames++;
+        VF_HIP(hipEventRecord(ev[0], stream));
+    }
     {
         uint64_t blocks = std::max<uint64_t>((k1 + 255) / 256, 1024);
         hipLaunchKernelGGL(k_prep_bin, dim3((unsigned)blocks), dim3(256), 0, stream, B.vols, nvol, (uint32_t)k1, B.nrec,
                            B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
                            B.touched + (uint64_t)prev * B.tile_cap, prev);
         VF_LAUNCH_CHECK("k_prep_bin");
+        if (ev) VF_HIP(hipEventRecord(ev[1], stream));
     }
     int dev = 0, cus = 0;
     VF_HIP(hipGetDevice(&dev));
@@ -743,6 +772,7 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         }
 #undef VF_PASS_A
         VF_LAUNCH_CHECK("k_pass_a");
+        if (ev) VF_HIP(hipEventRecord(ev[2], stream));
     }
     if (knn_only) {
         hipLaunchKernelGGL(k_reset_after_knn, dim3(1), dim3(1), 0, stream, B.ctrl);
@@ -752,6 +782,7 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         hipLaunchKernelGGL(k_pass_b, dim3((unsigned)blocks), dim3(256), 0, stream, B.vptr, B.records, B.ctrl, B.bitmap,
                            vbits, B.vols, nvol, B.nrec);
         VF_LAUNCH_CHECK("k_pass_b");
+        if (ev) VF_HIP(hipEventRecord(ev[3], stream));
         ws->frame++;
     }
     return VF_OK;
@@ -791,6 +822,7 @@ int vf_workspace_create(vf_workspace** out, int max_volumes, uint64_t max_total_
 
 int vf_workspace_destroy(vf_workspace* ws) {
     if (!ws) return VF_OK;
+    for (auto& e : ws->events) (void)hipEventDestroy(e);
     free_buffers(ws->buf);
     delete ws;
     return VF_OK;
@@ -828,6 +860,49 @@ int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream s
     stats->updated = c.stats[4];
     stats->active_bricks = c.stats[5];
     stats->overflow_bricks = c.overflow_bricks;
+    return VF_OK;
+}
+
+int vf_workspace_set_timing(vf_workspace* ws, int enable) {
+    if (!ws) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_set_timing: NULL workspace");
+    ws->timing = enable != 0;
+    ws->timed_frames = 0;
+    return VF_OK;
+}
+
+int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
+    if (!ws || !ms3 || !frames) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_kernel_ms: NULL argument");
+    ms3[0] = ms3[1] = ms3[2] = 0.0f;
+    *frames = ws->timed_frames;
+    if (ws->timed_frames) VF_HIP(hipEventSynchronize(ws->events[4 * ws->timed_frames - 1]));
+    for (int f = 0; f < ws->timed_frames; ++f) {
+        for (int k = 0; k < 3; ++k) {
+            float t = 0.0f;
+            VF_HIP(hipEventElapsedTime(&t, ws->events[4 * f + k], ws->events[4 * f + k + 1]));
+            ms3[k] += t;
+        }
+    }
+    ws->timed_frames = 0;
+    return VF_OK;
+}
+
+int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream) {
+    if (!ws || !totals) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_totals: NULL argument");
+    memset(totals, 0, sizeof(*totals));
+    if (!ws->buf.ctrl) return VF_OK;
+    Ctrl c;
+    VF_HIP(hipMemcpyAsync(&c, ws->buf.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    VF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    totals->supported = c.totals[0];
+    totals->in_bounds = c.totals[1];
+    totals->pending = c.totals[2];
+    totals->collided = c.totals[3];
+    totals->updated = c.totals[4];
+    totals->active_bricks = c.totals[5];
+    if (reset) {
+        VF_HIP(hipMemsetAsync(&ws->buf.ctrl->totals, 0, sizeof(c.totals), (hipStream_t)stream));
+        VF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    }
     return VF_OK;
 }
 

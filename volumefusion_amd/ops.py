@@ -203,6 +203,21 @@ class Workspace:
         L.call("vf_workspace_last_stats", self.handle, C.byref(st), _s(stream))
         return st.as_dict()
 
+    def totals(self, reset: bool = False, stream: Stream | None = None) -> dict:
+        st = L.FrameStats()
+        L.call("vf_workspace_totals", self.handle, C.byref(st), 1 if reset else 0, _s(stream))
+        return st.as_dict()
+
+    def set_timing(self, enable: bool) -> None:
+        L.call("vf_workspace_set_timing", self.handle, 1 if enable else 0)
+
+    def kernel_ms(self) -> tuple:
+        """(k_prep_bin, k_pass_a, k_pass_b) summed milliseconds, frames timed."""
+        ms = (C.c_float * 3)()
+        n = C.c_int(0)
+        L.call("vf_workspace_kernel_ms", self.handle, ms, C.byref(n))
+        return (float(ms[0]), float(ms[1]), float(ms[2])), int(n.value)
+
     def close(self) -> None:
         if self.handle:
             L.load().vf_workspace_destroy(self.handle)

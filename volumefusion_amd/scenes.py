@@ -166,9 +166,10 @@ def poisson_disk(surfaces: list[Surface], n: int, rng: np.random.Generator, rmin
     if rmin is None:
         rmin = 0.8 * math.sqrt(area / max(n, 1))
     weights = np.array([s.area() for s in surfaces]) / area
-    pts = np.zeros((0, 3))
+    pts: list = []
+    grid: dict = {}
     stale = 0
-    while pts.shape[0] < n:
+    while len(pts) < n:
         which = rng.choice(len(surfaces), size=4096, p=weights)
         u = rng.normal(size=(4096, 3))
         u /= np.linalg.norm(u, axis=1, keepdims=True)
@@ -177,21 +178,42 @@ def poisson_disk(surfaces: list[Surface], n: int, rng: np.random.Generator, rmin
             m = which == i
             if m.any():
                 cand[m] = s.point(u[m])
-        before = pts.shape[0]
+        before = len(pts)
         r2 = rmin * rmin
-        for c in cand:
-            if pts.shape[0] >= n:
+        cells = np.floor(cand / rmin).astype(np.int64)
+        for c, (cx, cy, cz) in zip(cand, cells.tolist()):
+            if len(pts) >= n:
                 break
-            if pts.shape[0] == 0 or np.min(np.sum((pts - c) ** 2, axis=1)) >= r2:
-                pts = np.vstack([pts, c])
-        if pts.shape[0] == before:
+            ok = True
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        for q in grid.get((cx + dx, cy + dy, cz + dz), ()):
+                            d = c - q
+                            if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pts.append(c)
+                grid.setdefault((cx, cy, cz), []).append(c)
+        if len(pts) == before:
             stale += 1
             if stale >= 30:
                 rmin *= 0.9
+                r2 = rmin * rmin
                 stale = 0
+                grid = {}
+                for q in pts:
+                    grid.setdefault(tuple(np.floor(q / rmin).astype(np.int64).tolist()), []).append(q)
         else:
             stale = 0
-    return pts[:n].astype(np.float32)
+    return np.array(pts[:n], np.float32)
 
 
 def rodrigues(w: np.ndarray) -> np.ndarray:
