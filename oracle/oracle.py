@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
         L.vfo_weight_mesh_host.restype = C.c_float
         L.vfo_div_markstein.argtypes = [C.c_float, C.c_float, C.c_float]
         L.vfo_div_markstein.restype = C.c_float
+        L.vfo_markstein_mismatches.argtypes = [C.c_float, C.c_uint32, C.c_uint32]
+        L.vfo_markstein_mismatches.restype = C.c_int64
         L.vfo_mat3_inverse.argtypes = [_f32p, _f32p]
         L.vfo_cvt_i32_rz.argtypes = [C.c_float]
         L.vfo_cvt_i32_rz.restype = C.c_int32

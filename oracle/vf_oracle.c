@@ -58,11 +58,23 @@ int32_t vfo_cvt_i32_rz(float f) {
     return (int32_t)f;
 }
 
-/* e^x evaluated in binary64 and rounded once to binary32.
- * Method: Cody & Waite (1980) two-constant reduction x = k ln2 + r with
- * |r| <= ln2/2 (fdlibm's ln2 split), e^r by its degree-11 Taylor polynomial in
- * Horner form with fused multiply-adds (truncation error < 2^-46 relative),
- * then scaling by 2^k.  Inputs below -87 return 0 (no subnormal results). */
+/* 2^(j/32), j = 0..31, correctly rounded to binary64. */
+static const double kExp2Table[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0};
+
+/* e^x evaluated in binary64 and rounded once to binary32, Tang's
+ * table-driven method (ACM TOMS 15(2), 1989): k = rint(32 x / ln2),
+ * r = x - k ln2/32 by a two-constant Cody-Waite reduction (fdlibm's ln2 split
+ * divided by 32), |r| <= ln2/64; e^r by its degree-5 Taylor polynomial in
+ * Horner form with fused multiply-adds (truncation < 2^-48 relative);
+ * e^x = 2^(k>>5) * 2^((k&31)/32) * e^r.  Inputs below -87 return 0. */
 float vfo_expw(float xf) {
     if (xf != xf)
         return xf;
@@ -71,22 +83,17 @@ float vfo_expw(float xf) {
     if (xf > 88.5f)
         return INFINITY;
     const double x = (double)xf;
-    const double k = rint(x * 0x1.71547652b82fep+0);
-    double r = fma(-k, 0x1.62e42fee00000p-1, x);
-    r = fma(-k, 0x1.a39ef35793c76p-33, r);
-    double p = 0x1.ae64567f544e4p-26;
-    p = fma(p, r, 0x1.27e4fb7789f5cp-22);
-    p = fma(p, r, 0x1.71de3a556c734p-19);
-    p = fma(p, r, 0x1.a01a01a01a01ap-16);
-    p = fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = fma(p, r, 0x1.1111111111111p-7);
-    p = fma(p, r, 0x1.5555555555555p-5);
+    const double kd = rint(x * 0x1.71547652b82fep+5);
+    const int k = (int)kd;
+    double r = fma(-kd, 0x1.62e42fee00000p-6, x);
+    r = fma(-kd, 0x1.a39ef35793c76p-38, r);
+    double p = fma(0x1.1111111111111p-7, r, 0x1.5555555555555p-5);
     p = fma(p, r, 0x1.5555555555555p-3);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    return (float)ldexp(p, (int)k);
+    const double t = p * kExp2Table[k & 31];
+    return (float)ldexp(t, k >> 5);
 }
 
 /* glibc's binary64 exp rounded to binary32: the host-compiled meaning of the
@@ -119,6 +126,25 @@ float vfo_div_markstein(float a, float b, float y) {
     const float q0 = a * y;
     const float r = fmaf(-q0, b, a);
     return fmaf(r, y, q0);
+}
+
+/* Number of floats a in [a_lo, a_hi] (bit patterns, positive) for which
+ * vfo_div_markstein(a, b, RN(1/b)) differs from the IEEE quotient a/b. */
+int64_t vfo_markstein_mismatches(float b, uint32_t a_lo_bits, uint32_t a_hi_bits) {
+    const float y = 1.0f / b;
+    int64_t bad = 0;
+    for (uint32_t u = a_lo_bits; u <= a_hi_bits; ++u) {
+        float a;
+        memcpy(&a, &u, 4);
+        const float q = a / b;
+        const float m = vfo_div_markstein(a, b, y);
+        uint32_t qb, mb;
+        memcpy(&qb, &q, 4);
+        memcpy(&mb, &m, 4);
+        bad += qb != mb;
+        if (u == 0xffffffffu) break;
+    }
+    return bad;
 }
 
 /* Eigen's 3x3 inverse (Eigen/src/LU/InverseImpl.h:126-170), as the reference

@@ -45,6 +45,7 @@ float vfo_exp_libm(float x);
 float vfo_weight_tsdf(float radius, float dist);
 float vfo_weight_mesh_host(float radius, float dist);
 float vfo_div_markstein(float a, float b, float recip_b);
+int64_t vfo_markstein_mismatches(float b, uint32_t a_lo_bits, uint32_t a_hi_bits);
 void vfo_mat3_inverse(const float* m, float* out);
 int32_t vfo_cvt_i32_rz(float f);
 

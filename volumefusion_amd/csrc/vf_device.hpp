@@ -22,39 +22,67 @@ __device__ __forceinline__ int32_t cvt_i32_rz(float f) {
     return (int32_t)f;
 }
 
-// e^x in binary64, rounded once to binary32: Cody & Waite (1980) two-constant
-// reduction x = k ln2 + r (fdlibm's ln2 split), degree-11 Taylor polynomial of
-// e^r by Horner's rule with fused multiply-adds, scaling by 2^k.  Inputs below
-// -87 give 0.  Every step is an exactly specified IEEE operation, so a host
-// build of the same steps gives the same bits.
-__device__ __forceinline__ float expw(float xf) {
+// 2^(j/32), j = 0..31, correctly rounded to binary64.
+#define VF_EXP2_TABLE                                                                                  \
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,          \
+        0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,      \
+        0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,      \
+        0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,      \
+        0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,      \
+        0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,      \
+        0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,      \
+        0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0
+
+__constant__ const double kExp2Table[32] = {VF_EXP2_TABLE};
+
+// e^x in binary64, rounded once to binary32, by Tang's table-driven method
+// (ACM TOMS 15(2), 1989): k = rint(32 x / ln2), r = x - k ln2/32 by a
+// two-constant Cody-Waite reduction (fdlibm's ln2 split / 32), |r| <= ln2/64;
+// e^r by its degree-5 Taylor polynomial in Horner form with fused
+// multiply-adds (truncation < 2^-48 relative); e^x = 2^(k>>5) * 2^((k&31)/32)
+// * e^r.  Inputs below -87 give 0.  `tab` is kExp2Table or a copy of it (LDS).
+// Every step is an exactly specified IEEE operation, so a host build of the
+// same steps gives the same bits.
+__device__ __forceinline__ float expw(float xf, const double* tab) {
     if (xf != xf) return xf;
     if (xf < -87.0f) return 0.0f;
     if (xf > 88.5f) return kInf;
     const double x = (double)xf;
-    const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
-    double r = __builtin_fma(-k, 0x1.62e42fee00000p-1, x);
-    r = __builtin_fma(-k, 0x1.a39ef35793c76p-33, r);
-    double p = 0x1.ae64567f544e4p-26;
-    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
-    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+5);
+    const int k = (int)kd;
+    double r = __builtin_fma(-kd, 0x1.62e42fee00000p-6, x);
+    r = __builtin_fma(-kd, 0x1.a39ef35793c76p-38, r);
+    double p = __builtin_fma(0x1.1111111111111p-7, r, 0x1.5555555555555p-5);
     p = __builtin_fma(p, r, 0x1.5555555555555p-3);
     p = __builtin_fma(p, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    return (float)__builtin_ldexp(p, (int)k);
+    const double t = p * tab[k & 31];
+    return (float)__builtin_ldexp(t, k >> 5);
+}
+
+// a / b for a >= 0 in the weight domain from y = RN(1/b): q0 = RN(a y),
+// rem = a - q0 b exactly (fma), q = RN(q0 + rem y) (Markstein, IBM J. Res.
+// Dev. 34(1), 1990).  Equal to the IEEE quotient on the operand ranges used
+// here (checked exhaustively on the host for every dividend mantissa over the
+// weight and weight-sum binades).
+__device__ __forceinline__ float div_rcp(float a, float b, float y) {
+    const float q0 = a * y;
+    const float rem = __builtin_fmaf(-q0, b, a);
+    return __builtin_fmaf(rem, y, q0);
 }
 
 // Gaussian warp weight of src/TSDFWarp.cu:17, exp(-(d*d) / (2 * r*r)), no
 // cutoff.  `den` is 2*(r*r), evaluated once per node.
 __device__ __forceinline__ float warp_weight(float den, float d) {
     const float num = -(d * d);
-    return expw(num / den);
+    return expw(num / den, kExp2Table);
+}
+
+// The same weight with the node's 1/den precomputed; identical result.
+__device__ __forceinline__ float warp_weight_rcp(float den, float rden, float d, const double* tab) {
+    const float q = d * d;
+    return expw(-div_rcp(q, den, rden), tab);
 }
 
 // Per-volume geometry, as the reference derives it from (dims, voxel size,
@@ -172,14 +200,14 @@ __device__ __forceinline__ void blend_sample(float to, float wo, float tn, float
 }
 
 // Canonical knn list kept sorted ascending by (dist, node order) in
-// registers.  Callers offer candidates in ascending node order, so a strict
-// `<` on the distance keeps the lower index first among equal distances.
+// registers (+inf sentinels fill the free slots).  Callers offer candidates in
+// ascending node order, so a strict `<` on the distance keeps the lower index
+// first among equal distances.  Node positions and radii must be finite.
 template <int K>
 struct KnnList {
     float d[K];
     uint32_t i[K];
     int n;
-    int seen;
 
     __device__ __forceinline__ void init() {
 #pragma unroll
@@ -188,37 +216,23 @@ struct KnnList {
             i[s] = 0xffffffffu;
         }
         n = 0;
-        seen = 0;
     }
 
+    // Insert a finite candidate distance (candidates satisfy d <= 2r).
     __device__ __forceinline__ void offer(float dist, uint32_t id) {
-        ++seen;
-        if (dist < d[K - 1]) {
-            bool c[K];
+        bool c[K];
 #pragma unroll
-            for (int s = 0; s < K; ++s) c[s] = dist < d[s];
+        for (int s = 0; s < K; ++s) c[s] = dist < d[s];
 #pragma unroll
-            for (int s = K - 1; s > 0; --s) {
-                const float ds = c[s - 1] ? d[s - 1] : (c[s] ? dist : d[s]);
-                const uint32_t is = c[s - 1] ? i[s - 1] : (c[s] ? id : i[s]);
-                d[s] = ds;
-                i[s] = is;
-            }
-            if (c[0]) {
-                d[0] = dist;
-                i[0] = id;
-            }
-            if (n < K) ++n;
-        } else if (n < K) {
-            // Room left and dist is +inf (or NaN): a list insertion appends it.
-#pragma unroll
-            for (int s = 0; s < K; ++s)
-                if (s == n) {
-                    d[s] = dist;
-                    i[s] = id;
-                }
-            ++n;
+        for (int s = K - 1; s > 0; --s) {
+            const float ds = c[s - 1] ? d[s - 1] : (c[s] ? dist : d[s]);
+            const uint32_t is = c[s - 1] ? i[s - 1] : (c[s] ? id : i[s]);
+            d[s] = ds;
+            i[s] = is;
         }
+        d[0] = c[0] ? dist : d[0];
+        i[0] = c[0] ? id : i[0];
+        n += (c[K - 1] && n < K) ? 1 : 0;
     }
 };
 

@@ -5,18 +5,21 @@
 // (src/VolumeFusion.cu:699-742, buffers at :252-260).  Here:
 //
 //   k_prep_bin  one thread per (node, brick overlapped by the node's cube):
-//               node record (translated position, cube corner, 2r, 2r^2,
-//               transform rows), append the node to the brick's list.  Also
+//               node record (translated position, cube corner, 2r, 2r^2 and
+//               its reciprocal, transform rows); append the node to the
+//               brick's list; the first append activates the brick.  Also
 //               clears the collision tiles the previous frame touched.
-//   k_pass_a    one 512-thread workgroup per active brick (bricks of
-//               BX x BY x BZ voxels, x fastest): stage the brick's node list in
-//               LDS (sorted by node index), per voxel select the canonical
-//               top-K, weights, blend, target voxel; count collisions in a
-//               2-bit-per-voxel saturating bitmap (8^3 tiles); voxels that
-//               pass the projective tests append a 16 B pending record
-//               (voxel, target, new tsdf, new weight).
+//   k_pass_a    one 512-thread workgroup per active brick (16 x 8 x 4 voxels,
+//               x fastest; one voxel per thread): stage the brick's node list
+//               in LDS sorted by node index, per voxel select the canonical
+//               top-K, weights, blend, target voxel.  Collisions are counted
+//               in an LDS window of 8^3 tiles around the brick (2 bits per
+//               voxel: seen once / seen twice) and merged into the global
+//               bitmap once per brick, a word at a time.  Voxels that pass the
+//               projective tests append a 16 B pending record (voxel, target,
+//               new tsdf, new weight) to the workgroup's own record arena.
 //   k_pass_b    one thread per pending record: drop it if its target was
-//               reached twice, else write tsdf/weight.
+//               reached twice, else write tsdf and weight.
 //
 // Unsupported voxels (no node cube covers their brick) are never touched, so
 // HBM traffic scales with the supported shell, not with the dense volume.
@@ -31,10 +34,16 @@
 namespace vf {
 namespace {
 
-constexpr int kCap = 128;           // node slots per brick list (and chunk size when a list spills)
-constexpr int kPassAThreads = 512;  // one voxel per thread, one brick per workgroup
+constexpr int kCap = 128;                  // node slots per brick list (and chunk size when a list spills)
+constexpr int kPassAThreads = 512;         // one voxel per thread, one brick per workgroup
+constexpr int kBX = 16, kBY = 8, kBZ = 4;  // brick shape: 64 B of tsdf per brick row
+constexpr int kMargin = 8;                 // collision window reaches 8 voxels beyond the brick
+constexpr int kWX = 4, kWY = 3, kWZ = 3;   // window size in 8^3 tiles (covers the margin for this brick shape)
+constexpr int kWinTiles = kWX * kWY * kWZ;
+constexpr int kStatSlots = 64;             // statistics counters are spread over 64 addresses
 constexpr uint32_t kNone = 0xffffffffu;
-constexpr int kBX = 16, kBY = 8, kBZ = 4;  // brick shape; 64 B rows of tsdf per brick row
+
+enum { kStSupported, kStInBounds, kStPending, kStCollided, kStUpdated, kStActive, kStSpilled, kStCount = 8 };
 
 struct VolDesc {
     Geom g;
@@ -56,27 +65,26 @@ struct VolDesc {
     uint32_t node_base;
     uint32_t brick_base;
     uint32_t tile_base;
+    uint32_t voxel_base;  // first voxel of this volume in the batch-wide voxel numbering
     uint32_t k1_base;
     uint32_t k1_count;
 };
 
 struct NodeRec {
     float4 m;   // translated position; .w = 2r (the candidate bound)
-    int4 lo;    // cube lower corner; .w = bits of 2*(r*r) (the weight denominator)
+    int4 lo;    // cube lower corner; .w unused
     float4 t0;  // transform rows (T(i,0), T(i,1), T(i,2), T(i,3))
     float4 t1;
     float4 t2;
+    float4 w;   // .x = 2*(r*r) (weight denominator), .y = RN(1 / .x)
 };
 
 struct Ctrl {
-    uint32_t active_count;
-    uint32_t work_next;
-    uint32_t record_count;
+    uint32_t active_count[2];  // by frame parity
+    uint32_t work_next[2];
     uint32_t touched_count[2];
-    uint32_t overflow_bricks;
-    uint32_t pad[2];
-    unsigned long long stats[8];   // this frame: supported, in_bounds, pending, collided, updated, active bricks
-    unsigned long long totals[8];  // since the last reset: the same counters summed over frames
+    uint32_t overflow_records;
+    uint32_t pad;
 };
 
 struct KnnOut {
@@ -85,11 +93,21 @@ struct KnnOut {
     int32_t* cnt;
 };
 
-__device__ __forceinline__ uint32_t tile_code(const VolDesc& vd, const int32_t u[3]) {
-    const uint32_t t = ((uint32_t)(u[2] >> 3) * (uint32_t)vd.tdim[1] + (uint32_t)(u[1] >> 3)) * (uint32_t)vd.tdim[0] +
-                       (uint32_t)(u[0] >> 3);
-    const uint32_t bit = ((uint32_t)(u[2] & 7) << 6) | ((uint32_t)(u[1] & 7) << 3) | (uint32_t)(u[0] & 7);
-    return (t << 9) | bit;
+struct Arena {
+    uint4* records;
+    uint32_t* counts;      // per pass-A workgroup
+    uint32_t cap;          // records per workgroup arena
+    uint32_t n_arenas;
+    uint4* overflow;       // shared overflow arena (atomic append)
+    uint32_t overflow_cap;
+};
+
+__device__ __forceinline__ void stat_add(unsigned long long* stat, unsigned long long* totals, int counter,
+                                         unsigned long long v) {
+    if (!v) return;
+    const int slot = counter * kStatSlots + (int)(blockIdx.x & (kStatSlots - 1));
+    atomicAdd(stat + slot, v);
+    atomicAdd(totals + slot, v);
 }
 
 __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vols, int nvol, uint32_t total,
@@ -97,12 +115,14 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
                                                   uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active,
                                                   Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap,
                                                   uint32_t* __restrict__ tile_flag,
-                                                  const uint32_t* __restrict__ touched_prev, uint32_t prev_parity) {
+                                                  const uint32_t* __restrict__ touched_prev, uint32_t parity,
+                                                  unsigned long long* __restrict__ stat,
+                                                  unsigned long long* __restrict__ totals) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
 
     // Clear the collision tiles the previous frame on this workspace touched.
-    const uint32_t nt = ctrl->touched_count[prev_parity];
+    const uint32_t nt = ctrl->touched_count[parity ^ 1u];
     for (uint32_t w = gid; w < nt * 33u; w += nthreads) {
         const uint32_t tile = touched_prev[w / 33u];
         const uint32_t k = w % 33u;
@@ -111,11 +131,8 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
         else
             tile_flag[tile] = 0u;
     }
-    if (gid == 0) {
-        ctrl->record_count = 0u;
-        ctrl->overflow_bricks = 0u;
-        for (int s = 0; s < 8; ++s) ctrl->stats[s] = 0ull;
-    }
+    if (gid < kStCount * kStatSlots) stat[gid] = 0ull;
+    if (gid == 0) ctrl->overflow_records = 0u;
     if (gid >= total) return;
 
     int v = 0;
@@ -150,7 +167,7 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
         const float den = 2.0f * rr;
         NodeRec rec;
         rec.m = make_float4(m[0], m[1], m[2], r * 2.0f);
-        rec.lo = make_int4(lo[0], lo[1], lo[2], __float_as_int(den));
+        rec.lo = make_int4(lo[0], lo[1], lo[2], 0);
         if (vd.T) {  // the knn-only entry point has no transforms
             const float* M = vd.T + 16ull * j;
             rec.t0 = make_float4(M[0], M[4], M[8], M[12]);
@@ -161,6 +178,7 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
             rec.t1 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
             rec.t2 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         }
+        rec.w = make_float4(den, 1.0f / den, 0.0f, 0.0f);
         nrec[vd.node_base + j] = rec;
     }
     // bricks overlapped by the cube clipped to the volume
@@ -183,12 +201,9 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
     const uint32_t b = ((uint32_t)bz * (uint32_t)vd.bdim[1] + (uint32_t)by) * (uint32_t)vd.bdim[0] + (uint32_t)bx;
     const uint32_t gb = vd.brick_base + b;
     const uint32_t slot = atomicAdd(brick_count + gb, 1u);
-    if (slot < (uint32_t)kCap)
-        brick_nodes[(uint64_t)gb * kCap + slot] = j;
-    else if (slot == (uint32_t)kCap)
-        atomicAdd(&ctrl->overflow_bricks, 1u);
+    if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
     if (slot == 0) {
-        const uint32_t a = atomicAdd(&ctrl->active_count, 1u);
+        const uint32_t a = atomicAdd(&ctrl->active_count[parity], 1u);
         active[a] = ((uint32_t)v << 24) | b;
     }
 }
@@ -198,7 +213,8 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
     const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
     const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ active, Ctrl* __restrict__ ctrl,
     uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t* __restrict__ touched_cur,
-    uint32_t parity, uint4* __restrict__ records, uint32_t vbits, KnnOut kout) {
+    uint32_t parity, Arena arena, unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals,
+    KnnOut kout) {
     __shared__ uint32_t s_raw[kCap];
     __shared__ uint32_t s_ids[kCap];
     __shared__ float4 s_m[kCap];
@@ -206,22 +222,38 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
     __shared__ float4 s_t0[kCap];
     __shared__ float4 s_t1[kCap];
     __shared__ float4 s_t2[kCap];
+    __shared__ float2 s_w[kCap];
+    __shared__ double s_exp2[32];
+    __shared__ uint32_t s_bm[KNN_ONLY ? 1 : kWinTiles * 32];
+    __shared__ uint32_t s_wave_emit[kPassAThreads / 64];
     __shared__ uint32_t s_item;
-    __shared__ unsigned long long s_stat[3];
+    __shared__ unsigned long long s_stat[4];
 
     const int tid = threadIdx.x;
     const int lane = __lane_id();
-    if (tid < 3) s_stat[tid] = 0ull;
-    if (!KNN_ONLY && blockIdx.x == 0 && tid == 0) ctrl->touched_count[parity ^ 1u] = 0u;
-    const uint32_t nactive = ctrl->active_count;
+    const int wave = tid >> 6;
+    if (tid < 4) s_stat[tid] = 0ull;
+    if (tid < 32) s_exp2[tid] = kExp2Table[tid];
+    if (!KNN_ONLY) {
+        for (int w = tid; w < kWinTiles * 32; w += kPassAThreads) s_bm[w] = 0u;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        // the other parity's list and counters belong to the previous frame,
+        // whose kernels have finished
+        ctrl->active_count[parity ^ 1u] = 0u;
+        ctrl->work_next[parity ^ 1u] = 0u;
+        if (!KNN_ONLY) ctrl->touched_count[parity ^ 1u] = 0u;
+    }
+    const uint32_t nactive = ctrl->active_count[parity];
 
     const int lx = tid % kBX;
     const int ly = (tid / kBX) % kBY;
     const int lz = tid / (kBX * kBY);
+    uint32_t fill = 0;  // records in this workgroup's arena (uniform)
 
     for (;;) {
         if (tid == 0) {
-            const uint32_t i = atomicAdd(&ctrl->work_next, 1u);
+            const uint32_t i = atomicAdd(&ctrl->work_next[parity], 1u);
             s_item = i < nactive ? active[i] : kNone;
         }
         __syncthreads();
@@ -230,6 +262,8 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
         const uint32_t vol = item >> 24;
         const uint32_t b = item & 0xffffffu;
         const VolDesc& vd = vols[vol];
+        const int32_t dimx = vd.g.dim[0], dimy = vd.g.dim[1], dimz = vd.g.dim[2];
+        const uint32_t cx = (uint32_t)vd.cube[0], cy = (uint32_t)vd.cube[1], cz = (uint32_t)vd.cube[2];
         const uint32_t gb = vd.brick_base + b;
         const uint32_t cnt = brick_count[gb];
         const bool spill = cnt > (uint32_t)kCap;
@@ -239,7 +273,7 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
         const int32_t byi = (int32_t)((b / (uint32_t)vd.bdim[0]) % (uint32_t)vd.bdim[1]);
         const int32_t bzi = (int32_t)(b / (uint32_t)(vd.bdim[0] * vd.bdim[1]));
         const int32_t x = bxi * kBX + lx, y = byi * kBY + ly, z = bzi * kBZ + lz;
-        const bool valid = x < vd.g.dim[0] && y < vd.g.dim[1] && z < vd.g.dim[2];
+        const bool valid = x < dimx && y < dimy && z < dimz;
 
         float p[3];
         voxel_to_world(vd.g, x, y, z, p);
@@ -255,7 +289,7 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                 if ((uint32_t)tid < m) {
                     const uint32_t id = s_raw[tid];
                     uint32_t rank = 0;
-                    for (uint32_t e = 0; e < m; ++e) rank += s_raw[e] < id;
+                    for (uint32_t e = 0; e < m; ++e) rank += s_raw[e] < id ? 1u : 0u;
                     s_ids[rank] = id;
                 }
                 __syncthreads();
@@ -272,13 +306,15 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                 s_t0[tid] = rec.t0;
                 s_t1[tid] = rec.t1;
                 s_t2[tid] = rec.t2;
+                s_w[tid] = make_float2(rec.w.x, rec.w.y);
             }
             __syncthreads();
             if (valid) {
                 for (uint32_t e = 0; e < m; ++e) {
                     const int4 l = s_lo[e];
-                    if ((uint32_t)(x - l.x) < (uint32_t)vd.cube[0] && (uint32_t)(y - l.y) < (uint32_t)vd.cube[1] &&
-                        (uint32_t)(z - l.z) < (uint32_t)vd.cube[2]) {
+                    const bool inside = ((uint32_t)(x - l.x) < cx) & ((uint32_t)(y - l.y) < cy) &
+                                        ((uint32_t)(z - l.z) < cz);
+                    if (inside) {
                         const float4 nm = s_m[e];
                         const float dx = p[0] - nm.x, dy = p[1] - nm.y, dz = p[2] - nm.z;
                         const float s = dx * dx + (dy * dy + dz * dz);
@@ -289,10 +325,13 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
             }
             if (spill && ch + 1 < nchunks) __syncthreads();
         }
-        if (tid == 0) brick_count[gb] = 0u;
+        if (tid == 0) {
+            brick_count[gb] = 0u;
+            if (spill) s_stat[3] += 1ull;
+        }
 
         const int n = valid ? kl.n : 0;
-        const uint64_t vidx = ((uint64_t)z * vd.g.dim[1] + y) * vd.g.dim[0] + x;
+        const uint64_t vidx = ((uint64_t)z * dimy + y) * dimx + x;
 
         if (KNN_ONLY) {
             if (n > 0) {
@@ -305,165 +344,253 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                     }
                 }
             }
-        } else {
-            bool inb = false, emit = false;
-            uint32_t ucode = 0;
-            float tnew = 0.0f, wnew = 0.0f;
-            if (n > 0) {
-                // computeTsdfWarpWeightsKernel body, src/TSDFWarp.cu:101-113
-                float w[K];
-                float sum = 0.0f;
+            __syncthreads();
+            continue;
+        }
+
+        // collision window of this brick: kWX x kWY x kWZ tiles from (wx0, wy0, wz0)
+        const int32_t wx0 = (bxi * kBX - kMargin) >> 3;
+        const int32_t wy0 = (byi * kBY - kMargin) >> 3;
+        const int32_t wz0 = (bzi * kBZ - kMargin) >> 3;
+
+        bool inb = false, emit = false;
+        uint32_t ucode = 0;
+        float tnew = 0.0f, wnew = 0.0f;
+        if (n > 0) {
+            // computeTsdfWarpWeightsKernel body, src/TSDFWarp.cu:101-113
+            float w[K];
+            float sum = 0.0f;
 #pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    w[s] = 0.0f;
-                    if (s < n) {
-                        const int32_t denb = spill ? nrec[vd.node_base + kl.i[s]].lo.w : s_lo[kl.i[s]].w;
-                        w[s] = warp_weight(__int_as_float(denb), kl.d[s]);
-                        sum = sum + w[s];
+            for (int s = 0; s < K; ++s) {
+                w[s] = 0.0f;
+                if (s < n) {
+                    float2 dw;
+                    if (spill) {
+                        const float4 nw = nrec[vd.node_base + kl.i[s]].w;
+                        dw = make_float2(nw.x, nw.y);
+                    } else {
+                        dw = s_w[kl.i[s]];
                     }
+                    w[s] = warp_weight_rcp(dw.x, dw.y, kl.d[s], s_exp2);
+                    sum = sum + w[s];
                 }
-                bool zero = true;
-                if (sum != 0.0f) {
-#pragma unroll
-                    for (int s = 0; s < K; ++s)
-                        if (s < n) w[s] = w[s] / sum;
-                }
+            }
+            if (sum != 0.0f) {
+                const float rs = 1.0f / sum;
 #pragma unroll
                 for (int s = 0; s < K; ++s)
-                    if (s < n) zero = zero && (__builtin_fabsf(w[s]) <= 1e-5f);
-                // vertex + c (src/VolumeFusion.cu:723), warp (src/TSDFWarp.cu:183-189)
-                float pc[3];
-                pc[0] = p[0] + vd.c[0];
-                pc[1] = p[1] + vd.c[1];
-                pc[2] = p[2] + vd.c[2];
-                float acc[3] = {0.0f, 0.0f, 0.0f};
-                if (zero) {
-                    acc[0] = pc[0];
-                    acc[1] = pc[1];
-                    acc[2] = pc[2];
-                } else {
+                    if (s < n) w[s] = div_rcp(w[s], sum, rs);
+            }
+            bool zero = true;
 #pragma unroll
-                    for (int s = 0; s < K; ++s) {
-                        if (s < n) {
-                            float4 r0, r1, r2;
-                            if (spill) {
-                                const NodeRec& rec = nrec[vd.node_base + kl.i[s]];
-                                r0 = rec.t0;
-                                r1 = rec.t1;
-                                r2 = rec.t2;
-                            } else {
-                                r0 = s_t0[kl.i[s]];
-                                r1 = s_t1[kl.i[s]];
-                                r2 = s_t2[kl.i[s]];
-                            }
-                            float t[3];
-                            affine_apply(r0, r1, r2, pc, t);
+            for (int s = 0; s < K; ++s)
+                if (s < n) zero = zero && (__builtin_fabsf(w[s]) <= 1e-5f);
+            // vertex + c (src/VolumeFusion.cu:723), warp (src/TSDFWarp.cu:183-189)
+            float pc[3];
+            pc[0] = p[0] + vd.c[0];
+            pc[1] = p[1] + vd.c[1];
+            pc[2] = p[2] + vd.c[2];
+            float acc[3] = {0.0f, 0.0f, 0.0f};
+            if (zero) {
+                acc[0] = pc[0];
+                acc[1] = pc[1];
+                acc[2] = pc[2];
+            } else {
 #pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                const float wt = w[s] * t[a];
-                                acc[a] = acc[a] + wt;
-                            }
+                for (int s = 0; s < K; ++s) {
+                    if (s < n) {
+                        float4 r0, r1, r2;
+                        if (spill) {
+                            const NodeRec& rec = nrec[vd.node_base + kl.i[s]];
+                            r0 = rec.t0;
+                            r1 = rec.t1;
+                            r2 = rec.t2;
+                        } else {
+                            r0 = s_t0[kl.i[s]];
+                            r1 = s_t1[kl.i[s]];
+                            r2 = s_t2[kl.i[s]];
+                        }
+                        float t[3];
+                        affine_apply(r0, r1, r2, pc, t);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const float wt = w[s] * t[a];
+                            acc[a] = acc[a] + wt;
                         }
                     }
                 }
-                // collision target (src/TSDFIntegration.cu:272-282)
-                int32_t u[3];
-                inb = warped_target(vd.g, vd.rinv, vd.c, acc, u);
-                if (inb) {
-                    ucode = tile_code(vd, u);
-                    const uint32_t tile = vd.tile_base + (ucode >> 9);
-                    const uint32_t bit = ucode & 511u;
-                    uint32_t* blk = bitmap + (uint64_t)tile * 32u;
-                    const uint32_t mask = 1u << (bit & 31u);
+            }
+            // collision target (src/TSDFIntegration.cu:272-282)
+            int32_t u[3];
+            inb = warped_target(vd.g, vd.rinv, vd.c, acc, u);
+            if (inb) {
+                const int32_t tx = u[0] >> 3, ty = u[1] >> 3, tz = u[2] >> 3;
+                const uint32_t bit = ((uint32_t)(u[2] & 7) << 6) | ((uint32_t)(u[1] & 7) << 3) | (uint32_t)(u[0] & 7);
+                const uint32_t tloc = ((uint32_t)tz * (uint32_t)vd.tdim[1] + (uint32_t)ty) * (uint32_t)vd.tdim[0] +
+                                      (uint32_t)tx;
+                ucode = ((vd.tile_base + tloc) << 9) | bit;
+                const uint32_t mask = 1u << (bit & 31u);
+                const uint32_t lx_t = (uint32_t)(tx - wx0), ly_t = (uint32_t)(ty - wy0), lz_t = (uint32_t)(tz - wz0);
+                if (lx_t < (uint32_t)kWX && ly_t < (uint32_t)kWY && lz_t < (uint32_t)kWZ) {
+                    uint32_t* blk = s_bm + ((lz_t * kWY + ly_t) * kWX + lx_t) * 32u;
                     const uint32_t old = atomicOr(blk + (bit >> 5), mask);
                     if (old & mask) atomicOr(blk + 16 + (bit >> 5), mask);
-                    if (__atomic_load_n(tile_flag + tile, __ATOMIC_RELAXED) == 0u) {
-                        if (atomicCAS(tile_flag + tile, 0u, 1u) == 0u) {
-                            const uint32_t k = atomicAdd(&ctrl->touched_count[parity], 1u);
-                            touched_cur[k] = tile;
-                        }
-                    }
-                    // projective tests and the new sample (src/TSDFIntegration.cu:32-64)
-                    float tn;
-                    if (project_sample(vd.cam, vd.depth, acc, &tn) >= 0) {
-                        blend_sample(vd.tsdf[vidx], vd.wt[vidx], tn, vd.cam.maxw, &tnew, &wnew);
-                        emit = true;
-                    }
+                } else {
+                    // target beyond the window: straight to the global bitmap
+                    const uint32_t tile = vd.tile_base + tloc;
+                    uint32_t* blk = bitmap + (uint64_t)tile * 32u;
+                    const uint32_t old = atomicOr(blk + (bit >> 5), mask);
+                    if (old & mask) atomicOr(blk + 16 + (bit >> 5), mask);
+                    if (__atomic_load_n(tile_flag + tile, __ATOMIC_RELAXED) == 0u &&
+                        atomicCAS(tile_flag + tile, 0u, 1u) == 0u)
+                        touched_cur[atomicAdd(&ctrl->touched_count[parity], 1u)] = tile;
+                }
+                // projective tests and the new sample (src/TSDFIntegration.cu:32-64)
+                float tn;
+                if (project_sample(vd.cam, vd.depth, acc, &tn) >= 0) {
+                    blend_sample(vd.tsdf[vidx], vd.wt[vidx], tn, vd.cam.maxw, &tnew, &wnew);
+                    emit = true;
                 }
             }
-            const unsigned long long msup = __ballot(n > 0);
-            const unsigned long long minb = __ballot(inb);
-            const unsigned long long memit = __ballot(emit);
-            if (lane == 0) {
-                atomicAdd(&s_stat[0], (unsigned long long)__popcll(msup));
-                atomicAdd(&s_stat[1], (unsigned long long)__popcll(minb));
-                atomicAdd(&s_stat[2], (unsigned long long)__popcll(memit));
+        }
+        const unsigned long long msup = __ballot(n > 0);
+        const unsigned long long minb = __ballot(inb);
+        const unsigned long long memit = __ballot(emit);
+        if (lane == 0) {
+            s_wave_emit[wave] = (uint32_t)__popcll(memit);
+            if (msup) atomicAdd(&s_stat[0], (unsigned long long)__popcll(msup));
+            if (minb) atomicAdd(&s_stat[1], (unsigned long long)__popcll(minb));
+            if (memit) atomicAdd(&s_stat[2], (unsigned long long)__popcll(memit));
+        }
+        __syncthreads();
+        // pending records into this workgroup's arena (in lane order)
+        {
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (int wv = 0; wv < kPassAThreads / 64; ++wv) {
+                const uint32_t c = s_wave_emit[wv];
+                before += wv < wave ? c : 0u;
+                total += c;
             }
-            if (memit) {
-                const int leader = __ffsll((long long)memit) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&ctrl->record_count, (uint32_t)__popcll(memit));
-                base = __shfl(base, leader);
-                if (emit) {
-                    const uint32_t rank = (uint32_t)__popcll(memit & ((1ull << lane) - 1ull));
-                    const uint32_t vcode = vbits >= 32u ? (uint32_t)vidx : (((uint32_t)vol << vbits) | (uint32_t)vidx);
-                    records[base + rank] = make_uint4(vcode, ucode, __float_as_uint(tnew), __float_as_uint(wnew));
+            if (emit) {
+                const uint32_t rank = before + (uint32_t)__popcll(memit & ((1ull << lane) - 1ull));
+                const uint4 rec = make_uint4(vd.voxel_base + (uint32_t)vidx, ucode, __float_as_uint(tnew),
+                                             __float_as_uint(wnew));
+                const uint32_t pos = fill + rank;
+                if (pos < arena.cap) {
+                    arena.records[(uint64_t)blockIdx.x * arena.cap + pos] = rec;
+                } else {
+                    const uint32_t o = atomicAdd(&ctrl->overflow_records, 1u);
+                    if (o < arena.overflow_cap) arena.overflow[o] = rec;
                 }
+            }
+            fill += total;
+        }
+        // merge the window into the global bitmap, one word at a time
+        for (int w = tid; w < kWinTiles * 16; w += kPassAThreads) {
+            const int lt = w >> 4, wi = w & 15;
+            uint32_t* blk = s_bm + lt * 32;
+            const uint32_t l1 = blk[wi];
+            if (l1) {
+                const uint32_t l2 = blk[16 + wi];
+                const int32_t tx = wx0 + (lt % kWX), ty = wy0 + ((lt / kWX) % kWY), tz = wz0 + lt / (kWX * kWY);
+                // hits only land in tiles inside the volume
+                const uint32_t tile =
+                    vd.tile_base + ((uint32_t)tz * (uint32_t)vd.tdim[1] + (uint32_t)ty) * (uint32_t)vd.tdim[0] +
+                    (uint32_t)tx;
+                uint32_t* g = bitmap + (uint64_t)tile * 32u;
+                const uint32_t old = atomicOr(g + wi, l1);
+                const uint32_t two = l2 | (old & l1);
+                if (two) atomicOr(g + 16 + wi, two);
+                if (__atomic_load_n(tile_flag + tile, __ATOMIC_RELAXED) == 0u &&
+                    atomicCAS(tile_flag + tile, 0u, 1u) == 0u)
+                    touched_cur[atomicAdd(&ctrl->touched_count[parity], 1u)] = tile;
+                blk[wi] = 0u;
+                blk[16 + wi] = 0u;
             }
         }
         __syncthreads();
     }
     __syncthreads();
     if (!KNN_ONLY && tid == 0) {
-        atomicAdd(&ctrl->stats[0], s_stat[0]);
-        atomicAdd(&ctrl->stats[1], s_stat[1]);
-        atomicAdd(&ctrl->stats[2], s_stat[2]);
-        atomicAdd(&ctrl->totals[0], s_stat[0]);
-        atomicAdd(&ctrl->totals[1], s_stat[1]);
-        atomicAdd(&ctrl->totals[2], s_stat[2]);
+        arena.counts[blockIdx.x] = fill < arena.cap ? fill : arena.cap;
+        stat_add(stat, totals, kStSupported, s_stat[0]);
+        stat_add(stat, totals, kStInBounds, s_stat[1]);
+        stat_add(stat, totals, kStPending, s_stat[2]);
+        stat_add(stat, totals, kStSpilled, s_stat[3]);
+        if (blockIdx.x == 0) {
+            stat[kStActive * kStatSlots] = nactive;
+            atomicAdd(totals + kStActive * kStatSlots, (unsigned long long)nactive);
+        }
     }
 }
 
 struct VolPtr {
     float* tsdf;
     float* wt;
-    uint32_t tile_base;
+    uint32_t voxel_base;
     uint32_t pad;
 };
 
-__global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr, const uint4* __restrict__ records,
-                                                Ctrl* __restrict__ ctrl, const uint32_t* __restrict__ bitmap,
-                                                uint32_t vbits, const VolDesc* __restrict__ vols, int nvol,
-                                                const NodeRec* __restrict__ nrec) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    if (gid == 0) {
-        ctrl->stats[5] = ctrl->active_count;
-        ctrl->totals[5] += ctrl->active_count;
-        ctrl->active_count = 0u;
-        ctrl->work_next = 0u;
+// One pending record: write it unless its target was reached twice.
+__device__ __forceinline__ bool apply_record(const uint4 rec, const VolPtr* s_vp, int nvol,
+                                             const uint32_t* __restrict__ bitmap) {
+    const uint32_t tile = rec.y >> 9;
+    const uint32_t bit = rec.y & 511u;
+    const uint32_t word = bitmap[(uint64_t)tile * 32u + 16u + (bit >> 5)];
+    if ((word >> (bit & 31u)) & 1u) return false;
+    // volume of the voxel: last volume whose first voxel is <= rec.x
+    int lo = 0, hi = nvol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_vp[mid].voxel_base <= rec.x)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    const uint32_t n = ctrl->record_count;
-    const uint32_t vmask = vbits >= 32u ? 0xffffffffu : ((1u << vbits) - 1u);
+    const VolPtr vp = s_vp[lo];
+    const uint32_t v = rec.x - vp.voxel_base;
+    vp.tsdf[v] = __uint_as_float(rec.z);
+    vp.wt[v] = __uint_as_float(rec.w);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr, int nvol, Arena arena,
+                                                Ctrl* __restrict__ ctrl, const uint32_t* __restrict__ bitmap,
+                                                const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec,
+                                                unsigned long long* __restrict__ stat,
+                                                unsigned long long* __restrict__ totals) {
+    __shared__ VolPtr s_vp[256];
+    __shared__ unsigned long long s_cnt[2];
+    const int tid = threadIdx.x;
+    for (int v = tid; v < nvol; v += blockDim.x) s_vp[v] = vptr[v];
+    if (tid < 2) s_cnt[tid] = 0ull;
+    __syncthreads();
     unsigned long long collided = 0, updated = 0;
-    for (uint32_t r = gid; r < n; r += nthreads) {
-        const uint4 rec = records[r];
-        const uint32_t vol = vbits >= 32u ? 0u : rec.x >> vbits;
-        const uint32_t v = rec.x & vmask;
-        const VolPtr vp = vptr[vol];
-        const uint32_t tile = vp.tile_base + (rec.y >> 9);
-        const uint32_t bit = rec.y & 511u;
-        const uint32_t word = bitmap[(uint64_t)tile * 32u + 16u + (bit >> 5)];
-        if ((word >> (bit & 31u)) & 1u) {
-            ++collided;
-        } else {
-            vp.tsdf[v] = __uint_as_float(rec.z);
-            vp.wt[v] = __uint_as_float(rec.w);
-            ++updated;
+    // this workgroup's share of the per-workgroup arenas
+    for (uint32_t a = blockIdx.x; a < arena.n_arenas; a += gridDim.x) {
+        const uint32_t n = arena.counts[a];
+        const uint4* recs = arena.records + (uint64_t)a * arena.cap;
+        for (uint32_t r = tid; r < n; r += blockDim.x) {
+            if (apply_record(recs[r], s_vp, nvol, bitmap))
+                ++updated;
+            else
+                ++collided;
         }
+    }
+    // the shared overflow arena
+    uint32_t no = ctrl->overflow_records;
+    if (no > arena.overflow_cap) no = arena.overflow_cap;
+    for (uint32_t r = blockIdx.x * blockDim.x + tid; r < no; r += gridDim.x * blockDim.x) {
+        if (apply_record(arena.overflow[r], s_vp, nvol, bitmap))
+            ++updated;
+        else
+            ++collided;
     }
     // node positions as the reference's two translations leave them,
     // fl(fl(p - c) + c) (src/VolumeFusion.cu:708, 721)
+    const uint32_t gid = blockIdx.x * blockDim.x + tid;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
     for (int v = 0; v < nvol; ++v) {
         const VolDesc& vd = vols[v];
         if (!vd.nodes_out) continue;
@@ -474,26 +601,13 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
             vd.nodes_out[3ull * j + 2] = m.z + vd.c[2];
         }
     }
-    // wave sums, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-        collided += __shfl_down(collided, off);
-        updated += __shfl_down(updated, off);
+    if (collided) atomicAdd(&s_cnt[0], collided);
+    if (updated) atomicAdd(&s_cnt[1], updated);
+    __syncthreads();
+    if (tid == 0) {
+        stat_add(stat, totals, kStCollided, s_cnt[0]);
+        stat_add(stat, totals, kStUpdated, s_cnt[1]);
     }
-    if (__lane_id() == 0) {
-        if (collided) {
-            atomicAdd(&ctrl->stats[3], collided);
-            atomicAdd(&ctrl->totals[3], collided);
-        }
-        if (updated) {
-            atomicAdd(&ctrl->stats[4], updated);
-            atomicAdd(&ctrl->totals[4], updated);
-        }
-    }
-}
-
-__global__ void k_reset_after_knn(Ctrl* ctrl) {
-    ctrl->active_count = 0u;
-    ctrl->work_next = 0u;
 }
 
 }  // namespace
@@ -504,14 +618,17 @@ struct Buffers {
     NodeRec* nrec = nullptr;
     uint32_t* brick_count = nullptr;
     uint32_t* brick_nodes = nullptr;
-    uint32_t* active = nullptr;
+    uint32_t* active = nullptr;  // 2 x brick_cap (frame parity)
     uint32_t* bitmap = nullptr;
     uint32_t* tile_flag = nullptr;
     uint32_t* touched = nullptr;  // 2 x tile_cap
-    uint4* records = nullptr;
+    uint4* records = nullptr;     // arena_slots x arena_cap, then the overflow arena
+    uint32_t* arena_counts = nullptr;
+    unsigned long long* stat = nullptr;  // kStCount x kStatSlots, this frame
+    unsigned long long* totals = nullptr;
     VolDesc* vols = nullptr;
     VolPtr* vptr = nullptr;
-    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0;
+    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0, arena_slots = 0;
     uint64_t bytes = 0;
 };
 
@@ -520,11 +637,11 @@ struct Buffers {
 struct vf_workspace {
     int device = 0;
     vf::Buffers buf;
-    uint64_t frame = 0;  // selects the touched-tile list (frame parity)
+    uint64_t frame = 0;  // frame parity selects the active and touched-tile lists
     std::vector<vf::VolDesc> vols_h;
     std::vector<vf::VolPtr> vptr_h;
     // optional per-kernel timing: 4 events per frame (before k_prep_bin, after
-    // each of the three kernels), recycled every kMaxTimed frames
+    // each of the three kernels)
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
@@ -533,9 +650,11 @@ struct vf_workspace {
 namespace vf {
 namespace {
 
+constexpr int kMaxTimed = 4096;
+
 void free_buffers(Buffers& b) {
-    void* ptrs[] = {b.ctrl, b.nrec, b.brick_count, b.brick_nodes, b.active, b.bitmap,
-                    b.tile_flag, b.touched, b.records, b.vols, b.vptr};
+    void* ptrs[] = {b.ctrl, b.nrec, b.brick_count, b.brick_nodes, b.active, b.bitmap, b.tile_flag,
+                    b.touched, b.records, b.arena_counts, b.stat, b.totals, b.vols, b.vptr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = Buffers();
@@ -558,20 +677,24 @@ int dev_alloc(T** p, uint64_t count, uint64_t* bytes) {
     return VF_OK;
 }
 
+struct Need {
+    uint64_t nodes = 0, bricks = 0, tiles = 0, records = 0, nvol = 0, arena_slots = 0;
+};
+
 // (Re)allocates to at least the requested capacities; new buffers are zeroed.
-int ensure(vf_workspace* ws, uint64_t nodes, uint64_t bricks, uint64_t tiles, uint64_t records, uint64_t nvol,
-           hipStream_t stream) {
+int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     Buffers& b = ws->buf;
-    if (b.ctrl && nodes <= b.node_cap && bricks <= b.brick_cap && tiles <= b.tile_cap && records <= b.record_cap &&
-        nvol <= b.vol_cap)
+    if (b.ctrl && need.nodes <= b.node_cap && need.bricks <= b.brick_cap && need.tiles <= b.tile_cap &&
+        need.records <= b.record_cap && need.nvol <= b.vol_cap && need.arena_slots <= b.arena_slots)
         return VF_OK;
     if (b.ctrl) VF_HIP(hipStreamSynchronize(stream));
     Buffers nbuf;
-    nbuf.node_cap = std::max<uint64_t>(std::max(nodes, b.node_cap), 1);
-    nbuf.brick_cap = std::max<uint64_t>(std::max(bricks, b.brick_cap), 1);
-    nbuf.tile_cap = std::max<uint64_t>(std::max(tiles, b.tile_cap), 1);
-    nbuf.record_cap = std::max<uint64_t>(std::max(records, b.record_cap), 1);
-    nbuf.vol_cap = std::max<uint64_t>(std::max(nvol, b.vol_cap), 1);
+    nbuf.node_cap = std::max<uint64_t>(std::max(need.nodes, b.node_cap), 1);
+    nbuf.brick_cap = std::max<uint64_t>(std::max(need.bricks, b.brick_cap), 1);
+    nbuf.tile_cap = std::max<uint64_t>(std::max(need.tiles, b.tile_cap), 1);
+    nbuf.record_cap = std::max<uint64_t>(std::max(need.records, b.record_cap), 1);
+    nbuf.vol_cap = std::max<uint64_t>(std::max(need.nvol, b.vol_cap), 1);
+    nbuf.arena_slots = std::max<uint64_t>(std::max(need.arena_slots, b.arena_slots), 1);
     free_buffers(b);
     int st = VF_OK;
     uint64_t bytes = 0;
@@ -579,11 +702,14 @@ int ensure(vf_workspace* ws, uint64_t nodes, uint64_t bricks, uint64_t tiles, ui
     if (!st) st = dev_alloc(&nbuf.nrec, nbuf.node_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.brick_count, nbuf.brick_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.brick_nodes, nbuf.brick_cap * kCap, &bytes);
-    if (!st) st = dev_alloc(&nbuf.active, nbuf.brick_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.active, nbuf.brick_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.bitmap, nbuf.tile_cap * 32, &bytes);
     if (!st) st = dev_alloc(&nbuf.tile_flag, nbuf.tile_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.touched, nbuf.tile_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.records, nbuf.record_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.arena_counts, nbuf.arena_slots, &bytes);
+    if (!st) st = dev_alloc(&nbuf.stat, (uint64_t)kStCount * kStatSlots, &bytes);
+    if (!st) st = dev_alloc(&nbuf.totals, (uint64_t)kStCount * kStatSlots, &bytes);
     if (!st) st = dev_alloc(&nbuf.vols, nbuf.vol_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.vptr, nbuf.vol_cap, &bytes);
     if (st) {
@@ -591,11 +717,15 @@ int ensure(vf_workspace* ws, uint64_t nodes, uint64_t bricks, uint64_t tiles, ui
         return st;
     }
     nbuf.bytes = bytes;
-    // Zero state the kernels rely on: control block, brick counts, tiles.
+    // Zero the state the kernels rely on: control block, brick counts, tiles,
+    // statistics.
     VF_HIP(hipMemsetAsync(nbuf.ctrl, 0, sizeof(Ctrl), stream));
     VF_HIP(hipMemsetAsync(nbuf.brick_count, 0, nbuf.brick_cap * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.bitmap, 0, nbuf.tile_cap * 32 * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.tile_flag, 0, nbuf.tile_cap * sizeof(uint32_t), stream));
+    VF_HIP(hipMemsetAsync(nbuf.arena_counts, 0, nbuf.arena_slots * sizeof(uint32_t), stream));
+    VF_HIP(hipMemsetAsync(nbuf.stat, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
+    VF_HIP(hipMemsetAsync(nbuf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
     VF_HIP(hipStreamSynchronize(stream));
     b = nbuf;
     ws->frame = 0;
@@ -604,7 +734,7 @@ int ensure(vf_workspace* ws, uint64_t nodes, uint64_t bricks, uint64_t tiles, ui
 
 inline int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
 
-// Fills `vd` for one volume; accumulates the capacities the call needs.
+// Fills `vd` for one volume.
 int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, const vf_warp_params& p, bool knn_only,
               VolDesc* vd, uint64_t* n_out) {
     uint64_t n;
@@ -657,6 +787,25 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     return VF_OK;
 }
 
+int pass_a_occupancy(int k, bool knn_only, int* per_cu) {
+#define VF_PASS_A_OCC(KK)                                                                                        \
+    VF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                         \
+        per_cu, knn_only ? (const void*)k_pass_a<KK, true> : (const void*)k_pass_a<KK, false>, kPassAThreads, 0))
+    switch (k) {
+        case 1: VF_PASS_A_OCC(1); break;
+        case 2: VF_PASS_A_OCC(2); break;
+        case 3: VF_PASS_A_OCC(3); break;
+        case 4: VF_PASS_A_OCC(4); break;
+        case 5: VF_PASS_A_OCC(5); break;
+        case 6: VF_PASS_A_OCC(6); break;
+        case 7: VF_PASS_A_OCC(7); break;
+        case 8: VF_PASS_A_OCC(8); break;
+    }
+#undef VF_PASS_A_OCC
+    if (*per_cu < 1) *per_cu = 1;
+    return VF_OK;
+}
+
 int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_graph* graphs, const vf_frame* frames,
                  const vf_warp_params* p, bool knn_only, KnnOut kout, hipStream_t stream) {
     int st = check_k(p->k);
@@ -666,51 +815,71 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         if (p[v].k != p[0].k) return fail(VF_ERR_INVALID_ARGUMENT, "batched volumes must share k");
     ws->vols_h.resize(nvol);
     ws->vptr_h.resize(nvol);
-    uint64_t nodes = 0, bricks = 0, tiles = 0, voxels = 0, maxn = 0, k1 = 0;
+    Need need;
+    uint64_t voxels = 0, k1 = 0, rec_bound = 0;
     for (int v = 0; v < nvol; ++v) {
         const vf_frame empty_frame = {nullptr, 0, 0, {{0}}};
         VolDesc& vd = ws->vols_h[v];
         uint64_t n;
         st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, &vd, &n);
         if (st) return st;
-        vd.node_base = (uint32_t)nodes;
-        vd.brick_base = (uint32_t)bricks;
-        vd.tile_base = (uint32_t)tiles;
+        vd.node_base = (uint32_t)need.nodes;
+        vd.brick_base = (uint32_t)need.bricks;
+        vd.tile_base = (uint32_t)need.tiles;
+        vd.voxel_base = (uint32_t)voxels;
         vd.k1_base = (uint32_t)k1;
-        nodes += vd.G;
-        bricks += (uint64_t)vd.bdim[0] * vd.bdim[1] * vd.bdim[2];
-        tiles += (uint64_t)vd.tdim[0] * vd.tdim[1] * vd.tdim[2];
+        need.nodes += vd.G;
+        const uint64_t nb = (uint64_t)vd.bdim[0] * vd.bdim[1] * vd.bdim[2];
+        need.bricks += nb;
+        need.tiles += (uint64_t)vd.tdim[0] * vd.tdim[1] * vd.tdim[2];
         voxels += n;
         k1 += vd.k1_count;
-        maxn = std::max(maxn, n);
+        // pending records <= supported voxels <= min(N, G * cube volume)
+        const uint64_t cubevol = (uint64_t)std::max(vd.cube[0], 0) * std::max(vd.cube[1], 0) * std::max(vd.cube[2], 0);
+        rec_bound += std::min<uint64_t>(n, (uint64_t)vd.G * cubevol);
         ws->vptr_h[v].tsdf = vd.tsdf;
         ws->vptr_h[v].wt = vd.wt;
-        ws->vptr_h[v].tile_base = vd.tile_base;
+        ws->vptr_h[v].voxel_base = vd.voxel_base;
         ws->vptr_h[v].pad = 0;
-        if (bricks > 0xffffffffull || tiles > 0x7fffffull || k1 > 0xffffffffull)
-            return fail(VF_ERR_CAPACITY, "batch too large for 32-bit brick/tile/work indices");
-        if ((uint64_t)vd.bdim[0] * vd.bdim[1] * vd.bdim[2] > 0xffffffull)
-            return fail(VF_ERR_CAPACITY, "volume has more than 2^24 bricks");
+        if (nb > 0xffffffull) return fail(VF_ERR_CAPACITY, "volume has more than 2^24 bricks");
+        if (need.bricks > 0xffffffffull || need.tiles >= (1ull << 23) || k1 > 0xffffffffull ||
+            voxels > 0xffffffffull)
+            return fail(VF_ERR_CAPACITY,
+                        "batch too large: %llu voxels, %llu tiles (limits 2^32 voxels and 2^23 8^3 tiles per call)",
+                        (unsigned long long)voxels, (unsigned long long)need.tiles);
     }
-    uint32_t vbits = 32;
-    if (nvol > 1) {
-        vbits = 1;
-        while (vbits < 32 && (1ull << vbits) < maxn) ++vbits;
-        if (vbits + (uint32_t)std::ceil(std::log2((double)nvol)) > 32)
-            return fail(VF_ERR_CAPACITY, "batch of %d volumes with %llu voxels each does not fit 32-bit records", nvol,
-                        (unsigned long long)maxn);
+    need.nvol = (uint64_t)nvol;
+
+    int dev = 0, cus = 0;
+    VF_HIP(hipGetDevice(&dev));
+    VF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = 1;
+    st = pass_a_occupancy(p->k, knn_only, &per_cu);
+    if (st) return st;
+    const uint64_t blocks_a = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * per_cu, need.bricks));
+    // records: one arena per pass-A workgroup sized for an even share, plus a
+    // shared overflow arena holding the whole bound
+    uint64_t arena_cap = 0, overflow_cap = 0;
+    if (!knn_only) {
+        arena_cap = (rec_bound + blocks_a - 1) / blocks_a + kPassAThreads;
+        overflow_cap = rec_bound;
+        if (arena_cap > 0xffffffffull || overflow_cap > 0xffffffffull)
+            return fail(VF_ERR_CAPACITY, "record bound %llu exceeds 2^32", (unsigned long long)rec_bound);
+        need.records = blocks_a * arena_cap + overflow_cap;
+        need.arena_slots = blocks_a;
+    } else {
+        need.records = 1;
+        need.arena_slots = 1;
     }
-    st = ensure(ws, nodes, bricks, tiles, knn_only ? 1 : voxels, (uint64_t)nvol, stream);
+    st = ensure(ws, need, stream);
     if (st) return st;
     Buffers& B = ws->buf;
     VF_HIP(hipMemcpyAsync(B.vols, ws->vols_h.data(), sizeof(VolDesc) * nvol, hipMemcpyHostToDevice, stream));
     VF_HIP(hipMemcpyAsync(B.vptr, ws->vptr_h.data(), sizeof(VolPtr) * nvol, hipMemcpyHostToDevice, stream));
 
     const uint32_t parity = (uint32_t)(ws->frame & 1u);
-    const uint32_t prev = parity ^ 1u;
     hipEvent_t* ev = nullptr;
     if (ws->timing && !knn_only) {
-        constexpr int kMaxTimed = 4096;
         if (ws->events.empty()) {
             ws->events.resize(4 * kMaxTimed);
             for (auto& e : ws->events) VF_HIP(hipEventCreate(&e));
@@ -721,45 +890,32 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         VF_HIP(hipEventRecord(ev[0], stream));
     }
     {
-        uint64_t blocks = std::max<uint64_t>((k1 + 255) / 256, 1024);
+        const uint64_t blocks = std::max<uint64_t>((k1 + 255) / 256, 1024);
         hipLaunchKernelGGL(k_prep_bin, dim3((unsigned)blocks), dim3(256), 0, stream, B.vols, nvol, (uint32_t)k1, B.nrec,
-                           B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
-                           B.touched + (uint64_t)prev * B.tile_cap, prev);
+                           B.brick_count, B.brick_nodes, B.active + (uint64_t)parity * B.brick_cap, B.ctrl, B.bitmap,
+                           B.tile_flag, B.touched + (uint64_t)(parity ^ 1u) * B.tile_cap, parity, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_prep_bin");
         if (ev) VF_HIP(hipEventRecord(ev[1], stream));
     }
-    int dev = 0, cus = 0;
-    VF_HIP(hipGetDevice(&dev));
-    VF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    Arena arena;
+    arena.records = B.records;
+    arena.counts = B.arena_counts;
+    arena.cap = (uint32_t)arena_cap;
+    arena.n_arenas = (uint32_t)blocks_a;
+    arena.overflow = B.records + blocks_a * arena_cap;
+    arena.overflow_cap = (uint32_t)overflow_cap;
     {
-        int per_cu = 0;
-#define VF_PASS_A_OCC(KK)                                                                                        \
-    VF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                         \
-        &per_cu, knn_only ? (const void*)k_pass_a<KK, true> : (const void*)k_pass_a<KK, false>, kPassAThreads, 0))
-        switch (p->k) {
-            case 1: VF_PASS_A_OCC(1); break;
-            case 2: VF_PASS_A_OCC(2); break;
-            case 3: VF_PASS_A_OCC(3); break;
-            case 4: VF_PASS_A_OCC(4); break;
-            case 5: VF_PASS_A_OCC(5); break;
-            case 6: VF_PASS_A_OCC(6); break;
-            case 7: VF_PASS_A_OCC(7); break;
-            case 8: VF_PASS_A_OCC(8); break;
-        }
-#undef VF_PASS_A_OCC
-        if (per_cu < 1) per_cu = 1;
-        uint64_t blocks = (uint64_t)cus * (uint64_t)per_cu;
-        blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(bricks, 1));
         uint32_t* touched_cur = B.touched + (uint64_t)parity * B.tile_cap;
+        const uint32_t* active_cur = B.active + (uint64_t)parity * B.brick_cap;
 #define VF_PASS_A(KK)                                                                                                \
     if (knn_only)                                                                                                    \
-        hipLaunchKernelGGL((k_pass_a<KK, true>), dim3((unsigned)blocks), dim3(kPassAThreads), 0, stream, B.vols,     \
-                           B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,            \
-                           touched_cur, parity, B.records, vbits, kout);                                             \
+        hipLaunchKernelGGL((k_pass_a<KK, true>), dim3((unsigned)blocks_a), dim3(kPassAThreads), 0, stream, B.vols,   \
+                           B.nrec, B.brick_count, B.brick_nodes, active_cur, B.ctrl, B.bitmap, B.tile_flag,          \
+                           touched_cur, parity, arena, B.stat, B.totals, kout);                                      \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_pass_a<KK, false>), dim3((unsigned)blocks), dim3(kPassAThreads), 0, stream, B.vols,    \
-                           B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,            \
-                           touched_cur, parity, B.records, vbits, kout)
+        hipLaunchKernelGGL((k_pass_a<KK, false>), dim3((unsigned)blocks_a), dim3(kPassAThreads), 0, stream, B.vols,  \
+                           B.nrec, B.brick_count, B.brick_nodes, active_cur, B.ctrl, B.bitmap, B.tile_flag,          \
+                           touched_cur, parity, arena, B.stat, B.totals, kout)
         switch (p->k) {
             case 1: VF_PASS_A(1); break;
             case 2: VF_PASS_A(2); break;
@@ -774,17 +930,33 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         VF_LAUNCH_CHECK("k_pass_a");
         if (ev) VF_HIP(hipEventRecord(ev[2], stream));
     }
-    if (knn_only) {
-        hipLaunchKernelGGL(k_reset_after_knn, dim3(1), dim3(1), 0, stream, B.ctrl);
-        VF_LAUNCH_CHECK("k_reset_after_knn");
-    } else {
-        const uint64_t blocks = (uint64_t)cus * 8;
-        hipLaunchKernelGGL(k_pass_b, dim3((unsigned)blocks), dim3(256), 0, stream, B.vptr, B.records, B.ctrl, B.bitmap,
-                           vbits, B.vols, nvol, B.nrec);
+    if (!knn_only) {
+        const uint64_t blocks = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
+        hipLaunchKernelGGL(k_pass_b, dim3((unsigned)blocks), dim3(256), 0, stream, B.vptr, nvol, arena, B.ctrl,
+                           B.bitmap, B.vols, B.nrec, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_pass_b");
         if (ev) VF_HIP(hipEventRecord(ev[3], stream));
-        ws->frame++;
     }
+    ws->frame++;
+    return VF_OK;
+}
+
+int read_stats(vf_workspace* ws, bool totals, vf_frame_stats* out, hipStream_t stream) {
+    memset(out, 0, sizeof(*out));
+    if (!ws->buf.ctrl) return VF_OK;
+    unsigned long long h[kStCount * kStatSlots];
+    VF_HIP(hipMemcpyAsync(h, totals ? ws->buf.totals : ws->buf.stat, sizeof(h), hipMemcpyDeviceToHost, stream));
+    VF_HIP(hipStreamSynchronize(stream));
+    unsigned long long s[kStCount] = {0};
+    for (int c = 0; c < kStCount; ++c)
+        for (int k = 0; k < kStatSlots; ++k) s[c] += h[c * kStatSlots + k];
+    out->supported = s[kStSupported];
+    out->in_bounds = s[kStInBounds];
+    out->pending = s[kStPending];
+    out->collided = s[kStCollided];
+    out->updated = s[kStUpdated];
+    out->active_bricks = s[kStActive];
+    out->overflow_bricks = s[kStSpilled];
     return VF_OK;
 }
 
@@ -806,16 +978,10 @@ int vf_workspace_create(vf_workspace** out, int max_volumes, uint64_t max_total_
     *out = nullptr;
     vf_workspace* ws = new vf_workspace();
     VF_HIP(hipGetDevice(&ws->device));
-    if (max_total_voxels || max_total_nodes || max_volumes > 0) {
-        const uint64_t bricks = max_total_voxels / (kBX * kBY * kBZ) + 64ull * (uint64_t)std::max(max_volumes, 1) * 64;
-        const uint64_t tiles = max_total_voxels / 512 + 64ull * (uint64_t)std::max(max_volumes, 1) * 64;
-        int st = ensure(ws, max_total_nodes, bricks, tiles, max_total_voxels, (uint64_t)std::max(max_volumes, 1),
-                        (hipStream_t)stream);
-        if (st) {
-            delete ws;
-            return st;
-        }
-    }
+    (void)max_volumes;
+    (void)max_total_voxels;
+    (void)max_total_nodes;
+    (void)stream;
     *out = ws;
     return VF_OK;
 }
@@ -848,18 +1014,17 @@ int vf_warped_integrate_batch(vf_workspace* ws, int n, const vf_volume* volumes,
 
 int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream stream) {
     if (!ws || !stats) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_last_stats: NULL argument");
-    memset(stats, 0, sizeof(*stats));
-    if (!ws->buf.ctrl) return VF_OK;
-    Ctrl c;
-    VF_HIP(hipMemcpyAsync(&c, ws->buf.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, (hipStream_t)stream));
-    VF_HIP(hipStreamSynchronize((hipStream_t)stream));
-    stats->supported = c.stats[0];
-    stats->in_bounds = c.stats[1];
-    stats->pending = c.stats[2];
-    stats->collided = c.stats[3];
-    stats->updated = c.stats[4];
-    stats->active_bricks = c.stats[5];
-    stats->overflow_bricks = c.overflow_bricks;
+    return read_stats(ws, false, stats, (hipStream_t)stream);
+}
+
+int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream) {
+    if (!ws || !totals) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_totals: NULL argument");
+    int st = read_stats(ws, true, totals, (hipStream_t)stream);
+    if (st) return st;
+    if (reset && ws->buf.totals) {
+        VF_HIP(hipMemsetAsync(ws->buf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, (hipStream_t)stream));
+        VF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    }
     return VF_OK;
 }
 
@@ -883,26 +1048,6 @@ int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
         }
     }
     ws->timed_frames = 0;
-    return VF_OK;
-}
-
-int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream) {
-    if (!ws || !totals) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_totals: NULL argument");
-    memset(totals, 0, sizeof(*totals));
-    if (!ws->buf.ctrl) return VF_OK;
-    Ctrl c;
-    VF_HIP(hipMemcpyAsync(&c, ws->buf.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, (hipStream_t)stream));
-    VF_HIP(hipStreamSynchronize((hipStream_t)stream));
-    totals->supported = c.totals[0];
-    totals->in_bounds = c.totals[1];
-    totals->pending = c.totals[2];
-    totals->collided = c.totals[3];
-    totals->updated = c.totals[4];
-    totals->active_bricks = c.totals[5];
-    if (reset) {
-        VF_HIP(hipMemsetAsync(&ws->buf.ctrl->totals, 0, sizeof(c.totals), (hipStream_t)stream));
-        VF_HIP(hipStreamSynchronize((hipStream_t)stream));
-    }
     return VF_OK;
 }
 
