@@ -94,6 +94,18 @@ def test_weight_tsdf_closed_forms(oracle):
     assert L.vfo_weight_mesh_host(r, 0.5 * r) == float(F(math.exp(-0.125)))
 
 
+def test_markstein_division_exhaustive_mantissas(oracle):
+    """The kernels divide by a per-node 1/(2 r^2) and a per-voxel 1/sum with a
+    single fma correction.  Every dividend mantissa in [2^-12, 4) against
+    divisors spanning 2r^2 values, weight sums and extreme mantissas gives the
+    IEEE quotient."""
+    L = oracle.lib()
+    lo = int(np.array(2.0 ** -12, F).view(np.uint32))
+    hi = int(np.array(4.0, F).view(np.uint32)) - 1
+    for b in (8e-4, 3.3e-4, 1.8e-2, 0.1216, 1.0, 1.9999999, 1.5, 7.4870):
+        assert L.vfo_markstein_mismatches(float(F(b)), lo, hi) == 0, b
+
+
 def test_markstein_division_equals_ieee_on_weight_domain(oracle):
     """a/b = fma correction of a*RN(1/b) (Markstein 1990) equals IEEE
     division for a in the weight range and b in [0.1, 8] (the weight sum)."""

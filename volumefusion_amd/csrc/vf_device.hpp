@@ -61,6 +61,23 @@ __device__ __forceinline__ float expw(float xf, const double* tab) {
     return (float)__builtin_ldexp(t, k >> 5);
 }
 
+// expw for arguments known to lie in [-87, 88.5] (the weight argument is in
+// [-2.000001, 0]): the same steps without the range checks.
+__device__ __forceinline__ float expw_dom(float xf, const double* tab) {
+    const double x = (double)xf;
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+5);
+    const int k = (int)kd;
+    double r = __builtin_fma(-kd, 0x1.62e42fee00000p-6, x);
+    r = __builtin_fma(-kd, 0x1.a39ef35793c76p-38, r);
+    double p = __builtin_fma(0x1.1111111111111p-7, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    const double t = p * tab[k & 31];
+    return (float)__builtin_ldexp(t, k >> 5);
+}
+
 // a / b for a >= 0 in the weight domain from y = RN(1/b): q0 = RN(a y),
 // rem = a - q0 b exactly (fma), q = RN(q0 + rem y) (Markstein, IBM J. Res.
 // Dev. 34(1), 1990).  Equal to the IEEE quotient on the operand ranges used
@@ -79,11 +96,92 @@ __device__ __forceinline__ float warp_weight(float den, float d) {
     return expw(num / den, kExp2Table);
 }
 
-// The same weight with the node's 1/den precomputed; identical result.
+// The same weight with the node's 1/den precomputed; identical result.  The
+// argument -(d*d)/den lies in [-2.000001, 0] for candidates (d <= 2r).
 __device__ __forceinline__ float warp_weight_rcp(float den, float rden, float d, const double* tab) {
     const float q = d * d;
-    return expw(-div_rcp(q, den, rden), tab);
+    return expw_dom(-div_rcp(q, den, rden), tab);
 }
+
+// Batcher odd-even merge sort networks on 64-bit keys for 1, 2, 4, 8 keys.
+__device__ __forceinline__ void cmpx(unsigned long long& a, unsigned long long& b) {
+    const bool sw = b < a;
+    const unsigned long long lo = sw ? b : a;
+    const unsigned long long hi = sw ? a : b;
+    a = lo;
+    b = hi;
+}
+
+template <int N>
+__device__ __forceinline__ void sort_keys(unsigned long long* k) {
+    if constexpr (N == 2) {
+        cmpx(k[0], k[1]);
+    } else if constexpr (N == 4) {
+        cmpx(k[0], k[1]); cmpx(k[2], k[3]);
+        cmpx(k[0], k[2]); cmpx(k[1], k[3]);
+        cmpx(k[1], k[2]);
+    } else if constexpr (N == 8) {
+        cmpx(k[0], k[1]); cmpx(k[2], k[3]); cmpx(k[4], k[5]); cmpx(k[6], k[7]);
+        cmpx(k[0], k[2]); cmpx(k[1], k[3]); cmpx(k[4], k[6]); cmpx(k[5], k[7]);
+        cmpx(k[1], k[2]); cmpx(k[5], k[6]);
+        cmpx(k[0], k[4]); cmpx(k[1], k[5]); cmpx(k[2], k[6]); cmpx(k[3], k[7]);
+        cmpx(k[2], k[4]); cmpx(k[3], k[5]);
+        cmpx(k[1], k[2]); cmpx(k[3], k[4]); cmpx(k[5], k[6]);
+    }
+}
+
+constexpr int pow2_at_least(int k) { return k <= 1 ? 1 : (k <= 2 ? 2 : (k <= 4 ? 4 : 8)); }
+
+// Canonical knn selection with per-thread LDS slots.  Each candidate is a key
+// (bits(dist) << 32) | id; for dist >= 0 the key order is the (dist, id)
+// order.  While a thread holds fewer than K keys a candidate is appended
+// (one LDS store); when full it replaces the largest key if smaller (rare).
+// finish() sorts the K slots with a sorting network.  Slots are strided by the
+// workgroup size so a wave's stores hit consecutive 8-byte words.
+template <int K, int STRIDE>
+struct KnnSlots {
+    unsigned long long* slot;  // this thread's slot 0 in LDS
+    int n;
+
+    __device__ __forceinline__ void init(unsigned long long* base, int tid) {
+        slot = base + tid;
+        n = 0;
+    }
+
+    __device__ __forceinline__ void offer(float dist, uint32_t id) {
+        const unsigned long long key = ((unsigned long long)__float_as_uint(dist) << 32) | id;
+        if (n < K) {
+            slot[n * STRIDE] = key;
+            ++n;
+        } else {
+            int mi = 0;
+            unsigned long long mk = slot[0];
+#pragma unroll
+            for (int s = 1; s < K; ++s) {
+                const unsigned long long v = slot[s * STRIDE];
+                if (v > mk) {
+                    mk = v;
+                    mi = s;
+                }
+            }
+            if (key < mk) slot[mi * STRIDE] = key;
+        }
+    }
+
+    // Sorted selection into registers: d[s], i[s] for s < n.
+    __device__ __forceinline__ void finish(float* d, uint32_t* i) const {
+        constexpr int P = pow2_at_least(K);
+        unsigned long long k[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) k[s] = (s < K && s < n) ? slot[s * STRIDE] : ~0ull;
+        sort_keys<P>(k);
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            d[s] = __uint_as_float((uint32_t)(k[s] >> 32));
+            i[s] = (uint32_t)k[s];
+        }
+    }
+};
 
 // Per-volume geometry, as the reference derives it from (dims, voxel size,
 // volume size): inv = float(1.0 / double(vs)) (src/TSDFWarp.cu:12) and
@@ -198,42 +296,5 @@ __device__ __forceinline__ void blend_sample(float to, float wo, float tn, float
     *t_out = (a + b) / wn;
     *w_out = (maxw < wn) ? maxw : wn;
 }
-
-// Canonical knn list kept sorted ascending by (dist, node order) in
-// registers (+inf sentinels fill the free slots).  Callers offer candidates in
-// ascending node order, so a strict `<` on the distance keeps the lower index
-// first among equal distances.  Node positions and radii must be finite.
-template <int K>
-struct KnnList {
-    float d[K];
-    uint32_t i[K];
-    int n;
-
-    __device__ __forceinline__ void init() {
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-            d[s] = kInf;
-            i[s] = 0xffffffffu;
-        }
-        n = 0;
-    }
-
-    // Insert a finite candidate distance (candidates satisfy d <= 2r).
-    __device__ __forceinline__ void offer(float dist, uint32_t id) {
-        bool c[K];
-#pragma unroll
-        for (int s = 0; s < K; ++s) c[s] = dist < d[s];
-#pragma unroll
-        for (int s = K - 1; s > 0; --s) {
-            const float ds = c[s - 1] ? d[s - 1] : (c[s] ? dist : d[s]);
-            const uint32_t is = c[s - 1] ? i[s - 1] : (c[s] ? id : i[s]);
-            d[s] = ds;
-            i[s] = is;
-        }
-        d[0] = c[0] ? dist : d[0];
-        i[0] = c[0] ? id : i[0];
-        n += (c[K - 1] && n < K) ? 1 : 0;
-    }
-};
 
 }  // namespace vf

@@ -224,9 +224,9 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
     __shared__ float4 s_t2[kCap];
     __shared__ float2 s_w[kCap];
     __shared__ double s_exp2[32];
+    __shared__ unsigned long long s_keys[K * kPassAThreads];
     __shared__ uint32_t s_bm[KNN_ONLY ? 1 : kWinTiles * 32];
     __shared__ uint32_t s_wave_emit[kPassAThreads / 64];
-    __shared__ uint32_t s_item;
     __shared__ unsigned long long s_stat[4];
 
     const int tid = threadIdx.x;
@@ -251,23 +251,43 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
     const int lz = tid / (kBX * kBY);
     uint32_t fill = 0;  // records in this workgroup's arena (uniform)
 
-    for (;;) {
-        if (tid == 0) {
-            const uint32_t i = atomicAdd(&ctrl->work_next[parity], 1u);
-            s_item = i < nactive ? active[i] : kNone;
+    // Static schedule: this workgroup takes active bricks blockIdx.x,
+    // blockIdx.x + gridDim.x, ...  The brick after the current one is fetched
+    // while the current one computes: its entry, its list count and, for this
+    // thread's list slot, the node id and the node record.
+    const uint32_t stride = gridDim.x;
+    uint32_t a = blockIdx.x;
+    uint32_t item = a < nactive ? active[a] : kNone;
+    uint32_t cnt = 0, pre_id = 0;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 pm = z4, plo = z4, pt0 = z4, pt1 = z4, pt2 = z4, pw = z4;  // this thread's prefetched node record
+    if (item != kNone) {
+        const VolDesc& v0 = vols[item >> 24];
+        const uint32_t gb0 = v0.brick_base + (item & 0xffffffu);
+        cnt = brick_count[gb0];
+        if (cnt <= (uint32_t)kCap && (uint32_t)tid < cnt) {
+            pre_id = brick_nodes[(uint64_t)gb0 * kCap + tid];
+            const float4* r = (const float4*)(nrec + v0.node_base + pre_id);
+            pm = r[0];
+            plo = r[1];
+            pt0 = r[2];
+            pt1 = r[3];
+            pt2 = r[4];
+            pw = r[5];
         }
-        __syncthreads();
-        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item == kNone) break;
+    }
+    __syncthreads();
+
+    while (item != kNone) {
+        item = __builtin_amdgcn_readfirstlane(item);
+        cnt = __builtin_amdgcn_readfirstlane(cnt);
         const uint32_t vol = item >> 24;
         const uint32_t b = item & 0xffffffu;
         const VolDesc& vd = vols[vol];
         const int32_t dimx = vd.g.dim[0], dimy = vd.g.dim[1], dimz = vd.g.dim[2];
         const uint32_t cx = (uint32_t)vd.cube[0], cy = (uint32_t)vd.cube[1], cz = (uint32_t)vd.cube[2];
         const uint32_t gb = vd.brick_base + b;
-        const uint32_t cnt = brick_count[gb];
         const bool spill = cnt > (uint32_t)kCap;
-        const uint32_t nchunks = spill ? (vd.G + kCap - 1) / kCap : 1u;
 
         const int32_t bxi = (int32_t)(b % (uint32_t)vd.bdim[0]);
         const int32_t byi = (int32_t)((b / (uint32_t)vd.bdim[0]) % (uint32_t)vd.bdim[1]);
@@ -275,42 +295,41 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
         const int32_t x = bxi * kBX + lx, y = byi * kBY + ly, z = bzi * kBZ + lz;
         const bool valid = x < dimx && y < dimy && z < dimz;
 
+        // next brick: entry, count and list ids (all kCap slots, in bounds)
+        const uint32_t a_n = a + stride;
+        const uint32_t item_n = a_n < nactive ? active[a_n] : kNone;
+        uint32_t cnt_n = 0, id_n = 0, nbase_n = 0;
+        if (item_n != kNone) {
+            const VolDesc& vn = vols[item_n >> 24];
+            const uint32_t gbn = vn.brick_base + (item_n & 0xffffffu);
+            nbase_n = vn.node_base;
+            cnt_n = brick_count[gbn];
+            if (tid < kCap) id_n = brick_nodes[(uint64_t)gbn * kCap + tid];
+        }
+
         float p[3];
         voxel_to_world(vd.g, x, y, z, p);
-        KnnList<K> kl;
-        kl.init();
+        KnnSlots<K, kPassAThreads> ks;
+        ks.init(s_keys, tid);
 
-        for (uint32_t ch = 0; ch < nchunks; ++ch) {
-            uint32_t m;
-            if (!spill) {
-                m = cnt;
-                if ((uint32_t)tid < m) s_raw[tid] = brick_nodes[(uint64_t)gb * kCap + tid];
-                __syncthreads();
-                if ((uint32_t)tid < m) {
-                    const uint32_t id = s_raw[tid];
-                    uint32_t rank = 0;
-                    for (uint32_t e = 0; e < m; ++e) rank += s_raw[e] < id ? 1u : 0u;
-                    s_ids[rank] = id;
-                }
-                __syncthreads();
-            } else {
-                const uint32_t base = ch * (uint32_t)kCap;
-                m = vd.G - base < (uint32_t)kCap ? vd.G - base : (uint32_t)kCap;
-                if ((uint32_t)tid < m) s_ids[tid] = base + tid;
-                __syncthreads();
-            }
-            if ((uint32_t)tid < m) {
-                const NodeRec rec = nrec[vd.node_base + s_ids[tid]];
-                s_m[tid] = rec.m;
-                s_lo[tid] = rec.lo;
-                s_t0[tid] = rec.t0;
-                s_t1[tid] = rec.t1;
-                s_t2[tid] = rec.t2;
-                s_w[tid] = make_float2(rec.w.x, rec.w.y);
+        if (!spill) {
+            // this brick's list, sorted by node index, with its node records
+            if ((uint32_t)tid < cnt) s_raw[tid] = pre_id;
+            __syncthreads();
+            if ((uint32_t)tid < cnt) {
+                uint32_t rank = 0;
+                for (uint32_t e = 0; e < cnt; ++e) rank += s_raw[e] < pre_id ? 1u : 0u;
+                s_ids[rank] = pre_id;
+                s_m[rank] = pm;
+                s_lo[rank] = make_int4(__float_as_int(plo.x), __float_as_int(plo.y), __float_as_int(plo.z), 0);
+                s_t0[rank] = pt0;
+                s_t1[rank] = pt1;
+                s_t2[rank] = pt2;
+                s_w[rank] = make_float2(pw.x, pw.y);
             }
             __syncthreads();
             if (valid) {
-                for (uint32_t e = 0; e < m; ++e) {
+                for (uint32_t e = 0; e < cnt; ++e) {
                     const int4 l = s_lo[e];
                     const bool inside = ((uint32_t)(x - l.x) < cx) & ((uint32_t)(y - l.y) < cy) &
                                         ((uint32_t)(z - l.z) < cz);
@@ -319,19 +338,64 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                         const float dx = p[0] - nm.x, dy = p[1] - nm.y, dz = p[2] - nm.z;
                         const float s = dx * dx + (dy * dy + dz * dz);
                         const float d = __builtin_sqrtf(s);
-                        if (!(d > nm.w)) kl.offer(d, spill ? s_ids[e] : e);
+                        if (!(d > nm.w)) ks.offer(d, e);
                     }
                 }
             }
-            if (spill && ch + 1 < nchunks) __syncthreads();
+        } else {
+            // the list spilled: walk every node of the volume in chunks
+            const uint32_t nchunks = (vd.G + kCap - 1) / kCap;
+            for (uint32_t ch = 0; ch < nchunks; ++ch) {
+                const uint32_t base = ch * (uint32_t)kCap;
+                const uint32_t m = vd.G - base < (uint32_t)kCap ? vd.G - base : (uint32_t)kCap;
+                __syncthreads();
+                if ((uint32_t)tid < m) {
+                    const NodeRec rec = nrec[vd.node_base + base + tid];
+                    s_ids[tid] = base + tid;
+                    s_m[tid] = rec.m;
+                    s_lo[tid] = rec.lo;
+                }
+                __syncthreads();
+                if (valid) {
+                    for (uint32_t e = 0; e < m; ++e) {
+                        const int4 l = s_lo[e];
+                        const bool inside = ((uint32_t)(x - l.x) < cx) & ((uint32_t)(y - l.y) < cy) &
+                                            ((uint32_t)(z - l.z) < cz);
+                        if (inside) {
+                            const float4 nm = s_m[e];
+                            const float dx = p[0] - nm.x, dy = p[1] - nm.y, dz = p[2] - nm.z;
+                            const float s = dx * dx + (dy * dy + dz * dz);
+                            const float d = __builtin_sqrtf(s);
+                            if (!(d > nm.w)) ks.offer(d, base + e);
+                        }
+                    }
+                }
+            }
         }
+        // next brick's node record for this thread's list slot
+        float4 pm_n = z4, plo_n = z4, pt0_n = z4, pt1_n = z4, pt2_n = z4, pw_n = z4;
+        if (item_n != kNone && cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) {
+            const float4* r = (const float4*)(nrec + nbase_n + id_n);
+            pm_n = r[0];
+            plo_n = r[1];
+            pt0_n = r[2];
+            pt1_n = r[3];
+            pt2_n = r[4];
+            pw_n = r[5];
+        }
+
         if (tid == 0) {
             brick_count[gb] = 0u;
             if (spill) s_stat[3] += 1ull;
         }
 
-        const int n = valid ? kl.n : 0;
+        const int n = valid ? ks.n : 0;
         const uint64_t vidx = ((uint64_t)z * dimy + y) * dimx + x;
+        struct {
+            float d[K];
+            uint32_t i[K];
+        } kl;
+        ks.finish(kl.d, kl.i);
 
         if (KNN_ONLY) {
             if (n > 0) {
@@ -344,10 +408,7 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                     }
                 }
             }
-            __syncthreads();
-            continue;
-        }
-
+        } else {
         // collision window of this brick: kWX x kWY x kWZ tiles from (wx0, wy0, wz0)
         const int32_t wx0 = (bxi * kBX - kMargin) >> 3;
         const int32_t wy0 = (byi * kBY - kMargin) >> 3;
@@ -509,7 +570,18 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
                 blk[16 + wi] = 0u;
             }
         }
+        }  // !KNN_ONLY
         __syncthreads();
+        a = a_n;
+        item = item_n;
+        cnt = cnt_n;
+        pre_id = id_n;
+        pm = pm_n;
+        plo = plo_n;
+        pt0 = pt0_n;
+        pt1 = pt1_n;
+        pt2 = pt2_n;
+        pw = pw_n;
     }
     __syncthreads();
     if (!KNN_ONLY && tid == 0) {
