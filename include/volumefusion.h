@@ -212,6 +212,61 @@ int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_
 int vf_workspace_set_timing(vf_workspace* ws, int enable);
 int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames);
 
+/* Launch tuning.  brick_shape selects the voxel brick one pass-A workgroup
+ * sweeps per step (one voxel per thread); wg_per_cu caps the resident pass-A
+ * workgroups per CU (0 = as many as fit).  Every setting gives identical
+ * results; only the speed differs.  ms_per_frame is what the autotuner
+ * measured (0 when set by hand). */
+enum {
+    VF_BRICK_16x8x4 = 0, /* 512 threads (default) */
+    VF_BRICK_16x4x4 = 1, /* 256 threads */
+    VF_BRICK_8x8x4 = 2,  /* 256 threads */
+    VF_BRICK_8x8x8 = 3,  /* 512 threads */
+    VF_BRICK_32x4x4 = 4, /* 512 threads */
+    VF_BRICK_SHAPES = 5
+};
+typedef struct vf_tuning {
+    int32_t brick_shape;
+    int32_t wg_per_cu;
+    float ms_per_frame;
+} vf_tuning;
+
+/* Tuning used by later vf_warped_integrate* calls on this workspace. */
+int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning);
+
+/* ------------------------------------------------------------------------ */
+/* Frame plans: a fixed batch, replayed every frame                         */
+/* ------------------------------------------------------------------------ */
+/* A plan fixes a batch of volumes and the device buffers of their graphs and
+ * frames; their contents (depth, transforms, node positions, tsdf) may change
+ * between launches, the pointers may not.  The plan owns its workspace, and
+ * with VF_PLAN_GRAPH it records its three kernels into a HIP graph on the
+ * first launch and replays that graph afterwards (the kernels read the frame
+ * parity from device memory, so the recorded launches stay valid).  Creating a
+ * plan uploads its descriptors on `stream` and waits for them.  A plan's
+ * launches must be ordered (one stream, or events); plans do not share
+ * buffers. */
+typedef struct vf_plan vf_plan;
+enum { VF_PLAN_GRAPH = 1 };
+
+/* tuning NULL: the tuning the autotuner last chose for a batch of this size
+ * class in this process, or the default. */
+int vf_plan_create(vf_plan** plan, int n, const vf_volume* volumes, const vf_graph* graphs, const vf_frame* frames,
+                   const vf_warp_params* params, const vf_tuning* tuning, int flags, vf_stream stream);
+int vf_plan_destroy(vf_plan* plan);
+/* One frame for every volume of the plan, asynchronous on `stream`. */
+int vf_plan_launch(vf_plan* plan, vf_stream stream);
+/* Times every (brick shape, workgroups per CU) pair for `frames` frames on
+ * scratch copies of the plan's volumes (the volumes and node positions are
+ * left as they are), keeps the fastest for this plan and for later plans of
+ * the same size class, and returns it.  Blocks on `stream`. */
+int vf_plan_autotune(vf_plan* plan, int frames, vf_tuning* chosen, vf_stream stream);
+int vf_plan_get_tuning(const vf_plan* plan, vf_tuning* tuning);
+/* Statistics of the last launch and the totals since the last reset
+ * (either pointer may be NULL); blocks on `stream`. */
+int vf_plan_stats(vf_plan* plan, vf_frame_stats* last, vf_frame_stats* totals, int reset, vf_stream stream);
+uint64_t vf_plan_bytes(const vf_plan* plan);
+
 /* ------------------------------------------------------------------------ */
 /* Device plumbing for hosts without a GPU framework (Python via ctypes)    */
 /* ------------------------------------------------------------------------ */

@@ -14,12 +14,14 @@ def dev(a):
     return DeviceArray.from_numpy(np.ascontiguousarray(a))
 
 
-def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True):
+def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None):
     n = s.n_voxels
     t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
     tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
     graph = ops.DeviceMotionGraph(s.nodes, s.transforms(frames[0]), s.radii)
     ws = ops.Workspace()
+    if tuning is not None:
+        ws.set_tuning(*tuning)
     for f in frames:
         depth, T = s.depth(f), s.transforms(f)
         if gather:

@@ -86,6 +86,13 @@ class FrameStats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("brick_shape", C.c_int32), ("wg_per_cu", C.c_int32), ("ms_per_frame", C.c_float)]
+
+
+BRICK_SHAPES = [(16, 8, 4), (16, 4, 4), (8, 8, 4), (8, 8, 8), (32, 4, 4)]  # VF_BRICK_* order
+PLAN_GRAPH = 1
+
 vp = C.c_void_p
 i32 = C.c_int32
 u32 = C.c_uint32
@@ -120,6 +127,15 @@ _SIGS = {
     "vf_workspace_totals": [vp, C.POINTER(FrameStats), C.c_int, vp],
     "vf_workspace_set_timing": [vp, C.c_int],
     "vf_workspace_kernel_ms": [vp, C.POINTER(f32), C.POINTER(C.c_int)],
+    "vf_workspace_set_tuning": [vp, C.POINTER(Tuning)],
+    "vf_plan_create": [C.POINTER(vp), C.c_int, C.POINTER(Volume), C.POINTER(Graph), C.POINTER(Frame),
+                       C.POINTER(WarpParams), C.POINTER(Tuning), C.c_int, vp],
+    "vf_plan_destroy": [vp],
+    "vf_plan_launch": [vp, vp],
+    "vf_plan_autotune": [vp, C.c_int, C.POINTER(Tuning), vp],
+    "vf_plan_get_tuning": [vp, C.POINTER(Tuning)],
+    "vf_plan_stats": [vp, C.POINTER(FrameStats), C.POINTER(FrameStats), C.c_int, vp],
+    "vf_plan_bytes": [vp],
     "vf_device_count": [C.POINTER(C.c_int)],
     "vf_set_device": [C.c_int],
     "vf_malloc": [C.POINTER(vp), u64],
@@ -146,6 +162,7 @@ _RESTYPES = {
     "vf_last_error": C.c_char_p,
     "vf_version": C.c_char_p,
     "vf_workspace_bytes": u64,
+    "vf_plan_bytes": u64,
 }
 
 _lib: C.CDLL | None = None

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -34,14 +35,41 @@
 namespace vf {
 namespace {
 
-constexpr int kCap = 128;                  // node slots per brick list (and chunk size when a list spills)
-constexpr int kPassAThreads = 512;         // one voxel per thread, one brick per workgroup
-constexpr int kBX = 16, kBY = 8, kBZ = 4;  // brick shape: 64 B of tsdf per brick row
-constexpr int kMargin = 8;                 // collision window reaches 8 voxels beyond the brick
-constexpr int kWX = 4, kWY = 3, kWZ = 3;   // window size in 8^3 tiles (covers the margin for this brick shape)
-constexpr int kWinTiles = kWX * kWY * kWZ;
-constexpr int kStatSlots = 64;             // statistics counters are spread over 64 addresses
+constexpr int kCap = 128;       // node slots per brick list (and chunk size when a list spills)
+constexpr int kMargin = 8;      // collision window reaches 8 voxels beyond the brick
+constexpr int kStatSlots = 64;  // statistics counters are spread over 64 addresses
 constexpr uint32_t kNone = 0xffffffffu;
+
+// Tiles of 8 voxels that cover [x0 - kMargin, x0 + b - 1 + kMargin] for every
+// brick origin x0 (a multiple of b); origins repeat modulo lcm(b, 8) <= 8 b.
+constexpr int window_tiles(int b) {
+    int m = 0;
+    for (int x0 = 0; x0 < 8 * b; x0 += b) {
+        const int lo = (x0 - kMargin) >= 0 ? (x0 - kMargin) / 8 : -((kMargin - x0 + 7) / 8);
+        const int t = (x0 + b - 1 + kMargin) / 8 - lo + 1;
+        m = t > m ? t : m;
+    }
+    return m;
+}
+
+// Brick shape: one voxel per thread, x fastest, one brick per workgroup step.
+template <int BX_, int BY_, int BZ_>
+struct Shape {
+    static constexpr int BX = BX_, BY = BY_, BZ = BZ_;
+    static constexpr int kThreads = BX * BY * BZ;
+    static constexpr int WX = window_tiles(BX), WY = window_tiles(BY), WZ = window_tiles(BZ);
+    static constexpr int kWinTiles = WX * WY * WZ;
+    static_assert(kThreads % 64 == 0 && kThreads >= kCap && kThreads <= 1024, "brick must be whole waves");
+};
+
+// The brick shapes the kernels are built for (VF_BRICK_* in volumefusion.h).
+using Shape0 = Shape<16, 8, 4>;  // 512 threads, 64 B rows
+using Shape1 = Shape<16, 4, 4>;  // 256 threads
+using Shape2 = Shape<8, 8, 4>;   // 256 threads
+using Shape3 = Shape<8, 8, 8>;   // 512 threads
+using Shape4 = Shape<32, 4, 4>;  // 512 threads, 128 B rows
+constexpr int kNumShapes = 5;
+constexpr int kShapeDims[kNumShapes][3] = {{16, 8, 4}, {16, 4, 4}, {8, 8, 4}, {8, 8, 8}, {32, 4, 4}};
 
 enum { kStSupported, kStInBounds, kStPending, kStCollided, kStUpdated, kStActive, kStSpilled, kStCount = 8 };
 
@@ -84,8 +112,14 @@ struct Ctrl {
     uint32_t work_next[2];
     uint32_t touched_count[2];
     uint32_t overflow_records;
-    uint32_t pad;
+    uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
 };
+
+// Frame parity: from the host, or (parity_arg < 0) from the device frame
+// counter, so that a recorded launch sequence can be replayed unchanged.
+__device__ __forceinline__ uint32_t frame_parity(const Ctrl* ctrl, int parity_arg) {
+    return parity_arg >= 0 ? (uint32_t)parity_arg : (ctrl->frame & 1u);
+}
 
 struct KnnOut {
     uint32_t* idx;
@@ -110,16 +144,21 @@ __device__ __forceinline__ void stat_add(unsigned long long* stat, unsigned long
     atomicAdd(totals + slot, v);
 }
 
+template <class S>
 __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vols, int nvol, uint32_t total,
                                                   NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
-                                                  uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active,
+                                                  uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active2,
                                                   Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap,
                                                   uint32_t* __restrict__ tile_flag,
-                                                  const uint32_t* __restrict__ touched_prev, uint32_t parity,
+                                                  const uint32_t* __restrict__ touched2, uint64_t brick_cap,
+                                                  uint64_t tile_cap, int parity_arg,
                                                   unsigned long long* __restrict__ stat,
                                                   unsigned long long* __restrict__ totals) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t parity = frame_parity(ctrl, parity_arg);
+    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
+    const uint32_t* __restrict__ touched_prev = touched2 + (uint64_t)(parity ^ 1u) * tile_cap;
 
     // Clear the collision tiles the previous frame on this workspace touched.
     const uint32_t nt = ctrl->touched_count[parity ^ 1u];
@@ -183,7 +222,7 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
     }
     // bricks overlapped by the cube clipped to the volume
     int32_t blo[3], bhi[3];
-    const int32_t bs[3] = {kBX, kBY, kBZ};
+    const int32_t bs[3] = {S::BX, S::BY, S::BZ};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const long long l = lo[a] < 0 ? 0 : lo[a];
@@ -208,13 +247,17 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
     }
 }
 
-template <int K, bool KNN_ONLY>
-__global__ __launch_bounds__(kPassAThreads) void k_pass_a(
+template <int K, bool KNN_ONLY, class S>
+__global__ __launch_bounds__(S::kThreads) void k_pass_a(
     const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
-    const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ active, Ctrl* __restrict__ ctrl,
-    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t* __restrict__ touched_cur,
-    uint32_t parity, Arena arena, unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals,
-    KnnOut kout) {
+    const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
+    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t* __restrict__ touched2,
+    uint64_t brick_cap, uint64_t tile_cap, int parity_arg, Arena arena, unsigned long long* __restrict__ stat,
+    unsigned long long* __restrict__ totals, KnnOut kout) {
+    constexpr int kPassAThreads = S::kThreads;
+    constexpr int kBX = S::BX, kBY = S::BY, kBZ = S::BZ;
+    constexpr int kWX = S::WX, kWY = S::WY, kWZ = S::WZ;
+    constexpr int kWinTiles = S::kWinTiles;
     __shared__ uint32_t s_raw[kCap];
     __shared__ uint32_t s_ids[kCap];
     __shared__ float4 s_m[kCap];
@@ -232,6 +275,9 @@ __global__ __launch_bounds__(kPassAThreads) void k_pass_a(
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int wave = tid >> 6;
+    const uint32_t parity = frame_parity(ctrl, parity_arg);
+    const uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
+    uint32_t* __restrict__ touched_cur = touched2 + (uint64_t)parity * tile_cap;
     if (tid < 4) s_stat[tid] = 0ull;
     if (tid < 32) s_exp2[tid] = kExp2Table[tid];
     if (!KNN_ONLY) {
@@ -679,6 +725,8 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
     if (tid == 0) {
         stat_add(stat, totals, kStCollided, s_cnt[0]);
         stat_add(stat, totals, kStUpdated, s_cnt[1]);
+        // no kernel of this frame reads the counter any more
+        if (blockIdx.x == 0) ctrl->frame = ctrl->frame + 1u;
     }
 }
 
@@ -717,6 +765,7 @@ struct vf_workspace {
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
+    vf_tuning tuning = {0, 0, 0.0f};
 };
 
 namespace vf {
@@ -806,9 +855,9 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
 
 inline int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
 
-// Fills `vd` for one volume.
+// Fills `vd` for one volume, with bricks of shape `shape`.
 int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, const vf_warp_params& p, bool knn_only,
-              VolDesc* vd, uint64_t* n_out) {
+              int shape, VolDesc* vd, uint64_t* n_out) {
     uint64_t n;
     int st = check_volume(vol.dim, vol.voxel_size, &n);
     if (st) return st;
@@ -841,7 +890,7 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     vd->cube[0] = p.voxels_per_node.x;
     vd->cube[1] = p.voxels_per_node.y;
     vd->cube[2] = p.voxels_per_node.z;
-    const int32_t bs[3] = {kBX, kBY, kBZ};
+    const int32_t* bs = kShapeDims[shape];
     const int32_t dims[3] = {vol.dim.x, vol.dim.y, vol.dim.z};
     for (int a = 0; a < 3; ++a) {
         vd->bdim[a] = ceil_div(dims[a], bs[a]);
@@ -859,32 +908,70 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     return VF_OK;
 }
 
-int pass_a_occupancy(int k, bool knn_only, int* per_cu) {
-#define VF_PASS_A_OCC(KK)                                                                                        \
-    VF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                         \
-        per_cu, knn_only ? (const void*)k_pass_a<KK, true> : (const void*)k_pass_a<KK, false>, kPassAThreads, 0))
+// Everything a frame's three launches need besides the device buffers.
+struct FrameSetup {
+    int nvol = 0;
+    int k = 0;
+    int shape = 0;
+    bool knn_only = false;
+    uint32_t k1 = 0;
+    uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0;
+};
+
+template <class S>
+const void* pass_a_kernel(int k, bool knn_only) {
+#define VF_PASS_A_FN(KK) \
+    return knn_only ? (const void*)k_pass_a<KK, true, S> : (const void*)k_pass_a<KK, false, S>
     switch (k) {
-        case 1: VF_PASS_A_OCC(1); break;
-        case 2: VF_PASS_A_OCC(2); break;
-        case 3: VF_PASS_A_OCC(3); break;
-        case 4: VF_PASS_A_OCC(4); break;
-        case 5: VF_PASS_A_OCC(5); break;
-        case 6: VF_PASS_A_OCC(6); break;
-        case 7: VF_PASS_A_OCC(7); break;
-        case 8: VF_PASS_A_OCC(8); break;
+        case 1: VF_PASS_A_FN(1);
+        case 2: VF_PASS_A_FN(2);
+        case 3: VF_PASS_A_FN(3);
+        case 4: VF_PASS_A_FN(4);
+        case 5: VF_PASS_A_FN(5);
+        case 6: VF_PASS_A_FN(6);
+        case 7: VF_PASS_A_FN(7);
+        default: VF_PASS_A_FN(8);
     }
-#undef VF_PASS_A_OCC
+#undef VF_PASS_A_FN
+}
+
+// Resident pass-A workgroups per CU for (k, shape).
+int pass_a_occupancy(int k, bool knn_only, int shape, int* per_cu) {
+    const void* fn = nullptr;
+    int threads = 0;
+    switch (shape) {
+        case 0: fn = pass_a_kernel<Shape0>(k, knn_only); threads = Shape0::kThreads; break;
+        case 1: fn = pass_a_kernel<Shape1>(k, knn_only); threads = Shape1::kThreads; break;
+        case 2: fn = pass_a_kernel<Shape2>(k, knn_only); threads = Shape2::kThreads; break;
+        case 3: fn = pass_a_kernel<Shape3>(k, knn_only); threads = Shape3::kThreads; break;
+        default: fn = pass_a_kernel<Shape4>(k, knn_only); threads = Shape4::kThreads; break;
+    }
+    VF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, threads, 0));
     if (*per_cu < 1) *per_cu = 1;
     return VF_OK;
 }
 
-int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_graph* graphs, const vf_frame* frames,
-                 const vf_warp_params* p, bool knn_only, KnnOut kout, hipStream_t stream) {
+int check_tuning(const vf_tuning& t) {
+    if (t.brick_shape < 0 || t.brick_shape >= kNumShapes)
+        return fail(VF_ERR_INVALID_ARGUMENT, "brick shape %d: supported 0..%d", t.brick_shape, kNumShapes - 1);
+    if (t.wg_per_cu < 0 || t.wg_per_cu > 64)
+        return fail(VF_ERR_INVALID_ARGUMENT, "workgroups per CU %d: supported 0 (occupancy limit) .. 64", t.wg_per_cu);
+    return VF_OK;
+}
+
+// Builds the descriptors of a batch, sizes the launches, grows the buffers and
+// uploads the descriptors on `stream`.
+int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_graph* graphs, const vf_frame* frames,
+                const vf_warp_params* p, bool knn_only, const vf_tuning& tuning, FrameSetup* fs,
+                hipStream_t stream) {
     int st = check_k(p->k);
+    if (st) return st;
+    st = check_tuning(tuning);
     if (st) return st;
     if (nvol < 1 || nvol > 256) return fail(VF_ERR_INVALID_ARGUMENT, "batch of %d volumes: supported 1..256", nvol);
     for (int v = 1; v < nvol; ++v)
         if (p[v].k != p[0].k) return fail(VF_ERR_INVALID_ARGUMENT, "batched volumes must share k");
+    const int shape = knn_only ? 0 : tuning.brick_shape;
     ws->vols_h.resize(nvol);
     ws->vptr_h.resize(nvol);
     Need need;
@@ -893,7 +980,7 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         const vf_frame empty_frame = {nullptr, 0, 0, {{0}}};
         VolDesc& vd = ws->vols_h[v];
         uint64_t n;
-        st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, &vd, &n);
+        st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, shape, &vd, &n);
         if (st) return st;
         vd.node_base = (uint32_t)need.nodes;
         vd.brick_base = (uint32_t)need.bricks;
@@ -926,14 +1013,16 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
     VF_HIP(hipGetDevice(&dev));
     VF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     int per_cu = 1;
-    st = pass_a_occupancy(p->k, knn_only, &per_cu);
+    st = pass_a_occupancy(p->k, knn_only, shape, &per_cu);
     if (st) return st;
-    const uint64_t blocks_a = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * per_cu, need.bricks));
+    const int wg = tuning.wg_per_cu > 0 ? std::min(tuning.wg_per_cu, per_cu) : per_cu;
+    const uint64_t blocks_a = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * wg, need.bricks));
+    const int threads_a = kShapeDims[shape][0] * kShapeDims[shape][1] * kShapeDims[shape][2];
     // records: one arena per pass-A workgroup sized for an even share, plus a
     // shared overflow arena holding the whole bound
     uint64_t arena_cap = 0, overflow_cap = 0;
     if (!knn_only) {
-        arena_cap = (rec_bound + blocks_a - 1) / blocks_a + kPassAThreads;
+        arena_cap = (rec_bound + blocks_a - 1) / blocks_a + (uint64_t)threads_a;
         overflow_cap = rec_bound;
         if (arena_cap > 0xffffffffull || overflow_cap > 0xffffffffull)
             return fail(VF_ERR_CAPACITY, "record bound %llu exceeds 2^32", (unsigned long long)rec_bound);
@@ -949,7 +1038,85 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
     VF_HIP(hipMemcpyAsync(B.vols, ws->vols_h.data(), sizeof(VolDesc) * nvol, hipMemcpyHostToDevice, stream));
     VF_HIP(hipMemcpyAsync(B.vptr, ws->vptr_h.data(), sizeof(VolPtr) * nvol, hipMemcpyHostToDevice, stream));
 
-    const uint32_t parity = (uint32_t)(ws->frame & 1u);
+    fs->nvol = nvol;
+    fs->k = p->k;
+    fs->shape = shape;
+    fs->knn_only = knn_only;
+    fs->k1 = (uint32_t)k1;
+    fs->blocks_prep = std::max<uint64_t>((k1 + 255) / 256, 1024);
+    fs->blocks_a = blocks_a;
+    fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
+    fs->arena_cap = arena_cap;
+    fs->overflow_cap = overflow_cap;
+    return VF_OK;
+}
+
+template <class S>
+int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOut kout, hipEvent_t* ev,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(256), 0, stream, B.vols, fs.nvol, fs.k1,
+                       B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag, B.touched,
+                       B.brick_cap, B.tile_cap, parity_arg, B.stat, B.totals);
+    VF_LAUNCH_CHECK("k_prep_bin");
+    if (ev) VF_HIP(hipEventRecord(ev[1], stream));
+    Arena arena;
+    arena.records = B.records;
+    arena.counts = B.arena_counts;
+    arena.cap = (uint32_t)fs.arena_cap;
+    arena.n_arenas = (uint32_t)fs.blocks_a;
+    arena.overflow = B.records + fs.blocks_a * fs.arena_cap;
+    arena.overflow_cap = (uint32_t)fs.overflow_cap;
+#define VF_PASS_A(KK)                                                                                                \
+    if (fs.knn_only)                                                                                                 \
+        hipLaunchKernelGGL((k_pass_a<KK, true, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,       \
+                           B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
+                           B.touched, B.brick_cap, B.tile_cap, parity_arg, arena, B.stat, B.totals, kout);           \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_pass_a<KK, false, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,      \
+                           B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
+                           B.touched, B.brick_cap, B.tile_cap, parity_arg, arena, B.stat, B.totals, kout)
+    switch (fs.k) {
+        case 1: VF_PASS_A(1); break;
+        case 2: VF_PASS_A(2); break;
+        case 3: VF_PASS_A(3); break;
+        case 4: VF_PASS_A(4); break;
+        case 5: VF_PASS_A(5); break;
+        case 6: VF_PASS_A(6); break;
+        case 7: VF_PASS_A(7); break;
+        case 8: VF_PASS_A(8); break;
+    }
+#undef VF_PASS_A
+    VF_LAUNCH_CHECK("k_pass_a");
+    if (ev) VF_HIP(hipEventRecord(ev[2], stream));
+    if (!fs.knn_only) {
+        hipLaunchKernelGGL(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), 0, stream, B.vptr, fs.nvol, arena, B.ctrl,
+                           B.bitmap, B.vols, B.nrec, B.stat, B.totals);
+        VF_LAUNCH_CHECK("k_pass_b");
+        if (ev) VF_HIP(hipEventRecord(ev[3], stream));
+    }
+    return VF_OK;
+}
+
+// The frame's kernels on `stream`: k_prep_bin, k_pass_a, k_pass_b.  With
+// parity_arg < 0 the kernels take the frame parity from the device counter
+// (what a recorded graph needs); ev (4 events) brackets the kernels.
+int launch_kernels(vf_workspace* ws, const FrameSetup& fs, int parity_arg, KnnOut kout, hipEvent_t* ev,
+                   hipStream_t stream) {
+    Buffers& B = ws->buf;
+    switch (fs.shape) {
+        case 0: return launch_kernels_shape<Shape0>(B, fs, parity_arg, kout, ev, stream);
+        case 1: return launch_kernels_shape<Shape1>(B, fs, parity_arg, kout, ev, stream);
+        case 2: return launch_kernels_shape<Shape2>(B, fs, parity_arg, kout, ev, stream);
+        case 3: return launch_kernels_shape<Shape3>(B, fs, parity_arg, kout, ev, stream);
+        default: return launch_kernels_shape<Shape4>(B, fs, parity_arg, kout, ev, stream);
+    }
+}
+
+int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_graph* graphs, const vf_frame* frames,
+                 const vf_warp_params* p, bool knn_only, KnnOut kout, hipStream_t stream) {
+    FrameSetup fs;
+    int st = setup_frame(ws, nvol, vols, graphs, frames, p, knn_only, ws->tuning, &fs, stream);
+    if (st) return st;
     hipEvent_t* ev = nullptr;
     if (ws->timing && !knn_only) {
         if (ws->events.empty()) {
@@ -961,54 +1128,8 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
         ws->timed_frames++;
         VF_HIP(hipEventRecord(ev[0], stream));
     }
-    {
-        const uint64_t blocks = std::max<uint64_t>((k1 + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_prep_bin, dim3((unsigned)blocks), dim3(256), 0, stream, B.vols, nvol, (uint32_t)k1, B.nrec,
-                           B.brick_count, B.brick_nodes, B.active + (uint64_t)parity * B.brick_cap, B.ctrl, B.bitmap,
-                           B.tile_flag, B.touched + (uint64_t)(parity ^ 1u) * B.tile_cap, parity, B.stat, B.totals);
-        VF_LAUNCH_CHECK("k_prep_bin");
-        if (ev) VF_HIP(hipEventRecord(ev[1], stream));
-    }
-    Arena arena;
-    arena.records = B.records;
-    arena.counts = B.arena_counts;
-    arena.cap = (uint32_t)arena_cap;
-    arena.n_arenas = (uint32_t)blocks_a;
-    arena.overflow = B.records + blocks_a * arena_cap;
-    arena.overflow_cap = (uint32_t)overflow_cap;
-    {
-        uint32_t* touched_cur = B.touched + (uint64_t)parity * B.tile_cap;
-        const uint32_t* active_cur = B.active + (uint64_t)parity * B.brick_cap;
-#define VF_PASS_A(KK)                                                                                                \
-    if (knn_only)                                                                                                    \
-        hipLaunchKernelGGL((k_pass_a<KK, true>), dim3((unsigned)blocks_a), dim3(kPassAThreads), 0, stream, B.vols,   \
-                           B.nrec, B.brick_count, B.brick_nodes, active_cur, B.ctrl, B.bitmap, B.tile_flag,          \
-                           touched_cur, parity, arena, B.stat, B.totals, kout);                                      \
-    else                                                                                                             \
-        hipLaunchKernelGGL((k_pass_a<KK, false>), dim3((unsigned)blocks_a), dim3(kPassAThreads), 0, stream, B.vols,  \
-                           B.nrec, B.brick_count, B.brick_nodes, active_cur, B.ctrl, B.bitmap, B.tile_flag,          \
-                           touched_cur, parity, arena, B.stat, B.totals, kout)
-        switch (p->k) {
-            case 1: VF_PASS_A(1); break;
-            case 2: VF_PASS_A(2); break;
-            case 3: VF_PASS_A(3); break;
-            case 4: VF_PASS_A(4); break;
-            case 5: VF_PASS_A(5); break;
-            case 6: VF_PASS_A(6); break;
-            case 7: VF_PASS_A(7); break;
-            case 8: VF_PASS_A(8); break;
-        }
-#undef VF_PASS_A
-        VF_LAUNCH_CHECK("k_pass_a");
-        if (ev) VF_HIP(hipEventRecord(ev[2], stream));
-    }
-    if (!knn_only) {
-        const uint64_t blocks = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
-        hipLaunchKernelGGL(k_pass_b, dim3((unsigned)blocks), dim3(256), 0, stream, B.vptr, nvol, arena, B.ctrl,
-                           B.bitmap, B.vols, B.nrec, B.stat, B.totals);
-        VF_LAUNCH_CHECK("k_pass_b");
-        if (ev) VF_HIP(hipEventRecord(ev[3], stream));
-    }
+    st = launch_kernels(ws, fs, (int)(ws->frame & 1u), kout, ev, stream);
+    if (st) return st;
     ws->frame++;
     return VF_OK;
 }
@@ -1029,6 +1150,179 @@ int read_stats(vf_workspace* ws, bool totals, vf_frame_stats* out, hipStream_t s
     out->updated = s[kStUpdated];
     out->active_bricks = s[kStActive];
     out->overflow_bricks = s[kStSpilled];
+    return VF_OK;
+}
+
+// ---- frame plans -----------------------------------------------------------
+
+// Size class of a batch for the tuning cache: k, volume count, and the
+// log2 buckets of its voxel and node totals.
+struct TuneKey {
+    int k, n, vox_log2, node_log2;
+    bool operator<(const TuneKey& o) const {
+        if (k != o.k) return k < o.k;
+        if (n != o.n) return n < o.n;
+        if (vox_log2 != o.vox_log2) return vox_log2 < o.vox_log2;
+        return node_log2 < o.node_log2;
+    }
+};
+
+int ilog2_u64(uint64_t v) {
+    int l = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++l;
+    }
+    return l;
+}
+
+TuneKey tune_key(int n, const vf_volume* vols, const vf_graph* graphs, const vf_warp_params* p) {
+    uint64_t vox = 0, nodes = 0;
+    for (int v = 0; v < n; ++v) {
+        vox += (uint64_t)std::max(vols[v].dim.x, 0) * std::max(vols[v].dim.y, 0) * std::max(vols[v].dim.z, 0);
+        nodes += graphs[v].count;
+    }
+    return TuneKey{p[0].k, n, ilog2_u64(vox), ilog2_u64(nodes)};
+}
+
+std::mutex g_tune_mutex;
+std::map<TuneKey, vf_tuning> g_tune_cache;
+
+// Device memory and events released on scope exit (autotuner scratch).
+struct ScratchGuard {
+    std::vector<void*> ptrs;
+    vf_workspace* ws = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~ScratchGuard() {
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        if (ws) free_buffers(ws->buf);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+}  // namespace vf
+
+struct vf_plan {
+    vf_workspace ws;
+    int n = 0;
+    int flags = 0;
+    std::vector<vf_volume> vols;
+    std::vector<vf_graph> graphs;
+    std::vector<vf_frame> frames;
+    std::vector<vf_warp_params> params;
+    vf_tuning tuning = {0, 0, 0.0f};
+    vf::FrameSetup fs;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace vf {
+namespace {
+
+void plan_drop_graph(vf_plan* pl) {
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    pl->exec = nullptr;
+    pl->graph = nullptr;
+}
+
+// (Re)computes the plan's launch setup for its tuning and uploads its
+// descriptors; waits for the upload.
+int plan_setup(vf_plan* pl, hipStream_t stream) {
+    plan_drop_graph(pl);
+    int st = setup_frame(&pl->ws, pl->n, pl->vols.data(), pl->graphs.data(), pl->frames.data(), pl->params.data(),
+                         false, pl->tuning, &pl->fs, stream);
+    if (st) return st;
+    VF_HIP(hipStreamSynchronize(stream));
+    return VF_OK;
+}
+
+int plan_record_graph(vf_plan* pl) {
+    if (!pl->capture_stream) VF_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
+    VF_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int st = launch_kernels(&pl->ws, pl->fs, -1, KnnOut{nullptr, nullptr, nullptr}, nullptr, pl->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(VF_ERR_HIP, "graph capture of the frame failed: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        pl->exec = nullptr;
+        return fail(VF_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    pl->graph = g;
+    return VF_OK;
+}
+
+int plan_autotune(vf_plan* pl, int frames, vf_tuning* chosen, hipStream_t stream) {
+    frames = std::max(1, std::min(frames, 64));
+    const int n = pl->n;
+    ScratchGuard guard;
+    vf_workspace tmp;
+    VF_HIP(hipGetDevice(&tmp.device));
+    guard.ws = &tmp;
+    // scratch copies of the volumes; node positions are not written back
+    std::vector<vf_volume> vols = pl->vols;
+    std::vector<vf_graph> graphs = pl->graphs;
+    for (int v = 0; v < n; ++v) {
+        const uint64_t nv = (uint64_t)vols[v].dim.x * vols[v].dim.y * vols[v].dim.z;
+        float* t = nullptr;
+        float* w = nullptr;
+        VF_HIP(hipMalloc((void**)&t, nv * sizeof(float)));
+        guard.ptrs.push_back(t);
+        VF_HIP(hipMalloc((void**)&w, nv * sizeof(float)));
+        guard.ptrs.push_back(w);
+        VF_HIP(hipMemcpyAsync(t, pl->vols[v].tsdf, nv * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        VF_HIP(hipMemcpyAsync(w, pl->vols[v].weights, nv * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        vols[v].tsdf = t;
+        vols[v].weights = w;
+        graphs[v].points_out = nullptr;
+    }
+    VF_HIP(hipEventCreate(&guard.ev[0]));
+    VF_HIP(hipEventCreate(&guard.ev[1]));
+    vf_tuning best = {-1, 0, 0.0f};
+    for (int shape = 0; shape < kNumShapes; ++shape) {
+        int per_cu = 1;
+        int st = pass_a_occupancy(pl->params[0].k, false, shape, &per_cu);
+        if (st) return st;
+        for (int wg = 1; wg <= per_cu; ++wg) {
+            const vf_tuning t = {shape, wg, 0.0f};
+            FrameSetup fs;
+            st = setup_frame(&tmp, n, vols.data(), graphs.data(), pl->frames.data(), pl->params.data(), false, t, &fs,
+                             stream);
+            if (st) return st;
+            st = launch_kernels(&tmp, fs, -1, KnnOut{nullptr, nullptr, nullptr}, nullptr, stream);
+            if (st) return st;
+            VF_HIP(hipEventRecord(guard.ev[0], stream));
+            for (int f = 0; f < frames; ++f) {
+                st = launch_kernels(&tmp, fs, -1, KnnOut{nullptr, nullptr, nullptr}, nullptr, stream);
+                if (st) return st;
+            }
+            VF_HIP(hipEventRecord(guard.ev[1], stream));
+            VF_HIP(hipEventSynchronize(guard.ev[1]));
+            float ms = 0.0f;
+            VF_HIP(hipEventElapsedTime(&ms, guard.ev[0], guard.ev[1]));
+            ms /= (float)frames;
+            if (best.brick_shape < 0 || ms < best.ms_per_frame) best = vf_tuning{shape, wg, ms};
+        }
+    }
+    VF_HIP(hipStreamSynchronize(stream));
+    pl->tuning = best;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[tune_key(n, pl->vols.data(), pl->graphs.data(), pl->params.data())] = best;
+    }
+    const int st = plan_setup(pl, stream);
+    if (st) return st;
+    if (chosen) *chosen = best;
     return VF_OK;
 }
 
@@ -1122,6 +1416,98 @@ int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
     ws->timed_frames = 0;
     return VF_OK;
 }
+
+int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning) {
+    if (!ws || !tuning) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_set_tuning: NULL argument");
+    int st = check_tuning(*tuning);
+    if (st) return st;
+    ws->tuning = *tuning;
+    return VF_OK;
+}
+
+int vf_plan_create(vf_plan** out, int n, const vf_volume* volumes, const vf_graph* graphs, const vf_frame* frames,
+                   const vf_warp_params* params, const vf_tuning* tuning, int flags, vf_stream stream) {
+    if (!out) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_create: out is NULL");
+    *out = nullptr;
+    if (!volumes || !graphs || !frames || !params) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_create: NULL argument");
+    if (n < 1 || n > 256) return fail(VF_ERR_INVALID_ARGUMENT, "plan of %d volumes: supported 1..256", n);
+    if (flags & ~VF_PLAN_GRAPH) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_create: unknown flags 0x%x", flags);
+    vf_plan* pl = new vf_plan();
+    hipError_t e = hipGetDevice(&pl->ws.device);
+    if (e != hipSuccess) {
+        delete pl;
+        return fail(VF_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    }
+    pl->n = n;
+    pl->flags = flags;
+    pl->vols.assign(volumes, volumes + n);
+    pl->graphs.assign(graphs, graphs + n);
+    pl->frames.assign(frames, frames + n);
+    pl->params.assign(params, params + n);
+    if (tuning) {
+        pl->tuning = *tuning;
+    } else {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(tune_key(n, volumes, graphs, params));
+        if (it != g_tune_cache.end()) pl->tuning = it->second;
+    }
+    int st = plan_setup(pl, (hipStream_t)stream);
+    if (st) {
+        free_buffers(pl->ws.buf);
+        delete pl;
+        return st;
+    }
+    *out = pl;
+    return VF_OK;
+}
+
+int vf_plan_destroy(vf_plan* pl) {
+    if (!pl) return VF_OK;
+    plan_drop_graph(pl);
+    if (pl->capture_stream) (void)hipStreamDestroy(pl->capture_stream);
+    for (auto& e : pl->ws.events) (void)hipEventDestroy(e);
+    free_buffers(pl->ws.buf);
+    delete pl;
+    return VF_OK;
+}
+
+int vf_plan_launch(vf_plan* pl, vf_stream stream) {
+    if (!pl) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_launch: NULL plan");
+    if (pl->flags & VF_PLAN_GRAPH) {
+        if (!pl->exec) {
+            int st = plan_record_graph(pl);
+            if (st) return st;
+        }
+        VF_HIP(hipGraphLaunch(pl->exec, (hipStream_t)stream));
+        return VF_OK;
+    }
+    return launch_kernels(&pl->ws, pl->fs, -1, KnnOut{nullptr, nullptr, nullptr}, nullptr, (hipStream_t)stream);
+}
+
+int vf_plan_autotune(vf_plan* pl, int frames, vf_tuning* chosen, vf_stream stream) {
+    if (!pl) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_autotune: NULL plan");
+    return plan_autotune(pl, frames, chosen, (hipStream_t)stream);
+}
+
+int vf_plan_get_tuning(const vf_plan* pl, vf_tuning* tuning) {
+    if (!pl || !tuning) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_get_tuning: NULL argument");
+    *tuning = pl->tuning;
+    return VF_OK;
+}
+
+int vf_plan_stats(vf_plan* pl, vf_frame_stats* last, vf_frame_stats* totals, int reset, vf_stream stream) {
+    if (!pl) return fail(VF_ERR_INVALID_ARGUMENT, "vf_plan_stats: NULL plan");
+    int st = VF_OK;
+    if (last) st = read_stats(&pl->ws, false, last, (hipStream_t)stream);
+    if (!st && totals) st = read_stats(&pl->ws, true, totals, (hipStream_t)stream);
+    if (!st && reset && pl->ws.buf.totals) {
+        VF_HIP(hipMemsetAsync(pl->ws.buf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, (hipStream_t)stream));
+        VF_HIP(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return st;
+}
+
+uint64_t vf_plan_bytes(const vf_plan* pl) { return pl ? pl->ws.buf.bytes : 0; }
 
 int vf_tsdf_knn_graph(int k, uint32_t* knn_idxs, float* knn_dists, int32_t* knn_counts, const vf_float3* graph,
                       const float* graph_radii, size_t graph_size, vf_float3 voxel_size, vf_float3 volume_size,

@@ -218,6 +218,12 @@ class Workspace:
         L.call("vf_workspace_kernel_ms", self.handle, ms, C.byref(n))
         return (float(ms[0]), float(ms[1]), float(ms[2])), int(n.value)
 
+    def set_tuning(self, brick_shape: int = 0, wg_per_cu: int = 0) -> None:
+        """Brick shape (index into _lib.BRICK_SHAPES) and pass-A workgroups
+        per CU (0 = occupancy limit) for later calls; results do not change."""
+        t = L.Tuning(int(brick_shape), int(wg_per_cu), 0.0)
+        L.call("vf_workspace_set_tuning", self.handle, C.byref(t))
+
     def close(self) -> None:
         if self.handle:
             L.load().vf_workspace_destroy(self.handle)
@@ -306,3 +312,65 @@ def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list
     L.call("vf_warped_integrate_batch", ws.handle, n, vols, grs, frs, ps, _s(stream))
     if sync:
         _done(stream)
+
+
+class Plan:
+    """A fixed batch of volumes replayed every frame (vf_plan): descriptors are
+    uploaded once and, with graph=True, the three kernels run as one recorded
+    HIP graph.  The device buffers of every argument must stay alive and in
+    place; their contents may change between launches."""
+
+    def __init__(self, tsdfs, graphs, depths, intrinsics_list, k: int, delta, max_weight, cubes,
+                 write_back: bool = True, graph: bool = True, tuning: tuple | None = None,
+                 stream: Stream | None = None):
+        n = len(tsdfs)
+        self.n = n
+        self._keep = (list(tsdfs), list(graphs), list(depths))
+        self.vols = (L.Volume * n)(*[make_volume(t) for t in tsdfs])
+        self.graphs = (L.Graph * n)(*[make_graph(g, write_back) for g in graphs])
+        self.frames = (L.Frame * n)(*[make_frame(d, K) for d, K in zip(depths, intrinsics_list)])
+        deltas = delta if isinstance(delta, (list, tuple)) else [delta] * n
+        maxws = max_weight if isinstance(max_weight, (list, tuple)) else [max_weight] * n
+        self.params = (L.WarpParams * n)(*[make_params(k, d, m, c) for d, m, c in zip(deltas, maxws, cubes)])
+        self.handle = C.c_void_p(None)
+        t = None
+        if tuning is not None:
+            t = L.Tuning(int(tuning[0]), int(tuning[1]), 0.0)
+        L.call("vf_plan_create", C.byref(self.handle), n, self.vols, self.graphs, self.frames, self.params,
+               C.byref(t) if t is not None else None, L.PLAN_GRAPH if graph else 0, _s(stream))
+
+    def launch(self, stream: Stream | None = None, sync: bool = False) -> None:
+        L.call("vf_plan_launch", self.handle, _s(stream))
+        if sync:
+            _done(stream)
+
+    def autotune(self, frames: int = 3, stream: Stream | None = None) -> dict:
+        t = L.Tuning()
+        L.call("vf_plan_autotune", self.handle, int(frames), C.byref(t), _s(stream))
+        return {"brick_shape": int(t.brick_shape), "brick": L.BRICK_SHAPES[t.brick_shape],
+                "wg_per_cu": int(t.wg_per_cu), "ms_per_frame": float(t.ms_per_frame)}
+
+    def tuning(self) -> dict:
+        t = L.Tuning()
+        L.call("vf_plan_get_tuning", self.handle, C.byref(t))
+        return {"brick_shape": int(t.brick_shape), "brick": L.BRICK_SHAPES[t.brick_shape],
+                "wg_per_cu": int(t.wg_per_cu), "ms_per_frame": float(t.ms_per_frame)}
+
+    def stats(self, reset: bool = False, stream: Stream | None = None) -> tuple:
+        last, tot = L.FrameStats(), L.FrameStats()
+        L.call("vf_plan_stats", self.handle, C.byref(last), C.byref(tot), 1 if reset else 0, _s(stream))
+        return last.as_dict(), tot.as_dict()
+
+    def bytes(self) -> int:
+        return int(L.load().vf_plan_bytes(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            L.load().vf_plan_destroy(self.handle)
+            self.handle = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
