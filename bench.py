@@ -16,13 +16,23 @@ JSON line:
   cpu_baseline  the C oracle's voxel-major frame on the host cores, on a bounded
              z-slab sample of the same workload.
 
+  batching   config 4: 16 volumes of uneven size, one frame each: 16 plain
+             per-volume calls per step against one autotuned frame plan
+             replayed as a HIP graph (the same work, results identical).
+  serving    config 5: 32 resident streams, Poisson frame arrivals at 30 fps
+             each, continuous batching; p50/p99 frame latency (submission to
+             completion) and aggregate updates/s; then the same streams
+             saturated against plain per-volume calls.
+
 Usage: python bench.py [--steps K] [--warmup W] [--workload config3]
+                       [--no-cpu] [--no-batching] [--no-serving]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -47,6 +57,9 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device copy of known size (HBM counter calibration under rocprofv3)")
+    ap.add_argument("--no-batching", action="store_true", help="skip the config-4 batching measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson arrival run")
     return ap.parse_args()
 
 
@@ -100,6 +113,250 @@ def cpu_baseline(s, depth, T, seconds: float) -> dict:
     return out
 
 
+def _pct(values, q):
+    v = sorted(values)
+    if not v:
+        return None
+    i = min(len(v) - 1, max(0, int(math.ceil(q / 100.0 * len(v))) - 1))
+    return v[i]
+
+
+def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
+    """16 plain per-volume calls per step against one frame plan (autotuned,
+    replayed as a HIP graph), each on its own fresh copy of the volumes."""
+    from volumefusion_amd import _lib as L
+    from volumefusion_amd import ops
+    from volumefusion_amd.device import DeviceArray, Event, Stream
+
+    lib = L.load()
+    stream = Stream()
+    n = len(vols)
+    inputs = []  # per volume: ([depth0, depth1], [T0, T1]) on the device
+    for i, v in enumerate(vols):
+        (d0, T0), (d1, T1) = rendered[2 * i], rendered[2 * i + 1]
+        inputs.append(([DeviceArray.from_numpy(d0, stream), DeviceArray.from_numpy(d1, stream)],
+                       [DeviceArray.from_numpy(T0, stream), DeviceArray.from_numpy(T1, stream)]))
+
+    def volume_set():
+        tsdfs = [ops.DeviceTsdf(v.size, v.dim, center=v.center, delta=v.delta, stream=stream) for v in vols]
+        graphs = [ops.DeviceMotionGraph(v.nodes, rendered[2 * i][1], v.radii, stream=stream)
+                  for i, v in enumerate(vols)]
+        return tsdfs, graphs
+
+    params = [ops.make_params(v.k, v.delta, v.max_weight, v.cube) for v in vols]
+    e0, e1 = Event(), Event()
+
+    # plain: one call per volume
+    tsdfs, graphs = volume_set()
+    ws = ops.Workspace(stream=stream)
+    vol_s = [ops.make_volume(t) for t in tsdfs]
+    gr_s, fr_s = [], []
+    for i in range(n):
+        pair_g, pair_f = [], []
+        for f in range(2):
+            g = ops.make_graph(graphs[i], write_back=True)
+            g.transforms = inputs[i][1][f].ptr
+            pair_g.append(g)
+            pair_f.append(ops.make_frame(inputs[i][0][f], vols[i].intrinsics()))
+        gr_s.append(pair_g)
+        fr_s.append(pair_f)
+    fn = lib.vf_warped_integrate
+
+    def plain_step(k):
+        for i in range(n):
+            L.check(fn(ws.handle, C.byref(vol_s[i]), C.byref(gr_s[i][k % 2]), C.byref(fr_s[i][k % 2]),
+                       C.byref(params[i]), stream.ptr), "vf_warped_integrate")
+
+    for k in range(warmup):
+        plain_step(k)
+    stream.synchronize()
+    ws.totals(reset=True, stream=stream)
+    e0.record(stream)
+    for k in range(steps):
+        plain_step(warmup + k)
+    e1.record(stream)
+    stream.synchronize()
+    plain_ms = e0.elapsed_ms(e1) / steps
+    plain_updates = ws.totals(stream=stream)["updated"] / steps
+    del tsdfs, graphs, ws
+
+    # batched: one plan per input parity over a second fresh set; the first
+    # is autotuned, the second starts from the tuning cache
+    tsdfs, graphs = volume_set()
+    plans = []
+    tune = None
+    for f in range(2):
+        plans.append(ops.Plan(tsdfs, graphs, [inputs[i][0][f] for i in range(n)], [v.intrinsics() for v in vols],
+                              vols[0].k, [v.delta for v in vols], [v.max_weight for v in vols],
+                              [v.cube for v in vols], transforms=[inputs[i][1][f] for i in range(n)], graph=True,
+                              stream=stream))
+        if f == 0:
+            tune = plans[0].autotune(frames=2, stream=stream)
+    assert plans[1].tuning()["brick_shape"] == tune["brick_shape"]
+    for k in range(warmup):
+        plans[k % 2].launch(stream)
+    stream.synchronize()
+    for p_ in plans:
+        p_.stats(reset=True, stream=stream)
+    e0.record(stream)
+    for k in range(steps):
+        plans[(warmup + k) % 2].launch(stream)
+    e1.record(stream)
+    stream.synchronize()
+    batched_ms = e0.elapsed_ms(e1) / steps
+    batched_updates = sum(p_.stats(stream=stream)[1]["updated"] for p_ in plans) / steps
+    for p_ in plans:
+        p_.close()
+    return {
+        "workload": f"config4: {n} volumes, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
+                    f"{min(v.nodes.shape[0] for v in vols)}..{max(v.nodes.shape[0] for v in vols)} nodes, k=8, "
+                    "640x576 frames",
+        "voxels": int(sum(v.n_voxels for v in vols)), "nodes": int(sum(v.nodes.shape[0] for v in vols)),
+        "plain_ms_per_step": plain_ms, "batched_ms_per_step": batched_ms, "speedup": plain_ms / batched_ms,
+        "plain_updates_per_s": plain_updates / (plain_ms * 1e-3),
+        "batched_updates_per_s": batched_updates / (batched_ms * 1e-3),
+        "updates_per_step": [plain_updates, batched_updates],
+        "tuning": {"brick": list(tune["brick"]), "wg_per_cu": tune["wg_per_cu"],
+                   "tuned_ms_per_frame": tune["ms_per_frame"]},
+        "steps": steps,
+    }
+
+
+def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int = 5) -> dict:
+    """Poisson arrivals into the serving loop at 30 fps per stream and at
+    rising rates, the streams saturated, and plain per-volume calls (each
+    frame uploaded, then integrated) on resident copies of the same streams."""
+    from volumefusion_amd import _lib as L
+    from volumefusion_amd import ops, serve
+    from volumefusion_amd.device import DeviceArray, Event, PinnedArray, Stream
+
+    n = len(vols)
+    srv = serve.Server(200 << 30, k=8, max_batch=n, lanes=2)
+    ids, keep = [], []
+    for v in vols:
+        d, kp = serve.scene_desc(v)
+        keep.append(kp)
+        ids.append(srv.open(d))
+    # every stream's two frames in page-locked memory, submitted without copies
+    pinned = [[(PinnedArray(rendered[2 * i + f][0]), PinnedArray(rendered[2 * i + f][1])) for f in range(2)]
+              for i in range(n)]
+    # warm-up: batches of every size class grow the lane workspaces
+    for r in range(3):
+        for i, sid in enumerate(ids):
+            srv.submit_borrowed(sid, *pinned[i][r % 2])
+        srv.drain()
+    srv.poll(1 << 20)
+
+    def poisson_run(rate, length, rseed):
+        rng = np.random.default_rng([seed, rseed])
+        arrivals = []
+        for i in range(n):
+            t = rng.exponential(1.0 / rate)
+            f = 1
+            while t < length:
+                arrivals.append((t, i, f))
+                t += rng.exponential(1.0 / rate)
+                f += 1
+        arrivals.sort()
+        upd0 = srv.stats()["totals"]["updated"]
+        late = []
+        t0 = time.perf_counter()
+        for t, i, f in arrivals:
+            target = t0 + t
+            while True:
+                now = time.perf_counter()
+                if now >= target:
+                    break
+                if target - now > 2e-3:
+                    time.sleep(target - now - 1e-3)
+            late.append(now - target)
+            srv.submit_borrowed(ids[i], *pinned[i][f % 2])
+        srv.drain()
+        wall = time.perf_counter() - t0
+        res = srv.poll(1 << 22)
+        lat = [r["latency_ms"] for r in res]
+        return {"fps_per_stream": rate, "offered_fps": len(arrivals) / length, "frames": len(res),
+                "achieved_fps": len(res) / wall, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                "max_ms": max(lat) if lat else None,
+                "mean_batch_frames": float(np.mean([r["batch_frames"] for r in res])) if res else 0.0,
+                "updates_per_s": (srv.stats()["totals"]["updated"] - upd0) / wall,
+                "submit_lateness_p99_ms": _pct([x * 1e3 for x in late], 99)}
+
+    poisson = poisson_run(fps, seconds, 0)
+    ramp = [poisson_run(rate, 1.0, r + 1) for r, rate in enumerate((120.0, 240.0, 480.0))]
+    within = [x for x in [poisson] + ramp if x["p99_ms"] is not None and x["p99_ms"] < 33.0]
+    capacity = max(within, key=lambda x: x["achieved_fps"]) if within else None
+
+    # saturated: every stream has frames queued
+    rounds = 8
+    upd1 = srv.stats()["totals"]["updated"]
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        for i, sid in enumerate(ids):
+            srv.submit_borrowed(sid, *pinned[i][r % 2])
+    srv.drain()
+    sat_wall = time.perf_counter() - t0
+    res = srv.poll(1 << 20)
+    st = srv.stats()
+    sat_lat = [r_["latency_ms"] for r_ in res]
+    saturated = {"frames": rounds * n, "seconds": sat_wall, "fps": rounds * n / sat_wall,
+                 "updates_per_s": (st["totals"]["updated"] - upd1) / sat_wall,
+                 "mean_batch_frames": float(np.mean([r_["batch_frames"] for r_ in res])),
+                 "p50_ms": _pct(sat_lat, 50), "p99_ms": _pct(sat_lat, 99)}
+    hbm = {"bytes_admitted": st["bytes_in_use"], "budget": st["hbm_budget"], "streams": st["streams_open"]}
+    srv.close_server()
+
+    # plain per-volume calls: upload the frame, integrate, next volume
+    lib = L.load()
+    stream = Stream()
+    ws = ops.Workspace(stream=stream)
+    items = []
+    for i, v in enumerate(vols):
+        t = ops.DeviceTsdf(v.size, v.dim, center=v.center, delta=v.delta, stream=stream)
+        g = ops.DeviceMotionGraph(v.nodes, rendered[2 * i][1], v.radii, stream=stream)
+        d = DeviceArray.from_numpy(rendered[2 * i][0], stream)
+        items.append((t, g, d, ops.make_volume(t), ops.make_graph(g, write_back=True),
+                      ops.make_frame(d, v.intrinsics()), ops.make_params(v.k, v.delta, v.max_weight, v.cube)))
+    fn = lib.vf_warped_integrate
+    h2d = lib.vf_memcpy_h2d
+
+    def plain_round(r, upload):
+        for i, it in enumerate(items):
+            if upload:
+                dp, Tp = pinned[i][r % 2]
+                L.check(h2d(it[2].ptr, dp.ptr, dp.nbytes, stream.ptr), "vf_memcpy_h2d")
+                L.check(h2d(it[1].transforms.ptr, Tp.ptr, Tp.nbytes, stream.ptr), "vf_memcpy_h2d")
+            L.check(fn(ws.handle, C.byref(it[3]), C.byref(it[4]), C.byref(it[5]), C.byref(it[6]), stream.ptr),
+                    "vf_warped_integrate")
+
+    plain = {}
+    for upload in (True, False):
+        plain_round(0, upload)
+        stream.synchronize()
+        ws.totals(reset=True, stream=stream)
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            plain_round(r + 1, upload)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        plain["with_upload" if upload else "resident"] = {
+            "frames": rounds * n, "seconds": wall, "fps": rounds * n / wall,
+            "updates_per_s": ws.totals(stream=stream)["updated"] / wall}
+    for p_ in pinned:
+        for a, b in p_:
+            a.free()
+            b.free()
+    return {
+        "workload": f"config5: {n} streams, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
+                    f"{min(v.nodes.shape[0] for v in vols)}..{max(v.nodes.shape[0] for v in vols)} nodes, k=8, "
+                    f"640x576 frames from page-locked host memory, Poisson arrivals",
+        "poisson": poisson, "ramp": ramp,
+        "capacity_within_p99_33ms_fps": capacity["achieved_fps"] if capacity else None,
+        "saturated": saturated, "plain": plain,
+        "saturated_vs_plain": saturated["fps"] / plain["with_upload"]["fps"], "hbm": hbm,
+    }
+
+
 def main() -> int:
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -107,6 +364,22 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); the warped "
               "integration path has no multi-GPU mode in this build", file=sys.stderr)
         return 2
+
+    from volumefusion_amd import scenes
+
+    # host-side inputs first: rendering forks worker processes, which must
+    # happen before this process touches the device
+    s = workload(args.workload)
+    c4 = c5 = None
+    jobs = [(s, 0), (s, 1)]
+    if not args.no_batching:
+        c4 = scenes.config4()
+        jobs += [(v, f) for v in c4 for f in (0, 1)]
+    if not args.no_serving:
+        c5 = scenes.config5()
+        jobs += [(v, f) for v in c5 for f in (0, 1)]
+    rendered = scenes.render_frames(jobs)
+    print(f"inputs ready ({len(jobs)} frames)", file=sys.stderr, flush=True)
 
     from volumefusion_amd import _lib as L
     from volumefusion_amd import device, ops
@@ -117,9 +390,8 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
-    s = workload(args.workload)
-    depths = [s.depth(f) for f in range(2)]
-    Ts = [s.transforms(f) for f in range(2)]
+    depths = [rendered[0][0], rendered[1][0]]
+    Ts = [rendered[0][1], rendered[1][1]]
     H, W = depths[0].shape
     G = s.nodes.shape[0]
 
@@ -162,11 +434,11 @@ def main() -> int:
     kms, nk = ws.kernel_ms()
     ws.set_timing(False)
     tot = ws.totals(stream=stream)
+    print(f"{s.name}: {wall / args.steps * 1e3:.3f} ms per frame", file=sys.stderr, flush=True)
 
     U = tot["updated"] / args.steps
     ab = algorithmic_bytes(U, W, H, G)
     t_a = kms[1] / nk * 1e-3
-    t_b = kms[2] / nk * 1e-3
     t_frame = sum(kms) / nk * 1e-3
     achieved = ab["k_pass_a"] / t_a
     traffic = None
@@ -185,6 +457,7 @@ def main() -> int:
         for _ in range(3):
             b.copy_from(a)
         stream.synchronize()
+        del a, b
 
     result = {
         "metric": METRIC,
@@ -220,6 +493,20 @@ def main() -> int:
         },
         "dense_equivalent_voxels_per_s": s.n_voxels * args.steps / wall,
     }
+    del tsdf, graph, ws
+    if c4 is not None:
+        off = 2
+        result["batching"] = bench_batching(c4, rendered[off:off + 2 * len(c4)], steps=20, warmup=3)
+        b = result["batching"]
+        print(f"batching: plain {b['plain_ms_per_step']:.3f} ms, batched {b['batched_ms_per_step']:.3f} ms "
+              f"per step ({b['speedup']:.2f}x)", file=sys.stderr, flush=True)
+    if c5 is not None:
+        off = 2 + (2 * len(c4) if c4 is not None else 0)
+        result["serving"] = bench_serving(c5, rendered[off:off + 2 * len(c5)], args.serve_seconds)
+        sv = result["serving"]
+        print(f"serving: p50 {sv['poisson']['p50_ms']:.2f} ms, p99 {sv['poisson']['p99_ms']:.2f} ms, "
+              f"saturated {sv['saturated']['fps']:.0f} fps vs plain {sv['plain']['with_upload']['fps']:.0f} fps",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(s, depths[0], Ts[0], args.cpu_seconds)
     else:

@@ -268,6 +268,98 @@ int vf_plan_stats(vf_plan* plan, vf_frame_stats* last, vf_frame_stats* totals, i
 uint64_t vf_plan_bytes(const vf_plan* plan);
 
 /* ------------------------------------------------------------------------ */
+/* Serving: resident reconstruction streams, continuous batching            */
+/* ------------------------------------------------------------------------ */
+/* A server keeps each open stream's TSDF volume (tsdf -1, weight 0 at open),
+ * node positions, radii, transforms and depth buffer on the device.  Frames
+ * (a host depth image and the node transforms) are submitted as they arrive;
+ * a dispatcher thread forms a batch from the streams that have a waiting
+ * frame and no frame in flight (oldest first, at most max_batch), uploads the
+ * inputs and launches one batched frame on one of `lanes` HIP streams, each
+ * with its own workspace, so that batches overlap on the device while every
+ * stream's frames stay in order.  A completion thread stamps each frame when
+ * its batch has finished.  Streams are admitted while the device memory they
+ * need fits the budget.  Per-volume results do not depend on what else is in
+ * the batch. */
+typedef struct vf_server vf_server;
+
+typedef struct vf_server_config {
+    uint64_t hbm_budget; /* bytes of device memory for streams and workspaces */
+    int32_t k;           /* knn slots, shared by every stream (1..8)          */
+    int32_t max_batch;   /* frames per launch, 1..256                         */
+    int32_t lanes;       /* batches in flight at once, 1..8                   */
+    int32_t device;      /* HIP device                                        */
+} vf_server_config;
+
+typedef struct vf_stream_desc {
+    vf_int3 dim;
+    vf_float3 size;
+    vf_float3 center;
+    int32_t width, height;    /* depth frame size                            */
+    vf_mat3 intrinsics;
+    float delta;
+    float max_weight;
+    vf_int3 voxels_per_node;
+    uint32_t nodes;           /* G                                           */
+    const vf_float3* points;  /* host Vec3f[G], copied at open               */
+    const float* radii;       /* host float[G], copied at open               */
+} vf_stream_desc;
+
+typedef struct vf_frame_result {
+    uint64_t ticket;
+    int32_t stream_id;
+    int32_t status;           /* VF_OK, or the status of its batch's launch  */
+    double submit_s;          /* steady-clock seconds at submission          */
+    double issue_s;           /* its batch was handed to the device          */
+    double done_s;            /* its batch was seen complete                 */
+    uint64_t batch;           /* batch sequence number                       */
+    uint32_t batch_frames;    /* frames in that batch                        */
+    uint32_t lane;
+} vf_frame_result;
+
+typedef struct vf_server_stats {
+    uint64_t frames_submitted;
+    uint64_t frames_done;
+    uint64_t batches;
+    uint64_t bytes_in_use;    /* admitted streams' device memory estimate    */
+    uint64_t hbm_budget;
+    int32_t streams_open;
+    int32_t max_batch_seen;
+    vf_frame_stats totals;    /* kernel counters summed over every batch     */
+} vf_server_stats;
+
+/* Device bytes a stream with this description is charged at admission:
+ * its buffers plus its share of `lanes` workspaces. */
+uint64_t vf_server_stream_bytes(const vf_stream_desc* desc, int lanes);
+int vf_server_create(vf_server** server, const vf_server_config* config);
+/* Finishes every submitted frame, then frees everything. */
+int vf_server_destroy(vf_server* server);
+/* Admits a stream; when its bytes do not fit the budget, waits up to wait_ms
+ * for streams to close, then fails with VF_ERR_CAPACITY. */
+int vf_server_open(vf_server* server, const vf_stream_desc* desc, int wait_ms, int32_t* stream_id);
+/* Waits for the stream's frames, then releases its memory. */
+int vf_server_close(vf_server* server, int32_t stream_id);
+/* Queues one frame: depth float[W*H] and transforms Mat4f[G] (host memory,
+ * copied before the call returns). */
+int vf_server_submit(vf_server* server, int32_t stream_id, const float* depth, const vf_mat4* transforms,
+                     uint64_t* ticket);
+/* The same without the copy: depth and transforms are read where they are
+ * (pinned host or device memory) when the frame's batch is issued, so the
+ * caller leaves them untouched until the frame is done. */
+int vf_server_submit_borrowed(vf_server* server, int32_t stream_id, const float* depth, const vf_mat4* transforms,
+                              uint64_t* ticket);
+/* Blocks until the frame is done. */
+int vf_server_wait(vf_server* server, uint64_t ticket, vf_frame_result* result);
+/* Completed frames not returned by an earlier poll, oldest first. */
+int vf_server_poll(vf_server* server, vf_frame_result* results, int capacity, int* count);
+/* Blocks until every submitted frame is done. */
+int vf_server_drain(vf_server* server);
+/* Copies a stream's volume and node positions to the host once its frames
+ * are done (any pointer may be NULL). */
+int vf_server_read(vf_server* server, int32_t stream_id, float* tsdf, float* weights, vf_float3* points);
+int vf_server_get_stats(vf_server* server, vf_server_stats* stats);
+
+/* ------------------------------------------------------------------------ */
 /* Device plumbing for hosts without a GPU framework (Python via ctypes)    */
 /* ------------------------------------------------------------------------ */
 int vf_device_count(int* count);

@@ -76,3 +76,22 @@ def test_voxels_per_node_matches_reference_formula():
     assert scenes.voxels_per_node(0.025, (0.004,) * 3) == (15, 15, 15)
     assert scenes.voxels_per_node(0.020, (0.002,) * 3) == (21, 21, 21)
     assert scenes.voxels_per_node(0.020, (0.003,) * 3) == (15, 15, 15)
+
+
+def test_server_admission_bytes_formula():
+    """The admission charge grows with the volume, the graph and the lanes
+    (pure host arithmetic, no device needed)."""
+    import numpy as np
+
+    from volumefusion_amd import serve
+
+    def charge(dim, G, lanes):
+        d, keep = serve.stream_desc((dim,) * 3, (0.5,) * 3, (0, 0, 1), 640, 576, np.eye(3).reshape(-1), 0.01, 64.0,
+                                    (11, 11, 11), np.zeros((G, 3), np.float32), np.full(G, 0.02, np.float32))
+        return serve.stream_bytes(d, lanes)
+
+    base = charge(128, 512, 1)
+    assert base >= 8 * 128 ** 3 + 512 * 80 + 640 * 576 * 4
+    assert charge(256, 512, 1) > base
+    assert charge(128, 1024, 1) > base
+    assert charge(128, 512, 2) > base

@@ -93,6 +93,55 @@ class Tuning(C.Structure):
 BRICK_SHAPES = [(16, 8, 4), (16, 4, 4), (8, 8, 4), (8, 8, 8), (32, 4, 4)]  # VF_BRICK_* order
 PLAN_GRAPH = 1
 
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("k", C.c_int32), ("max_batch", C.c_int32), ("lanes", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class StreamDesc(C.Structure):
+    _fields_ = [
+        ("dim", Int3),
+        ("size", Float3),
+        ("center", Float3),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("intrinsics", Mat3),
+        ("delta", C.c_float),
+        ("max_weight", C.c_float),
+        ("voxels_per_node", Int3),
+        ("nodes", C.c_uint32),
+        ("points", C.c_void_p),
+        ("radii", C.c_void_p),
+    ]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("stream_id", C.c_int32),
+        ("status", C.c_int32),
+        ("submit_s", C.c_double),
+        ("issue_s", C.c_double),
+        ("done_s", C.c_double),
+        ("batch", C.c_uint64),
+        ("batch_frames", C.c_uint32),
+        ("lane", C.c_uint32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("frames_submitted", C.c_uint64),
+        ("frames_done", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("bytes_in_use", C.c_uint64),
+        ("hbm_budget", C.c_uint64),
+        ("streams_open", C.c_int32),
+        ("max_batch_seen", C.c_int32),
+        ("totals", FrameStats),
+    ]
+
 vp = C.c_void_p
 i32 = C.c_int32
 u32 = C.c_uint32
@@ -136,6 +185,18 @@ _SIGS = {
     "vf_plan_get_tuning": [vp, C.POINTER(Tuning)],
     "vf_plan_stats": [vp, C.POINTER(FrameStats), C.POINTER(FrameStats), C.c_int, vp],
     "vf_plan_bytes": [vp],
+    "vf_server_stream_bytes": [C.POINTER(StreamDesc), C.c_int],
+    "vf_server_create": [C.POINTER(vp), C.POINTER(ServerConfig)],
+    "vf_server_destroy": [vp],
+    "vf_server_open": [vp, C.POINTER(StreamDesc), C.c_int, C.POINTER(C.c_int32)],
+    "vf_server_close": [vp, C.c_int32],
+    "vf_server_submit": [vp, C.c_int32, vp, vp, C.POINTER(C.c_uint64)],
+    "vf_server_submit_borrowed": [vp, C.c_int32, vp, vp, C.POINTER(C.c_uint64)],
+    "vf_server_wait": [vp, C.c_uint64, C.POINTER(FrameResult)],
+    "vf_server_poll": [vp, C.POINTER(FrameResult), C.c_int, C.POINTER(C.c_int)],
+    "vf_server_drain": [vp],
+    "vf_server_read": [vp, C.c_int32, vp, vp, vp],
+    "vf_server_get_stats": [vp, C.POINTER(ServerStats)],
     "vf_device_count": [C.POINTER(C.c_int)],
     "vf_set_device": [C.c_int],
     "vf_malloc": [C.POINTER(vp), u64],
@@ -163,6 +224,7 @@ _RESTYPES = {
     "vf_version": C.c_char_p,
     "vf_workspace_bytes": u64,
     "vf_plan_bytes": u64,
+    "vf_server_stream_bytes": u64,
 }
 
 _lib: C.CDLL | None = None

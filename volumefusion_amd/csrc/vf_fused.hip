@@ -809,16 +809,25 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
         need.records <= b.record_cap && need.nvol <= b.vol_cap && need.arena_slots <= b.arena_slots)
         return VF_OK;
     if (b.ctrl) VF_HIP(hipStreamSynchronize(stream));
+    // a capacity that has to grow gets half again as much headroom, so that
+    // batches of varying size settle after a few frames
+    const auto grow = [](uint64_t want, uint64_t have) {
+        return std::max<uint64_t>(want <= have ? have : std::max(want, have + have / 2), 1);
+    };
     Buffers nbuf;
-    nbuf.node_cap = std::max<uint64_t>(std::max(need.nodes, b.node_cap), 1);
-    nbuf.brick_cap = std::max<uint64_t>(std::max(need.bricks, b.brick_cap), 1);
-    nbuf.tile_cap = std::max<uint64_t>(std::max(need.tiles, b.tile_cap), 1);
-    nbuf.record_cap = std::max<uint64_t>(std::max(need.records, b.record_cap), 1);
-    nbuf.vol_cap = std::max<uint64_t>(std::max(need.nvol, b.vol_cap), 1);
-    nbuf.arena_slots = std::max<uint64_t>(std::max(need.arena_slots, b.arena_slots), 1);
+    nbuf.node_cap = grow(need.nodes, b.node_cap);
+    nbuf.brick_cap = grow(need.bricks, b.brick_cap);
+    nbuf.tile_cap = grow(need.tiles, b.tile_cap);
+    nbuf.record_cap = grow(need.records, b.record_cap);
+    nbuf.vol_cap = grow(need.nvol, b.vol_cap);
+    nbuf.arena_slots = grow(need.arena_slots, b.arena_slots);
+    // the running totals survive the reallocation
+    unsigned long long* totals = b.totals;
+    b.totals = nullptr;
     free_buffers(b);
     int st = VF_OK;
-    uint64_t bytes = 0;
+    uint64_t bytes = totals ? (uint64_t)kStCount * kStatSlots * 8 : 0;
+    nbuf.totals = totals;
     if (!st) st = dev_alloc(&nbuf.ctrl, 1, &bytes);
     if (!st) st = dev_alloc(&nbuf.nrec, nbuf.node_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.brick_count, nbuf.brick_cap, &bytes);
@@ -830,7 +839,10 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     if (!st) st = dev_alloc(&nbuf.records, nbuf.record_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.arena_counts, nbuf.arena_slots, &bytes);
     if (!st) st = dev_alloc(&nbuf.stat, (uint64_t)kStCount * kStatSlots, &bytes);
-    if (!st) st = dev_alloc(&nbuf.totals, (uint64_t)kStCount * kStatSlots, &bytes);
+    if (!st && !nbuf.totals) {
+        st = dev_alloc(&nbuf.totals, (uint64_t)kStCount * kStatSlots, &bytes);
+        if (!st) VF_HIP(hipMemsetAsync(nbuf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
+    }
     if (!st) st = dev_alloc(&nbuf.vols, nbuf.vol_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.vptr, nbuf.vol_cap, &bytes);
     if (st) {
@@ -846,7 +858,6 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     VF_HIP(hipMemsetAsync(nbuf.tile_flag, 0, nbuf.tile_cap * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.arena_counts, 0, nbuf.arena_slots * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.stat, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
-    VF_HIP(hipMemsetAsync(nbuf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
     VF_HIP(hipStreamSynchronize(stream));
     b = nbuf;
     ws->frame = 0;

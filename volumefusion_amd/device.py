@@ -112,6 +112,35 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """Page-locked host memory (vf_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self.nbytes = max(int(a.nbytes), 1)
+        self._ptr = C.c_void_p(None)
+        L.call("vf_host_alloc", C.byref(self._ptr), self.nbytes)
+        buf = (C.c_char * self.nbytes).from_address(self._ptr.value)
+        self.array = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape)
+        self.array[...] = a
+
+    @property
+    def ptr(self) -> int:
+        return int(self._ptr.value or 0)
+
+    def free(self) -> None:
+        if self._ptr:
+            self.array = None
+            L.load().vf_host_free(self._ptr)
+            self._ptr = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Event:
     def __init__(self):
         self.handle = C.c_void_p(None)

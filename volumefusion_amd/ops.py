@@ -322,12 +322,18 @@ class Plan:
 
     def __init__(self, tsdfs, graphs, depths, intrinsics_list, k: int, delta, max_weight, cubes,
                  write_back: bool = True, graph: bool = True, tuning: tuple | None = None,
-                 stream: Stream | None = None):
+                 transforms=None, stream: Stream | None = None):
+        """transforms: optional per-volume device arrays used instead of the
+        graphs' own transform buffers."""
         n = len(tsdfs)
         self.n = n
-        self._keep = (list(tsdfs), list(graphs), list(depths))
+        self._keep = (list(tsdfs), list(graphs), list(depths), list(transforms or []))
         self.vols = (L.Volume * n)(*[make_volume(t) for t in tsdfs])
-        self.graphs = (L.Graph * n)(*[make_graph(g, write_back) for g in graphs])
+        grs = [make_graph(g, write_back) for g in graphs]
+        if transforms is not None:
+            for gr, T in zip(grs, transforms):
+                gr.transforms = T.ptr
+        self.graphs = (L.Graph * n)(*grs)
         self.frames = (L.Frame * n)(*[make_frame(d, K) for d, K in zip(depths, intrinsics_list)])
         deltas = delta if isinstance(delta, (list, tuple)) else [delta] * n
         maxws = max_weight if isinstance(max_weight, (list, tuple)) else [max_weight] * n

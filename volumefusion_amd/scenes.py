@@ -160,14 +160,17 @@ def render_depth(cam: Camera, surfaces: list[Surface], rng: np.random.Generator,
 
 
 def poisson_disk(surfaces: list[Surface], n: int, rng: np.random.Generator, rmin: float | None = None) -> np.ndarray:
-    """n points on the surfaces, pairwise >= rmin apart (dart throwing; rmin
-    shrinks by 10% whenever 30 candidate batches add nothing)."""
+    """n points on the surfaces, pairwise >= rmin apart (dart throwing in
+    batches of 4096 candidates, each accepted in order when no accepted point
+    lies closer than rmin; rmin shrinks by 10% whenever 30 batches add
+    nothing)."""
+    from scipy.spatial import cKDTree
+
     area = sum(s.area() for s in surfaces)
     if rmin is None:
         rmin = 0.8 * math.sqrt(area / max(n, 1))
     weights = np.array([s.area() for s in surfaces]) / area
-    pts: list = []
-    grid: dict = {}
+    pts = np.empty((0, 3))
     stale = 0
     while len(pts) < n:
         which = rng.choice(len(surfaces), size=4096, p=weights)
@@ -178,41 +181,29 @@ def poisson_disk(surfaces: list[Surface], n: int, rng: np.random.Generator, rmin
             m = which == i
             if m.any():
                 cand[m] = s.point(u[m])
-        before = len(pts)
-        r2 = rmin * rmin
-        cells = np.floor(cand / rmin).astype(np.int64)
-        for c, (cx, cy, cz) in zip(cand, cells.tolist()):
-            if len(pts) >= n:
-                break
-            ok = True
-            for dz in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dx in (-1, 0, 1):
-                        for q in grid.get((cx + dx, cy + dy, cz + dz), ()):
-                            d = c - q
-                            if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2:
-                                ok = False
-                                break
-                        if not ok:
-                            break
-                    if not ok:
-                        break
-                if not ok:
+        # against the points accepted so far
+        if len(pts):
+            d, _ = cKDTree(pts).query(cand, k=1, distance_upper_bound=rmin)
+            cand = cand[~(d < rmin)]
+        # within the batch, in candidate order
+        accepted = []
+        if len(cand):
+            near = cKDTree(cand).query_ball_point(cand, rmin * (1.0 - 1e-12))
+            taken = np.zeros(len(cand), bool)
+            for i in range(len(cand)):
+                if len(pts) + len(accepted) >= n:
                     break
-            if ok:
-                pts.append(c)
-                grid.setdefault((cx, cy, cz), []).append(c)
-        if len(pts) == before:
+                if not any(taken[j] for j in near[i] if j < i):
+                    taken[i] = True
+                    accepted.append(i)
+        if accepted:
+            pts = np.concatenate([pts, cand[accepted]])
+            stale = 0
+        else:
             stale += 1
             if stale >= 30:
                 rmin *= 0.9
-                r2 = rmin * rmin
                 stale = 0
-                grid = {}
-                for q in pts:
-                    grid.setdefault(tuple(np.floor(q / rmin).astype(np.int64).tolist()), []).append(q)
-        else:
-            stale = 0
     return np.array(pts[:n], np.float32)
 
 
@@ -414,3 +405,24 @@ def small_scene(seed: int = 0, dim=(40, 36, 44), nodes: int = 48, k: int = 4, ra
     return VolumeScene("small", tuple(dim), size, c, cam, pts, np.full(nodes, radius, np.float32), radius, k, 0.012,
                        64.0, [s], seed, noise_sigma=0.001, warp_angle_deg=5.0 if warps else 0.0,
                        warp_translation=0.01 if warps else 0.0)
+
+
+def _render_job(job):
+    s, frame = job
+    return s.depth(frame), s.transforms(frame)
+
+
+def render_frames(jobs: list, workers: int | None = None) -> list:
+    """(depth, transforms) for each (scene, frame) in `jobs`, rendered on a
+    pool of forked CPU processes.  Call it before any device work: the pool
+    forks this process."""
+    import concurrent.futures
+    import multiprocessing
+    import os
+
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(jobs) <= 1:
+        return [_render_job(j) for j in jobs]
+    ctx = multiprocessing.get_context("fork")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+        return list(ex.map(_render_job, jobs, chunksize=1))
