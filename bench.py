@@ -60,6 +60,7 @@ def parse_args():
     ap.add_argument("--no-batching", action="store_true", help="skip the config-4 batching measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson arrival run")
+    ap.add_argument("--no-tune", action="store_true", help="default brick shape instead of the autotuned one")
     return ap.parse_args()
 
 
@@ -416,6 +417,18 @@ def main() -> int:
         L.check(fn(ws.handle, C.byref(vol), C.byref(graphs[i % 2]), C.byref(frames[i % 2]), C.byref(params),
                    stream.ptr), "vf_warped_integrate")
 
+    # brick shape and workgroups per CU from the autotuner (scratch copies of
+    # the volume; results do not depend on the choice)
+    tuning = {"brick": [16, 8, 4], "wg_per_cu": 0, "tuned_ms_per_frame": None}
+    if not args.no_tune:
+        plan = ops.Plan([tsdf], [graph], [d_dev[0]], [s.intrinsics()], s.k, s.delta, s.max_weight, [s.cube],
+                        transforms=[T_dev[0]], graph=False, stream=stream)
+        t = plan.autotune(frames=3, stream=stream)
+        plan.close()
+        ws.set_tuning(t["brick_shape"], t["wg_per_cu"])
+        tuning = {"brick": list(t["brick"]), "wg_per_cu": t["wg_per_cu"], "tuned_ms_per_frame": t["ms_per_frame"]}
+        print(f"autotuned: brick {t['brick']}, {t['wg_per_cu']} workgroups per CU", file=sys.stderr, flush=True)
+
     for i in range(args.warmup):
         step(i)
     stream.synchronize()
@@ -479,6 +492,7 @@ def main() -> int:
             "updates_per_frame": U, "supported_per_frame": tot["supported"] / args.steps,
             "active_bricks_per_frame": tot["active_bricks"] / args.steps,
             "collided_per_frame": tot["collided"] / args.steps,
+            "tuning": tuning,
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_pass_a", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,

@@ -29,9 +29,10 @@
  *    (src/TSDFWarp.cu:61-67), so its slot order is a race; its chosen set equals
  *    this one whenever a voxel has at most K candidates.
  *  - exp: the reference calls the CUDA device exp (src/TSDFWarp.cu:17).  Here
- *    e^x is evaluated by vfo_expw below (Cody-Waite reduction, degree-11 Taylor
- *    polynomial in binary64, one final rounding to binary32); the HIP kernels
- *    evaluate the same algorithm so both agree bit for bit.
+ *    e^x is evaluated by vfo_expw below (Tang's table-driven method in
+ *    binary64, one final rounding to binary32); the HIP kernels evaluate the
+ *    same algorithm so both agree bit for bit.  Its last bit against the
+ *    reference's device exp is not pinned (that library is not available).
  *  - pow(radius, 2) is radius*radius in float (the device overload of pow for
  *    (float, int)).
  */

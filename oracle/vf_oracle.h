@@ -12,6 +12,46 @@
  * rounded on its own (no fused multiply-add contraction).  Vectors are packed
  * float triples (Eigen::Vector3f, 12 bytes); 4x4 and 3x3 matrices are
  * column-major float arrays (Eigen default storage).
+ *
+ * What pins each function (tests/test_oracle_pins.py unless noted):
+ *   vfo_expw              glibc exp rounded to binary32 on the weight domain
+ *                         and a wide range (0 mismatches), special values.
+ *                         Against the reference's device exp (the CUDA math
+ *                         library, absent here): parity unpinned.
+ *   vfo_weight_tsdf       closed forms: d = 0 -> 1, d = r -> e^-1/2, d = 2r -> e^-2
+ *   vfo_weight_mesh_host, vfo_mesh_warp
+ *                         MotionGraphTest.Warp golden vectors
+ *                         (test/src/MotionGraphTest.cpp:98-133), bit for bit
+ *   vfo_div_markstein     IEEE division, exhaustively over mantissas
+ *   vfo_mat3_inverse      M * inverse = I; cofactor order of Eigen's inverse
+ *   vfo_translate, vfo_generate_vertices
+ *                         closed forms; the translate round trip is not the
+ *                         identity (a reference quirk the tests keep)
+ *   vfo_knn_graph         brute force over every (voxel, node) pair with the
+ *                         float decisions replayed, k = 1, 3, 4, 8; cube
+ *                         truncation, boundary distances and ties
+ *   vfo_warp_weights      closed forms (sum 1, one node -> 1)
+ *   vfo_warp_tsdf         closed forms (k = 1 -> T p, identity -> p); Eigen's
+ *                         summation order
+ *   vfo_count_collisions, vfo_integrate_warped
+ *                         identity warp with k = 1 equals plain integration
+ *                         except on the voxels a numpy float32 round trip
+ *                         predicts to collide or leave the volume
+ *   vfo_integrate_plain   fronto-parallel plane closed form, pixel rounding,
+ *                         running average and weight cap
+ *   vfo_integrate_camera_frame
+ *                         identity pose equals vfo_integrate_plain
+ *   vfo_tsdfvolume_integrate
+ *                         equals the device semantics on one frame within
+ *                         4 ulps (the reference's own GPU-vs-CPU test,
+ *                         test/src/TSDFIntegrationTest.cu:56-59)
+ *   vfo_warped_frame      equals the independently structured voxel-major
+ *                         vfo_warped_frame_gather over several frames; every
+ *                         stage above
+ *   vfo_warped_frame_f64  reporting only (max/RMS error of the float path)
+ * The knn selection when a voxel has more than K candidates is a race in the
+ * reference (src/TSDFWarp.cu:61-67); the canonical (distance, index) rule
+ * used here is pinned by brute force, not by the reference.
  */
 #ifndef VF_ORACLE_H
 #define VF_ORACLE_H

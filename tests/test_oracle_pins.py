@@ -435,3 +435,61 @@ def test_frame_f64_error_small(oracle):
     unit = float(np.spacing(F(0.6))) / s.delta
     assert err.max() <= 4 * unit
     assert math.sqrt(float(np.mean(err ** 2))) <= unit
+
+
+def test_identity_warp_k1_equals_plain_integration_except_collisions(oracle):
+    """With one knn slot and identity transforms the warp is exact (w = 1,
+    T p = p), so the warped frame integrates each voxel at the point the plain
+    kernel uses, fl(p + c) (src/TSDFIntegration.cu:86-93 vs :197-226).  The
+    voxels the warped frame skips are exactly those whose target,
+    worldToVoxel(fl(fl(p + c) - c)) computed here in numpy float32, is outside
+    or shared with another supported voxel (the count > 1 rule,
+    src/TSDFIntegration.cu:214-223)."""
+    from volumefusion_amd import scenes
+
+    s = scenes.small_scene(seed=9, k=1, nodes=80)
+    n = s.n_voxels
+    T = scenes.identity_transforms(s.nodes.shape[0])
+    tw, ww = np.full(n, -1, F), np.zeros(n, F)
+    r = oracle.warped_frame(1, tw, ww, s.nodes.copy(), s.radii, T, s.dim, s.voxel_size, s.size, s.cube, s.center,
+                            s.depth(0), s.delta, s.intrinsics(), s.max_weight, want=("cnt",))
+    tp, wp, _ = oracle.integrate_plain(np.full(n, -1, F), np.zeros(n, F), s.depth(0), s.delta, s.dim, s.voxel_size,
+                                       s.size, np.eye(3, dtype=F).reshape(9), np.array(s.center, F), s.intrinsics(),
+                                       s.max_weight)
+    X, Y, Z = s.dim
+    vs = np.array(s.voxel_size, F)
+    halfs = (np.array(s.size, F) * F(0.5)).astype(F)
+    inv = np.array([F(1.0 / np.float64(v)) for v in vs], F)
+    c = np.array(s.center, F)
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    v = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(F)
+    p = (v * vs).astype(F) - halfs
+    q = ((p + c).astype(F) - c).astype(F)
+    u = np.trunc(((q + halfs).astype(F) * inv).astype(F)).astype(np.int64)
+    inb = np.all((u >= 0) & (u < np.array(s.dim)), axis=1)
+    sup = r.cnt > 0
+    lin = (u[:, 2] * Y + u[:, 1]) * X + u[:, 0]
+    hits = np.bincount(lin[sup & inb], minlength=n)
+    ok = sup & inb & (hits[np.clip(lin, 0, n - 1)] == 1)
+    assert ok.sum() > 1000 and (sup & ~ok).sum() > 0, "the scene must exercise both cases"
+    assert r.stats["in_bounds"] == int((sup & inb).sum())
+    np.testing.assert_array_equal(tw[ok], tp[ok])
+    np.testing.assert_array_equal(ww[ok], wp[ok])
+    assert (tw[~ok] == -1).all() and (ww[~ok] == 0).all()
+
+
+def test_camera_frame_identity_pose_equals_plain(oracle):
+    """integrateTSDFFromCameraFrame with world->camera = I, 0
+    (src/TSDFIntegration.cu:118-148) is the plain kernel."""
+    dim, vs, size, K = _plane_setup()
+    rng = np.random.default_rng(3)
+    depth = (1.0 + 0.05 * rng.standard_normal((64, 64))).astype(F)
+    n = 16 ** 3
+    I = np.eye(3, dtype=F).reshape(9)
+    c = np.array([0.01, -0.02, 1.0], F)
+    a = oracle.integrate_plain(np.full(n, -1, F), np.zeros(n, F), depth, 0.03, dim, vs, size, I, c, K, 15.0)
+    b = oracle.integrate_camera_frame(np.full(n, -1, F), np.zeros(n, F), depth, 0.03, dim, vs, size, I,
+                                      np.zeros(3, F), I, c, K, 15.0)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2] and a[2] > 100
