@@ -192,7 +192,7 @@ void runWarpedTsdfIntegration(vf_workspace* workspace, float* tsdf, float* tsdfW
                               Eigen::Vector3f* motionGraph, const Eigen::Matrix4f* motionGraphTransforms,
                               const float* motionGraphRadiuses, std::size_t graphSize, const float* depthFrame,
                               const Eigen::Vector2i frameDims, const Eigen::Matrix3f depthIntrinsics, float delta,
-                              float motionGraphRadius, float maxWeight = 15.0f) {
+                              float motionGraphRadius, float maxWeight = 15.0f, int blend = VF_BLEND_LBS) {
     using namespace vf_detail;
     const Eigen::Vector3f voxelSize = volSize.cwiseQuotient(volDim.cast<float>());
     // src/VolumeFusion.cu:702-705
@@ -202,7 +202,7 @@ void runWarpedTsdfIntegration(vf_workspace* workspace, float* tsdf, float* tsdfW
     vf_graph graph{reinterpret_cast<const vf_float3*>(motionGraph), reinterpret_cast<vf_float3*>(motionGraph),
                    reinterpret_cast<const vf_mat4*>(motionGraphTransforms), motionGraphRadiuses, graphSize};
     vf_frame frame{depthFrame, frameDims[0], frameDims[1], m3(depthIntrinsics)};
-    vf_warp_params params{K, delta, maxWeight, i3(perNode)};
+    vf_warp_params params{K, delta, maxWeight, i3(perNode), blend};
     done(vf_warped_integrate(workspace, &vol, &graph, &frame, &params, nullptr), "runWarpedTsdfIntegration");
 }
 

@@ -161,11 +161,18 @@ typedef struct vf_frame {
     vf_mat3 intrinsics;
 } vf_frame;
 
+/* Blend of the selected nodes' transforms.  VF_BLEND_LBS is the reference's
+ * linear blend of Mat4f (src/TSDFWarp.cu:183-189).  VF_BLEND_DQB blends each
+ * node's rigid part as a unit dual quaternion (Kavan et al., ACM TOG 27(4),
+ * 2008); it has no counterpart in the reference. */
+enum { VF_BLEND_LBS = 0, VF_BLEND_DQB = 1 };
+
 typedef struct vf_warp_params {
     int k;                    /* knn slots, 1..8                           */
     float delta;              /* truncation distance                       */
     float max_weight;         /* weight cap (reference: 15)                */
     vf_int3 voxels_per_node;  /* node cube, 2*(int(R/vs)+1)+1 per axis     */
+    int32_t blend;            /* VF_BLEND_LBS (default) or VF_BLEND_DQB    */
 } vf_warp_params;
 
 typedef struct vf_frame_stats {
@@ -303,6 +310,7 @@ typedef struct vf_stream_desc {
     uint32_t nodes;           /* G                                           */
     const vf_float3* points;  /* host Vec3f[G], copied at open               */
     const float* radii;       /* host float[G], copied at open               */
+    int32_t blend;            /* VF_BLEND_LBS or VF_BLEND_DQB                */
 } vf_stream_desc;
 
 typedef struct vf_frame_result {

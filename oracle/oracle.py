@@ -110,15 +110,20 @@ def lib() -> C.CDLL:
                                     C.c_int]
         L.vfo_warped_frame.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol), _i32p,
                                        _f32p, _f32p, C.c_int, C.c_int, C.c_float, _f32p, C.c_float, C.POINTER(Stats),
-                                       C.POINTER(FrameOut)]
+                                       C.POINTER(FrameOut), C.c_int]
         L.vfo_warped_frame.restype = C.c_int
         L.vfo_warped_frame_gather.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_size_t,
                                               C.POINTER(Vol), _i32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float,
-                                              _f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(Stats)]
+                                              _f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(Stats), C.c_int]
         L.vfo_warped_frame_gather.restype = C.c_int
         L.vfo_warped_frame_f64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol),
-                                           _f32p, _f32p, C.c_int, C.c_float, C.c_float, _i32p, _u32p, _i32p]
+                                           _f32p, _f32p, C.c_int, C.c_float, C.c_float, _i32p, _u32p, _i32p,
+                                           C.c_int]
         L.vfo_warped_frame_f64.restype = C.c_int
+        L.vfo_mat4_to_dq.argtypes = [_f32p, _f32p]
+        L.vfo_mat4_to_dq_f64.argtypes = [_f32p, _f64p]
+        L.vfo_dqb_apply.argtypes = [C.POINTER(C.c_void_p), _f32p, C.c_int, _f32p, _f32p]
+        L.vfo_dqb_apply_f64.argtypes = [C.POINTER(C.c_void_p), _f64p, C.c_int, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -267,7 +272,31 @@ def mesh_warp(pts, nodes, radii, T, k, host_order=True, squared_dist=False) -> n
 
 # ---- whole frame ----------------------------------------------------------
 
-def warped_frame(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw, want=()):
+def mat4_to_dq(T16) -> np.ndarray:
+    """Dual quaternion (real xyzw, dual xyzw) of a column-major Mat4f, float32."""
+    out = np.zeros(8, np.float32)
+    lib().vfo_mat4_to_dq(_f32(T16), out)
+    return out
+
+
+def mat4_to_dq_f64(T16) -> np.ndarray:
+    out = np.zeros(8, np.float64)
+    lib().vfo_mat4_to_dq_f64(_f32(T16), out)
+    return out
+
+
+def dqb_apply(dqs, w, p, f64=False) -> np.ndarray:
+    """DQB of the dual quaternions dqs [n, 8] with weights w [n] applied to p."""
+    dt = np.float64 if f64 else np.float32
+    rows = [np.ascontiguousarray(q, dt) for q in dqs]
+    ptrs = (C.c_void_p * len(rows))(*[r.ctypes.data for r in rows])
+    out = np.zeros(3, dt)
+    fn = lib().vfo_dqb_apply_f64 if f64 else lib().vfo_dqb_apply
+    fn(ptrs, np.ascontiguousarray(w, dt), len(rows), np.ascontiguousarray(p, dt), out)
+    return out
+
+
+def warped_frame(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw, want=(), blend=0):
     """One frame of src/VolumeFusion.cu:699-742.  tsdf, wt and nodes are
     updated in place (nodes get the reference's round trip).  `want` names the
     per-voxel outputs to return: cnt, idx, dist, w, verts, col, pix."""
@@ -288,14 +317,14 @@ def warped_frame(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, de
     assert tsdf.dtype == np.float32 and wt.dtype == np.float32 and nodes.dtype == np.float32
     rc = lib().vfo_warped_frame(k, tsdf, wt, nodes, _f32(radii), _f32(T), nodes.shape[0], C.byref(v),
                                 np.ascontiguousarray(cube, np.int32), _f32(c), _f32(depth), W, H, float(delta),
-                                _f32(Kin), float(maxw), C.byref(st), C.byref(out))
+                                _f32(Kin), float(maxw), C.byref(st), C.byref(out), int(blend))
     if rc != 0:
         raise MemoryError("oracle frame allocation failed")
     return FrameResult(stats=st.as_dict(), **keep)
 
 
 def warped_frame_gather(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw,
-                        nthreads=None, z0=0, z1=0) -> dict:
+                        nthreads=None, z0=0, z1=0, blend=0) -> dict:
     """Voxel-major multi-threaded restatement of the same frame (sources in
     z in [z0, z1); z1 <= 0 = whole volume).  Updates tsdf, wt, nodes in place."""
     if nthreads is None:
@@ -307,16 +336,17 @@ def warped_frame_gather(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, de
     rc = lib().vfo_warped_frame_gather(k, tsdf, wt, nodes, _f32(radii), _f32(T), nodes.shape[0], C.byref(v),
                                        np.ascontiguousarray(cube, np.int32), _f32(c), _f32(depth), W, H,
                                        float(delta), _f32(Kin), float(maxw), int(nthreads), int(z0), int(z1),
-                                       C.byref(st))
+                                       C.byref(st), int(blend))
     if rc != 0:
         raise RuntimeError(f"oracle gather frame failed ({rc})")
     return st.as_dict()
 
 
-def warped_frame_f64(k, tsdf64, wt64, nodes_before, radii, T, dim, vs, size, c, depth, delta, maxw, cnt, idx, pix):
+def warped_frame_f64(k, tsdf64, wt64, nodes_before, radii, T, dim, vs, size, c, depth, delta, maxw, cnt, idx, pix,
+                     blend=0):
     v = make_vol(dim, vs, size)
     H, W = depth.shape
     lib().vfo_warped_frame_f64(k, tsdf64, wt64, _f32(nodes_before), _f32(radii), _f32(T), nodes_before.shape[0],
                                C.byref(v), _f32(c), _f32(depth), W, float(delta), float(maxw),
                                np.ascontiguousarray(cnt, np.int32), np.ascontiguousarray(idx, np.uint32),
-                               np.ascontiguousarray(pix, np.int32))
+                               np.ascontiguousarray(pix, np.int32), int(blend))

@@ -404,6 +404,211 @@ void vfo_warp_tsdf(int K, float* verts, const float* T, const uint32_t* idx, con
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* dual-quaternion blending (second blend mode; not in the reference)        */
+/* ------------------------------------------------------------------------ */
+/* The reference blends Mat4f node transforms linearly (LBS,
+ * src/TSDFWarp.cu:183-189).  DQB (Kavan, Collins, Zara, O'Sullivan, ACM TOG
+ * 27(4), 2008) blends each node's rigid part as a unit dual quaternion.  The
+ * float32 operation order below is the one the device evaluates (no fused
+ * multiply-add); the binary64 versions are the accuracy reference.  Dual
+ * quaternions are 8 floats: real (x, y, z, w), dual (x, y, z, w). */
+
+/* Rotation part of a column-major Mat4f (Shepperd's method, largest
+ * diagonal branch), normalised, sign chosen with w >= 0; dual part
+ * 0.5 (0, t) q. */
+void vfo_mat4_to_dq(const float* M, float* dq) {
+    const float r00 = M[0], r10 = M[1], r20 = M[2];
+    const float r01 = M[4], r11 = M[5], r21 = M[6];
+    const float r02 = M[8], r12 = M[9], r22 = M[10];
+    const float tr = (r00 + r11) + r22;
+    float x, y, z, w;
+    if (tr > 0.0f) {
+        const float s = sqrtf(tr + 1.0f) * 2.0f;
+        w = 0.25f * s;
+        x = (r21 - r12) / s;
+        y = (r02 - r20) / s;
+        z = (r10 - r01) / s;
+    } else if (r00 > r11 && r00 > r22) {
+        const float s = sqrtf(((1.0f + r00) - r11) - r22) * 2.0f;
+        w = (r21 - r12) / s;
+        x = 0.25f * s;
+        y = (r01 + r10) / s;
+        z = (r02 + r20) / s;
+    } else if (r11 > r22) {
+        const float s = sqrtf(((1.0f + r11) - r00) - r22) * 2.0f;
+        w = (r02 - r20) / s;
+        x = (r01 + r10) / s;
+        y = 0.25f * s;
+        z = (r12 + r21) / s;
+    } else {
+        const float s = sqrtf(((1.0f + r22) - r00) - r11) * 2.0f;
+        w = (r10 - r01) / s;
+        x = (r02 + r20) / s;
+        y = (r12 + r21) / s;
+        z = 0.25f * s;
+    }
+    const float nn = sqrtf(((x * x + y * y) + z * z) + w * w);
+    x = x / nn;
+    y = y / nn;
+    z = z / nn;
+    w = w / nn;
+    if (w < 0.0f) {
+        x = -x;
+        y = -y;
+        z = -z;
+        w = -w;
+    }
+    const float tx = M[12], ty = M[13], tz = M[14];
+    dq[0] = x;
+    dq[1] = y;
+    dq[2] = z;
+    dq[3] = w;
+    /* (0, t) q = (w t + t x q_v, -t . q_v) */
+    dq[4] = 0.5f * (w * tx + (ty * z - tz * y));
+    dq[5] = 0.5f * (w * ty + (tz * x - tx * z));
+    dq[6] = 0.5f * (w * tz + (tx * y - ty * x));
+    dq[7] = 0.5f * -((tx * x + ty * y) + tz * z);
+}
+
+/* Blend n dual quaternions (dq[s]: 8 floats each) with weights w[s] (the
+ * normalised knn weights), each aligned to the hemisphere of the first, and
+ * transform p:  b = sum w_s sign_s dq_s;  (r, d) = b / |b_real|;
+ * p' = p + 2 r_v x (r_v x p + r_w p) + 2 (r_w d_v - d_w r_v + r_v x d_v). */
+void vfo_dqb_apply(const float* const* dq, const float* w, int n, const float* p, float* out) {
+    float br[4] = {0.0f, 0.0f, 0.0f, 0.0f}, bd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float* f = dq[0];
+    for (int s = 0; s < n; ++s) {
+        const float* q = dq[s];
+        const float dot = ((q[0] * f[0] + q[1] * f[1]) + q[2] * f[2]) + q[3] * f[3];
+        const float ws = dot < 0.0f ? -w[s] : w[s];
+        for (int a = 0; a < 4; ++a) {
+            br[a] = br[a] + ws * q[a];
+            bd[a] = bd[a] + ws * q[4 + a];
+        }
+    }
+    const float len = sqrtf(((br[0] * br[0] + br[1] * br[1]) + br[2] * br[2]) + br[3] * br[3]);
+    if (!(len > 0.0f)) {
+        out[0] = p[0];
+        out[1] = p[1];
+        out[2] = p[2];
+        return;
+    }
+    const float il = 1.0f / len;
+    float r[4], d[4];
+    for (int a = 0; a < 4; ++a) {
+        r[a] = br[a] * il;
+        d[a] = bd[a] * il;
+    }
+    /* rotation: t1 = r_v x p; t2 = t1 + r_w p; t3 = r_v x t2; p + 2 t3 */
+    const float t1x = r[1] * p[2] - r[2] * p[1];
+    const float t1y = r[2] * p[0] - r[0] * p[2];
+    const float t1z = r[0] * p[1] - r[1] * p[0];
+    const float t2x = t1x + r[3] * p[0];
+    const float t2y = t1y + r[3] * p[1];
+    const float t2z = t1z + r[3] * p[2];
+    const float t3x = r[1] * t2z - r[2] * t2y;
+    const float t3y = r[2] * t2x - r[0] * t2z;
+    const float t3z = r[0] * t2y - r[1] * t2x;
+    /* translation: 2 ((r_w d_v - d_w r_v) + r_v x d_v) */
+    const float ux = r[1] * d[2] - r[2] * d[1];
+    const float uy = r[2] * d[0] - r[0] * d[2];
+    const float uz = r[0] * d[1] - r[1] * d[0];
+    const float trx = 2.0f * ((r[3] * d[0] - d[3] * r[0]) + ux);
+    const float try_ = 2.0f * ((r[3] * d[1] - d[3] * r[1]) + uy);
+    const float trz = 2.0f * ((r[3] * d[2] - d[3] * r[2]) + uz);
+    out[0] = (p[0] + 2.0f * t3x) + trx;
+    out[1] = (p[1] + 2.0f * t3y) + try_;
+    out[2] = (p[2] + 2.0f * t3z) + trz;
+}
+
+/* The same in binary64 (conversion and blend), the accuracy reference. */
+void vfo_mat4_to_dq_f64(const float* Mf, double* dq) {
+    double M[16];
+    for (int i = 0; i < 16; ++i)
+        M[i] = (double)Mf[i];
+    const double r00 = M[0], r10 = M[1], r20 = M[2], r01 = M[4], r11 = M[5], r21 = M[6];
+    const double r02 = M[8], r12 = M[9], r22 = M[10];
+    const double tr = r00 + r11 + r22;
+    double x, y, z, w;
+    if (tr > 0.0) {
+        const double s = sqrt(tr + 1.0) * 2.0;
+        w = 0.25 * s; x = (r21 - r12) / s; y = (r02 - r20) / s; z = (r10 - r01) / s;
+    } else if (r00 > r11 && r00 > r22) {
+        const double s = sqrt(1.0 + r00 - r11 - r22) * 2.0;
+        w = (r21 - r12) / s; x = 0.25 * s; y = (r01 + r10) / s; z = (r02 + r20) / s;
+    } else if (r11 > r22) {
+        const double s = sqrt(1.0 + r11 - r00 - r22) * 2.0;
+        w = (r02 - r20) / s; x = (r01 + r10) / s; y = 0.25 * s; z = (r12 + r21) / s;
+    } else {
+        const double s = sqrt(1.0 + r22 - r00 - r11) * 2.0;
+        w = (r10 - r01) / s; x = (r02 + r20) / s; y = (r12 + r21) / s; z = 0.25 * s;
+    }
+    const double nn = sqrt(x * x + y * y + z * z + w * w);
+    x /= nn; y /= nn; z /= nn; w /= nn;
+    if (w < 0.0) { x = -x; y = -y; z = -z; w = -w; }
+    const double tx = M[12], ty = M[13], tz = M[14];
+    dq[0] = x; dq[1] = y; dq[2] = z; dq[3] = w;
+    dq[4] = 0.5 * (w * tx + ty * z - tz * y);
+    dq[5] = 0.5 * (w * ty + tz * x - tx * z);
+    dq[6] = 0.5 * (w * tz + tx * y - ty * x);
+    dq[7] = -0.5 * (tx * x + ty * y + tz * z);
+}
+
+void vfo_dqb_apply_f64(const double* const* dq, const double* w, int n, const double* p, double* out) {
+    double br[4] = {0, 0, 0, 0}, bd[4] = {0, 0, 0, 0};
+    const double* f = dq[0];
+    for (int s = 0; s < n; ++s) {
+        const double* q = dq[s];
+        const double dot = q[0] * f[0] + q[1] * f[1] + q[2] * f[2] + q[3] * f[3];
+        const double ws = dot < 0.0 ? -w[s] : w[s];
+        for (int a = 0; a < 4; ++a) {
+            br[a] += ws * q[a];
+            bd[a] += ws * q[4 + a];
+        }
+    }
+    const double len = sqrt(br[0] * br[0] + br[1] * br[1] + br[2] * br[2] + br[3] * br[3]);
+    if (!(len > 0.0)) {
+        out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+        return;
+    }
+    double r[4], d[4];
+    for (int a = 0; a < 4; ++a) {
+        r[a] = br[a] / len;
+        d[a] = bd[a] / len;
+    }
+    const double t1[3] = {r[1] * p[2] - r[2] * p[1], r[2] * p[0] - r[0] * p[2], r[0] * p[1] - r[1] * p[0]};
+    const double t2[3] = {t1[0] + r[3] * p[0], t1[1] + r[3] * p[1], t1[2] + r[3] * p[2]};
+    const double t3[3] = {r[1] * t2[2] - r[2] * t2[1], r[2] * t2[0] - r[0] * t2[2], r[0] * t2[1] - r[1] * t2[0]};
+    const double u[3] = {r[1] * d[2] - r[2] * d[1], r[2] * d[0] - r[0] * d[2], r[0] * d[1] - r[1] * d[0]};
+    for (int a = 0; a < 3; ++a)
+        out[a] = p[a] + 2.0 * t3[a] + 2.0 * (r[3] * d[a] - d[3] * r[a] + u[a]);
+}
+
+/* warpTsdfKernel's loop (src/TSDFWarp.cu:158-190) with DQB in place of the
+ * linear blend; DQ holds each node's dual quaternion (8 floats). */
+void vfo_warp_tsdf_dqb(int K, float* verts, const float* DQ, const uint32_t* idx, const float* w, const int32_t* cnt,
+                       size_t N) {
+    for (size_t v = 0; v < N; ++v) {
+        int c = cnt[v];
+        if (!c)
+            continue;
+        if (c > K)
+            c = K;
+        const float* wv = w + (size_t)K * v;
+        if (weights_zero(wv, K))
+            continue;
+        const float* q[64];
+        for (int s = 0; s < c; ++s)
+            q[s] = DQ + 8 * (size_t)idx[(size_t)K * v + s];
+        float o[3];
+        vfo_dqb_apply(q, wv, c, verts + 3 * v, o);
+        verts[3 * v + 0] = o[0];
+        verts[3 * v + 1] = o[1];
+        verts[3 * v + 2] = o[2];
+    }
+}
+
 /* Target voxel of a warped vertex, src/TSDFIntegration.cu:272-279 and
  * :210-216: u = worldToVoxel(Rinv * (p' - t)).  Returns 0 when outside. */
 static inline int warped_target(const geom_t* g, const float* Rinv, const float* t, const float* pw, size_t* u) {
@@ -673,7 +878,8 @@ void vfo_mesh_warp(float* out, const float* pts, size_t n, const float* nodes, c
  * failure. */
 int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T, size_t G,
                      const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth, int W, int H,
-                     float delta, const float* Kin, float maxw, vfo_stats* st, const vfo_frame_out* out) {
+                     float delta, const float* Kin, float maxw, vfo_stats* st, const vfo_frame_out* out,
+                     int blend) {
     geom_t g;
     geom_init(&g, vol);
     const size_t N = g.n;
@@ -683,10 +889,13 @@ int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* r
     float* w = (float*)malloc(sizeof(float) * N * (size_t)K);
     float* verts = (float*)malloc(sizeof(float) * N * 3);
     int32_t* col = (int32_t*)calloc(N, sizeof(int32_t));
-    if (!cnt || !idx || !dist || !w || !verts || !col) {
-        free(cnt); free(idx); free(dist); free(w); free(verts); free(col);
+    float* DQ = blend ? (float*)malloc(sizeof(float) * 8 * (G ? G : 1)) : NULL;
+    if (!cnt || !idx || !dist || !w || !verts || !col || (blend && !DQ)) {
+        free(cnt); free(idx); free(dist); free(w); free(verts); free(col); free(DQ);
         return -1;
     }
+    for (size_t j = 0; blend && j < G; ++j)
+        vfo_mat4_to_dq(T + 16 * j, DQ + 8 * j);
     const float negc[3] = {-c[0], -c[1], -c[2]};
     static const float I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 
@@ -696,7 +905,10 @@ int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* r
     vfo_generate_vertices(verts, vol);                              /* :718 */
     vfo_translate(nodes, G, c);                                     /* :721 */
     vfo_translate(verts, N, c);                                     /* :723 */
-    vfo_warp_tsdf(K, verts, T, idx, w, cnt, N);                     /* :726 */
+    if (blend)
+        vfo_warp_tsdf_dqb(K, verts, DQ, idx, w, cnt, N);
+    else
+        vfo_warp_tsdf(K, verts, T, idx, w, cnt, N);                 /* :726 */
     vfo_count_collisions(col, verts, cnt, vol, I3, c);              /* :730 */
 
     /* :735-741, counting the statistics on the way */
@@ -737,7 +949,7 @@ int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* r
         if (out->verts) memcpy(out->verts, verts, sizeof(float) * N * 3);
         if (out->col) memcpy(out->col, col, sizeof(int32_t) * N);
     }
-    free(cnt); free(idx); free(dist); free(w); free(verts); free(col);
+    free(cnt); free(idx); free(dist); free(w); free(verts); free(col); free(DQ);
     return 0;
 }
 
@@ -764,6 +976,7 @@ typedef struct {
     const gnode_t* gn;
     const float* radii;
     const float* T;
+    const float* DQ;      /* per-node dual quaternions when blending with DQB */
     const float* c;
     const uint32_t* boff; /* brick list offsets */
     const uint32_t* bl;   /* brick lists        */
@@ -834,7 +1047,12 @@ static void* gather_phase1(void* arg) {
                 for (int a = 0; a < 3; ++a)
                     p[a] = p[a] + wk->c[a];
                 float acc[3] = {0.0f, 0.0f, 0.0f};
-                if (!weights_zero(w, n)) {
+                if (!weights_zero(w, n) && wk->DQ) {
+                    const float* q[64];
+                    for (int s = 0; s < n; ++s)
+                        q[s] = wk->DQ + 8 * (size_t)il[s];
+                    vfo_dqb_apply(q, w, n, p, acc);
+                } else if (!weights_zero(w, n)) {
                     for (int s = 0; s < n; ++s) {
                         float t[3];
                         mat4_apply(wk->T + 16 * (size_t)il[s], p, t);
@@ -882,7 +1100,7 @@ static void* gather_phase2(void* arg) {
 int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T,
                             size_t G, const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth,
                             int W, int H, float delta, const float* Kin, float maxw, int nthreads, int z0, int z1,
-                            vfo_stats* st) {
+                            vfo_stats* st, int blend) {
     if (K < 1 || K > 64)
         return -2;
     geom_t g;
@@ -902,10 +1120,13 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
     uint8_t* sup = (uint8_t*)calloc(N, 1);
     uint32_t* tgt = (uint32_t*)malloc(sizeof(uint32_t) * N);
     int32_t* col = (int32_t*)calloc(N, sizeof(int32_t));
-    if (!gn || !boff || !pw || !sup || !tgt || !col) {
-        free(gn); free(boff); free(pw); free(sup); free(tgt); free(col);
+    float* DQ = blend ? (float*)malloc(sizeof(float) * 8 * (G ? G : 1)) : NULL;
+    if (!gn || !boff || !pw || !sup || !tgt || !col || (blend && !DQ)) {
+        free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ);
         return -1;
     }
+    for (size_t j = 0; blend && j < G; ++j)
+        vfo_mat4_to_dq(T + 16 * j, DQ + 8 * j);
     const int valid_cube = cube[0] >= 1 && cube[1] >= 1 && cube[2] >= 1;
     /* node prep: nodes - c (src/VolumeFusion.cu:708), cube, 2r */
     for (size_t j = 0; j < G; ++j) {
@@ -960,7 +1181,7 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
             gwork_t* wk = (gwork_t*)calloc((size_t)nthreads, sizeof(gwork_t));
             for (int t = 0; t < nthreads; ++t) {
                 gwork_t* w = wk + t;
-                w->K = K; w->g = &g; w->gn = gn; w->radii = radii; w->T = T; w->c = c; w->boff = boff; w->bl = bl;
+                w->K = K; w->g = &g; w->gn = gn; w->radii = radii; w->T = T; w->DQ = DQ; w->c = c; w->boff = boff; w->bl = bl;
                 w->bdim[0] = bdim[0]; w->bdim[1] = bdim[1]; w->bdim[2] = bdim[2];
                 w->z0 = z0; w->z1 = z1; w->pw = pw; w->sup = sup; w->tgt = tgt; w->col = col;
                 w->tsdf = tsdf; w->wt = wt; w->depth = depth; w->W = W; w->H = H; w->delta = delta; w->maxw = maxw;
@@ -995,7 +1216,7 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
     for (size_t j = 0; j < G; ++j)
         for (int a = 0; a < 3; ++a)
             nodes[3 * j + a] = gn[j].m[a] + c[a];
-    free(gn); free(boff); free(pw); free(sup); free(tgt); free(col);
+    free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ);
     return 0;
 }
 
@@ -1009,7 +1230,8 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
  * positions before the frame's translations. */
 int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes, const float* radii,
                          const float* T, size_t G, const vfo_vol* vol, const float* c, const float* depth, int W,
-                         float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix) {
+                         float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix,
+                         int blend) {
     (void)G;
     geom_t g;
     geom_init(&g, vol);
@@ -1039,7 +1261,17 @@ int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes
                 double pc[3], acc[3] = {0.0, 0.0, 0.0};
                 for (int a = 0; a < 3; ++a)
                     pc[a] = p[a] + (double)c[a];
-                for (int s = 0; s < n; ++s) {
+                if (blend) {
+                    double dqs[64][8], wn[64];
+                    const double* q[64];
+                    for (int s = 0; s < n; ++s) {
+                        vfo_mat4_to_dq_f64(T + 16 * (size_t)idx[(size_t)K * v + s], dqs[s]);
+                        q[s] = dqs[s];
+                        wn[s] = w[s] / sum;
+                    }
+                    vfo_dqb_apply_f64(q, wn, n, pc, acc);
+                }
+                for (int s = 0; s < n && !blend; ++s) {
                     const float* M = T + 16 * (size_t)idx[(size_t)K * v + s];
                     for (int a = 0; a < 3; ++a) {
                         const double t = (double)M[0 * 4 + a] * pc[0] + (double)M[1 * 4 + a] * pc[1] +

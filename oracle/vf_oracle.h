@@ -49,6 +49,13 @@
  *                         vfo_warped_frame_gather over several frames; every
  *                         stage above
  *   vfo_warped_frame_f64  reporting only (max/RMS error of the float path)
+ *   vfo_mat4_to_dq, vfo_dqb_apply (DQB, no reference counterpart)
+ *                         closed forms: identity -> p bit for bit; one rigid
+ *                         node -> R p + t; identity and a rotation by theta,
+ *                         equal weights -> rotation by theta/2; +179 and -179
+ *                         degrees -> 180 (hemisphere alignment); pure
+ *                         translations blend linearly; against the binary64
+ *                         versions within a stated bound
  * The knn selection when a voxel has more than K candidates is a race in the
  * reference (src/TSDFWarp.cu:61-67); the canonical (distance, index) rule
  * used here is pinned by brute force, not by the reference.
@@ -113,7 +120,18 @@ int64_t vfo_tsdfvolume_integrate(float* tsdf, float* wt, const float* depth, int
 void vfo_mesh_warp(float* out, const float* pts, size_t n, const float* nodes, const float* radii, const float* T,
                    size_t G, int K, int host_order, int squared_dist);
 
+/* ---- dual-quaternion blending (second blend mode; not in the reference) --- */
+/* dq: 8 floats, real (x, y, z, w) then dual (x, y, z, w). */
+void vfo_mat4_to_dq(const float* M, float* dq);
+void vfo_dqb_apply(const float* const* dq, const float* w, int n, const float* p, float* out);
+void vfo_mat4_to_dq_f64(const float* M, double* dq);
+void vfo_dqb_apply_f64(const double* const* dq, const double* w, int n, const double* p, double* out);
+void vfo_warp_tsdf_dqb(int K, float* verts, const float* DQ, const uint32_t* idx, const float* w, const int32_t* cnt,
+                       size_t N);
+
 /* ---- the per-frame call sequence (src/VolumeFusion.cu:699-742) ----------- */
+/* blend: 0 = the reference's linear blend of Mat4f, 1 = DQB of each node's
+ * rigid part. */
 /* Optional per-voxel outputs of one frame; any pointer may be NULL. */
 typedef struct {
     int32_t* cnt;     /* [N]   knn counts                        */
@@ -127,16 +145,18 @@ typedef struct {
 
 int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T, size_t G,
                      const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth, int W, int H,
-                     float delta, const float* Kin, float maxw, vfo_stats* st, const vfo_frame_out* out);
+                     float delta, const float* Kin, float maxw, vfo_stats* st, const vfo_frame_out* out,
+                     int blend);
 
 int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T,
                             size_t G, const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth,
                             int W, int H, float delta, const float* Kin, float maxw, int nthreads, int z0, int z1,
-                            vfo_stats* st);
+                            vfo_stats* st, int blend);
 
 int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes, const float* radii,
                          const float* T, size_t G, const vfo_vol* vol, const float* c, const float* depth, int W,
-                         float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix);
+                         float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix,
+                         int blend);
 
 #ifdef __cplusplus
 }

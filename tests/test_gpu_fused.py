@@ -14,7 +14,7 @@ def dev(a):
     return DeviceArray.from_numpy(np.ascontiguousarray(a))
 
 
-def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None):
+def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None, blend=0):
     n = s.n_voxels
     t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
     tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
@@ -27,12 +27,13 @@ def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True
         if gather:
             st = oracle.warped_frame_gather(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size,
                                             s.cube, s.center, depth, s.delta, s.intrinsics(), s.max_weight,
-                                            nthreads=threads)
+                                            nthreads=threads, blend=blend)
         else:
             st = oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size, s.cube,
-                                     s.center, depth, s.delta, s.intrinsics(), s.max_weight).stats
+                                     s.center, depth, s.delta, s.intrinsics(), s.max_weight, blend=blend).stats
         graph.transforms.upload(T)
-        ops.warped_integrate(ws, tsdf, graph, dev(depth), s.intrinsics(), s.k, s.delta, s.max_weight, s.cube)
+        ops.warped_integrate(ws, tsdf, graph, dev(depth), s.intrinsics(), s.k, s.delta, s.max_weight, s.cube,
+                             blend=blend)
         gs = ws.last_stats()
         if check_stats:
             assert gs["supported"] == st["supported"]
@@ -116,3 +117,33 @@ def test_fused_deterministic_repeat(vf, oracle):
         out.append((tsdf.tsdf.download(), tsdf.weights.download()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("k", [1, 4, 8])
+def test_fused_dqb_multi_frame(vf, oracle, k):
+    """Dual-quaternion blending: the device frame equals the oracle's DQB
+    frame bit for bit."""
+    s = scenes.small_scene(seed=21, k=k, nodes=70)
+    st = run_frames(vf, oracle, s, [0, 1, 2], blend=1)
+    assert st["updated"] > 500
+
+
+def test_fused_dqb_spill_path(vf, oracle):
+    s = scenes.small_scene(seed=22, k=6, nodes=400, radius=0.03)
+    run_frames(vf, oracle, s, [0, 1], blend=1)
+
+
+def test_batch_mixes_blend_modes(vf, oracle):
+    """Volumes with either blend mode in one batched call."""
+    vols = [scenes.small_scene(seed=23 + i, k=8, nodes=40 + 10 * i, dim=(24 + 8 * i, 28, 24)) for i in range(4)]
+    blends = [0, 1, 1, 0]
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii) for s in vols]
+    ops.warped_integrate_batch(ops.Workspace(), tsdfs, graphs, [dev(s.depth(0)) for s in vols],
+                               [s.intrinsics() for s in vols], 8, 0.012, 64.0, [s.cube for s in vols], blend=blends)
+    for s, t, b in zip(vols, tsdfs, blends):
+        t_or, w_or, nodes_or = np.full(s.n_voxels, -1, F), np.zeros(s.n_voxels, F), s.nodes.copy()
+        oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, s.transforms(0), s.dim, s.voxel_size, s.size,
+                            s.cube, s.center, s.depth(0), s.delta, s.intrinsics(), s.max_weight, blend=b)
+        np.testing.assert_array_equal(t.tsdf.download(), t_or)
+        np.testing.assert_array_equal(t.weights.download(), w_or)

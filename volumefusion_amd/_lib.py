@@ -68,7 +68,11 @@ class Frame(C.Structure):
 
 
 class WarpParams(C.Structure):
-    _fields_ = [("k", C.c_int), ("delta", C.c_float), ("max_weight", C.c_float), ("voxels_per_node", Int3)]
+    _fields_ = [("k", C.c_int), ("delta", C.c_float), ("max_weight", C.c_float), ("voxels_per_node", Int3),
+                ("blend", C.c_int32)]
+
+
+BLEND_LBS, BLEND_DQB = 0, 1
 
 
 class FrameStats(C.Structure):
@@ -113,6 +117,7 @@ class StreamDesc(C.Structure):
         ("nodes", C.c_uint32),
         ("points", C.c_void_p),
         ("radii", C.c_void_p),
+        ("blend", C.c_int32),
     ]
 
 

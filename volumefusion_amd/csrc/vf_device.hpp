@@ -245,6 +245,109 @@ __device__ __forceinline__ void affine_apply(const float4& r0, const float4& r1,
     }
 }
 
+// Dual quaternion (real xyzw, dual xyzw) of a column-major Mat4f's rigid
+// part: Shepperd's method on the largest diagonal, normalised, w >= 0; dual
+// 0.5 (0, t) q.  A second blend mode with no reference counterpart (the
+// reference blends Mat4f linearly); every operation rounded on its own, in
+// the order the CPU restatement uses.
+__device__ __forceinline__ void mat4_to_dq(const float* M, float4* qr, float4* qd) {
+    const float r00 = M[0], r10 = M[1], r20 = M[2];
+    const float r01 = M[4], r11 = M[5], r21 = M[6];
+    const float r02 = M[8], r12 = M[9], r22 = M[10];
+    const float tr = (r00 + r11) + r22;
+    float x, y, z, w;
+    if (tr > 0.0f) {
+        const float s = __builtin_sqrtf(tr + 1.0f) * 2.0f;
+        w = 0.25f * s;
+        x = (r21 - r12) / s;
+        y = (r02 - r20) / s;
+        z = (r10 - r01) / s;
+    } else if (r00 > r11 && r00 > r22) {
+        const float s = __builtin_sqrtf(((1.0f + r00) - r11) - r22) * 2.0f;
+        w = (r21 - r12) / s;
+        x = 0.25f * s;
+        y = (r01 + r10) / s;
+        z = (r02 + r20) / s;
+    } else if (r11 > r22) {
+        const float s = __builtin_sqrtf(((1.0f + r11) - r00) - r22) * 2.0f;
+        w = (r02 - r20) / s;
+        x = (r01 + r10) / s;
+        y = 0.25f * s;
+        z = (r12 + r21) / s;
+    } else {
+        const float s = __builtin_sqrtf(((1.0f + r22) - r00) - r11) * 2.0f;
+        w = (r10 - r01) / s;
+        x = (r02 + r20) / s;
+        y = (r12 + r21) / s;
+        z = 0.25f * s;
+    }
+    const float nn = __builtin_sqrtf(((x * x + y * y) + z * z) + w * w);
+    x = x / nn;
+    y = y / nn;
+    z = z / nn;
+    w = w / nn;
+    if (w < 0.0f) {
+        x = -x;
+        y = -y;
+        z = -z;
+        w = -w;
+    }
+    const float tx = M[12], ty = M[13], tz = M[14];
+    *qr = make_float4(x, y, z, w);
+    *qd = make_float4(0.5f * (w * tx + (ty * z - tz * y)), 0.5f * (w * ty + (tz * x - tx * z)),
+                      0.5f * (w * tz + (tx * y - ty * x)), 0.5f * -((tx * x + ty * y) + tz * z));
+}
+
+// Accumulates w * sign * (qr, qd) into (br, bd), the sign aligning qr with
+// the first node's real part f.
+__device__ __forceinline__ void dqb_accumulate(const float4& qr, const float4& qd, const float4& f, float w, float4* br,
+                                               float4* bd) {
+    const float dot = ((qr.x * f.x + qr.y * f.y) + qr.z * f.z) + qr.w * f.w;
+    const float ws = dot < 0.0f ? -w : w;
+    br->x = br->x + ws * qr.x;
+    br->y = br->y + ws * qr.y;
+    br->z = br->z + ws * qr.z;
+    br->w = br->w + ws * qr.w;
+    bd->x = bd->x + ws * qd.x;
+    bd->y = bd->y + ws * qd.y;
+    bd->z = bd->z + ws * qd.z;
+    bd->w = bd->w + ws * qd.w;
+}
+
+// Applies the blended dual quaternion (br, bd), normalised by |br|, to p
+// (Kavan et al., ACM TOG 27(4), 2008):
+// p' = p + 2 r_v x (r_v x p + r_w p) + 2 ((r_w d_v - d_w r_v) + r_v x d_v).
+__device__ __forceinline__ void dqb_transform(const float4& br, const float4& bd, const float p[3], float o[3]) {
+    const float len = __builtin_sqrtf(((br.x * br.x + br.y * br.y) + br.z * br.z) + br.w * br.w);
+    if (!(len > 0.0f)) {
+        o[0] = p[0];
+        o[1] = p[1];
+        o[2] = p[2];
+        return;
+    }
+    const float il = 1.0f / len;
+    const float rx = br.x * il, ry = br.y * il, rz = br.z * il, rw = br.w * il;
+    const float dx = bd.x * il, dy = bd.y * il, dz = bd.z * il, dw = bd.w * il;
+    const float t1x = ry * p[2] - rz * p[1];
+    const float t1y = rz * p[0] - rx * p[2];
+    const float t1z = rx * p[1] - ry * p[0];
+    const float t2x = t1x + rw * p[0];
+    const float t2y = t1y + rw * p[1];
+    const float t2z = t1z + rw * p[2];
+    const float t3x = ry * t2z - rz * t2y;
+    const float t3y = rz * t2x - rx * t2z;
+    const float t3z = rx * t2y - ry * t2x;
+    const float ux = ry * dz - rz * dy;
+    const float uy = rz * dx - rx * dz;
+    const float uz = rx * dy - ry * dx;
+    const float trx = 2.0f * ((rw * dx - dw * rx) + ux);
+    const float try_ = 2.0f * ((rw * dy - dw * ry) + uy);
+    const float trz = 2.0f * ((rw * dz - dw * rz) + uz);
+    o[0] = (p[0] + 2.0f * t3x) + trx;
+    o[1] = (p[1] + 2.0f * t3y) + try_;
+    o[2] = (p[2] + 2.0f * t3z) + trz;
+}
+
 // Target voxel of a warped vertex (src/TSDFIntegration.cu:272-279):
 // u = worldToVoxel(Rinv * (p' - t)).  Returns false when u is outside.
 __device__ __forceinline__ bool warped_target(const Geom& g, const float* Rinv, const float t[3], const float pw[3],

@@ -96,6 +96,7 @@ struct VolDesc {
     uint32_t voxel_base;  // first voxel of this volume in the batch-wide voxel numbering
     uint32_t k1_base;
     uint32_t k1_count;
+    int32_t blend;  // VF_BLEND_LBS or VF_BLEND_DQB
 };
 
 struct NodeRec {
@@ -207,7 +208,11 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
         NodeRec rec;
         rec.m = make_float4(m[0], m[1], m[2], r * 2.0f);
         rec.lo = make_int4(lo[0], lo[1], lo[2], 0);
-        if (vd.T) {  // the knn-only entry point has no transforms
+        if (vd.T && vd.blend == VF_BLEND_DQB) {
+            // t0, t1: the node's dual quaternion (real, dual)
+            mat4_to_dq(vd.T + 16ull * j, &rec.t0, &rec.t1);
+            rec.t2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else if (vd.T) {  // the knn-only entry point has no transforms
             const float* M = vd.T + 16ull * j;
             rec.t0 = make_float4(M[0], M[4], M[8], M[12]);
             rec.t1 = make_float4(M[1], M[5], M[9], M[13]);
@@ -502,6 +507,26 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 acc[0] = pc[0];
                 acc[1] = pc[1];
                 acc[2] = pc[2];
+            } else if (vd.blend == VF_BLEND_DQB) {
+                // dual-quaternion blend of the selected nodes' rigid parts
+                float4 br = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bd = br;
+                const float4 f = spill ? nrec[vd.node_base + kl.i[0]].t0 : s_t0[kl.i[0]];
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    if (s < n) {
+                        float4 qr, qd;
+                        if (spill) {
+                            const NodeRec& rec = nrec[vd.node_base + kl.i[s]];
+                            qr = rec.t0;
+                            qd = rec.t1;
+                        } else {
+                            qr = s_t0[kl.i[s]];
+                            qd = s_t1[kl.i[s]];
+                        }
+                        dqb_accumulate(qr, qd, f, w[s], &br, &bd);
+                    }
+                }
+                dqb_transform(br, bd, pc, acc);
             } else {
 #pragma unroll
                 for (int s = 0; s < K; ++s) {
@@ -901,6 +926,9 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     vd->cube[0] = p.voxels_per_node.x;
     vd->cube[1] = p.voxels_per_node.y;
     vd->cube[2] = p.voxels_per_node.z;
+    if (p.blend != VF_BLEND_LBS && p.blend != VF_BLEND_DQB)
+        return fail(VF_ERR_INVALID_ARGUMENT, "blend mode %d: supported VF_BLEND_LBS (0), VF_BLEND_DQB (1)", p.blend);
+    vd->blend = knn_only ? VF_BLEND_LBS : p.blend;
     const int32_t* bs = kShapeDims[shape];
     const int32_t dims[3] = {vol.dim.x, vol.dim.y, vol.dim.z};
     for (int a = 0; a < 3; ++a) {

@@ -163,7 +163,7 @@ int issue_batch(vf_server* sv, Lane& lane) {
         v.center = d.center;
         graphs[i] = vf_graph{s.points, s.points, s.T, s.radii, (size_t)d.nodes};
         frames[i] = vf_frame{s.depth, d.width, d.height, d.intrinsics};
-        params[i] = vf_warp_params{sv->cfg.k, d.delta, d.max_weight, d.voxels_per_node};
+        params[i] = vf_warp_params{sv->cfg.k, d.delta, d.max_weight, d.voxels_per_node, d.blend};
     }
     int st = vf_warped_integrate_batch(lane.ws, n, vols.data(), graphs.data(), frames.data(), params.data(),
                                        lane.stream);

@@ -275,24 +275,25 @@ def make_frame(depth: DeviceArray, intrinsics) -> L.Frame:
     return f
 
 
-def make_params(k: int, delta: float, max_weight: float, cube) -> L.WarpParams:
+def make_params(k: int, delta: float, max_weight: float, cube, blend: int = L.BLEND_LBS) -> L.WarpParams:
     p = L.WarpParams()
     p.k = int(k)
     p.delta = float(delta)
     p.max_weight = float(max_weight)
     p.voxels_per_node = L.i3(cube)
+    p.blend = int(blend)
     return p
 
 
 def warped_integrate(ws: Workspace, tsdf: DeviceTsdf, graph: DeviceMotionGraph, depth: DeviceArray, intrinsics,
                      k: int, delta: float, max_weight: float, cube, write_back: bool = True,
-                     stream: Stream | None = None, sync: bool = True) -> None:
+                     stream: Stream | None = None, sync: bool = True, blend: int = L.BLEND_LBS) -> None:
     """The reference's per-frame integration block (src/VolumeFusion.cu:702-742)
     as one call: volume centre = tsdf.center, rotation I."""
     vol = make_volume(tsdf)
     gr = make_graph(graph, write_back)
     fr = make_frame(depth, intrinsics)
-    p = make_params(k, delta, max_weight, cube)
+    p = make_params(k, delta, max_weight, cube, blend)
     L.call("vf_warped_integrate", ws.handle, C.byref(vol), C.byref(gr), C.byref(fr), C.byref(p), _s(stream))
     if sync:
         _done(stream)
@@ -300,7 +301,7 @@ def warped_integrate(ws: Workspace, tsdf: DeviceTsdf, graph: DeviceMotionGraph, 
 
 def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list, k: int, delta: float,
                            max_weight: float, cubes, write_back: bool = True, stream: Stream | None = None,
-                           sync: bool = True) -> None:
+                           sync: bool = True, blend: int = L.BLEND_LBS) -> None:
     """Many independent volumes, one frame each, in one set of launches."""
     n = len(tsdfs)
     vols = (L.Volume * n)(*[make_volume(t) for t in tsdfs])
@@ -308,7 +309,8 @@ def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list
     frs = (L.Frame * n)(*[make_frame(d, K) for d, K in zip(depths, intrinsics_list)])
     deltas = delta if isinstance(delta, (list, tuple)) else [delta] * n
     maxws = max_weight if isinstance(max_weight, (list, tuple)) else [max_weight] * n
-    ps = (L.WarpParams * n)(*[make_params(k, d, m, c) for d, m, c in zip(deltas, maxws, cubes)])
+    blends = blend if isinstance(blend, (list, tuple)) else [blend] * n
+    ps = (L.WarpParams * n)(*[make_params(k, d, m, c, b) for d, m, c, b in zip(deltas, maxws, cubes, blends)])
     L.call("vf_warped_integrate_batch", ws.handle, n, vols, grs, frs, ps, _s(stream))
     if sync:
         _done(stream)
@@ -322,7 +324,7 @@ class Plan:
 
     def __init__(self, tsdfs, graphs, depths, intrinsics_list, k: int, delta, max_weight, cubes,
                  write_back: bool = True, graph: bool = True, tuning: tuple | None = None,
-                 transforms=None, stream: Stream | None = None):
+                 transforms=None, stream: Stream | None = None, blend: int = L.BLEND_LBS):
         """transforms: optional per-volume device arrays used instead of the
         graphs' own transform buffers."""
         n = len(tsdfs)
@@ -337,7 +339,7 @@ class Plan:
         self.frames = (L.Frame * n)(*[make_frame(d, K) for d, K in zip(depths, intrinsics_list)])
         deltas = delta if isinstance(delta, (list, tuple)) else [delta] * n
         maxws = max_weight if isinstance(max_weight, (list, tuple)) else [max_weight] * n
-        self.params = (L.WarpParams * n)(*[make_params(k, d, m, c) for d, m, c in zip(deltas, maxws, cubes)])
+        self.params = (L.WarpParams * n)(*[make_params(k, d, m, c, blend) for d, m, c in zip(deltas, maxws, cubes)])
         self.handle = C.c_void_p(None)
         t = None
         if tuning is not None:

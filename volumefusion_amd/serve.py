@@ -15,7 +15,7 @@ from . import _lib as L
 
 
 def stream_desc(dim, size, center, width: int, height: int, intrinsics, delta: float, max_weight: float, cube,
-                points: np.ndarray, radii: np.ndarray) -> tuple:
+                points: np.ndarray, radii: np.ndarray, blend: int = L.BLEND_LBS) -> tuple:
     """A vf_stream_desc and the host arrays it points at (keep both alive
     until vf_server_open returns)."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
@@ -32,13 +32,14 @@ def stream_desc(dim, size, center, width: int, height: int, intrinsics, delta: f
     d.nodes = pts.shape[0]
     d.points = pts.ctypes.data
     d.radii = rad.ctypes.data
+    d.blend = int(blend)
     return d, (pts, rad)
 
 
-def scene_desc(s) -> tuple:
+def scene_desc(s, blend: int = L.BLEND_LBS) -> tuple:
     """Stream description of a synthetic scene (volumefusion_amd.scenes)."""
     return stream_desc(s.dim, s.size, s.center, s.cam.width, s.cam.height, s.intrinsics(), s.delta,
-                       s.max_weight, s.cube, s.nodes, s.radii)
+                       s.max_weight, s.cube, s.nodes, s.radii, blend)
 
 
 def stream_bytes(desc: L.StreamDesc, lanes: int) -> int:
