@@ -1,8 +1,10 @@
 // af/vf_dropin.h -- the reference's af:: TSDF-integration entry points on the
 // MI355X kernels of volumefusion.h.
 //
-// Replaces include/af/TSDFWarp.h and include/af/TSDFIntegration.cuh (and
-// af::runTranslateKernel of include/af/VertexManipulation.cuh:16) for callers
+// Replaces include/af/TSDFWarp.h and include/af/TSDFIntegration.cuh, and
+// af::runTranslateKernel, af::runComputeWarpWeightsKernel
+// (include/af/VertexManipulation.cuh:16, 19-23) and af::runWarpKernel
+// (include/af/ComputeCorrespondence.cuh:11-16), for callers
 // such as af::runVolumeFusion (src/VolumeFusion.cu:334, 708-741) and
 // af::DeviceTsdf::integrate (include/af/DeviceTsdf.cuh:28-32).  Same names,
 // argument order and meaning, same Eigen types (spelled out, so this header
@@ -174,6 +176,31 @@ inline void runCountCollisionsPerVoxelKernel(int* collisionCounts, const Eigen::
                                        f3(cVoxelSize), f3(cVolumeSize), i3(cVolumeDim), m3(cVolumeRotate),
                                        f3(cVolumeTranslate), nullptr),
          "runCountCollisionsPerVoxelKernel");
+}
+
+// src/VertexManipulation.cu:138-145 (the mesh weights have the d >= r cutoff)
+template <int K>
+void runComputeWarpWeightsKernel(Eigen::Matrix<float, K, 1>* knnWeights, const float* cMotionGraphRadiuses,
+                                 const Eigen::Matrix<unsigned int, K, 1>* cKnnIdxs,
+                                 const Eigen::Matrix<float, K, 1>* cKnnDists, const std::size_t cMeshSize) {
+    vf_detail::check_k_layout<K>();
+    vf_detail::done(vf_compute_warp_weights(K, reinterpret_cast<float*>(knnWeights), cMotionGraphRadiuses,
+                                            reinterpret_cast<const uint32_t*>(cKnnIdxs),
+                                            reinterpret_cast<const float*>(cKnnDists), cMeshSize, 0, nullptr),
+                    "runComputeWarpWeightsKernel");
+}
+
+// src/ComputeCorrespondence.cu:32-43
+template <int K>
+void runWarpKernel(Eigen::Vector3f* warpedMesh, const Eigen::Vector3f* cMesh, const Eigen::Matrix4f* cMotionGraph,
+                   const Eigen::Matrix<unsigned int, K, 1>* cKnnIdxs, const Eigen::Matrix<float, K, 1>* cKnnWeights,
+                   const std::size_t cMeshSize) {
+    vf_detail::check_k_layout<K>();
+    vf_detail::done(vf_warp_mesh(K, reinterpret_cast<vf_float3*>(warpedMesh), reinterpret_cast<const vf_float3*>(cMesh),
+                                 reinterpret_cast<const vf_mat4*>(cMotionGraph),
+                                 reinterpret_cast<const uint32_t*>(cKnnIdxs), reinterpret_cast<const float*>(cKnnWeights),
+                                 cMeshSize, 0, nullptr),
+                    "runWarpKernel");
 }
 
 // src/VertexManipulation.cu:81-87

@@ -39,6 +39,7 @@ typedef struct vf_int2 { int32_t x, y; } vf_int2;      /* Eigen::Vector2i       
 typedef struct vf_mat3 { float m[9]; } vf_mat3;        /* Eigen::Matrix3f, column-major */
 typedef struct vf_mat4 { float m[16]; } vf_mat4;       /* Eigen::Matrix4f, column-major */
 typedef void* vf_stream;                               /* hipStream_t */
+typedef struct vf_workspace vf_workspace;              /* device scratch, see below */
 
 enum {
     VF_OK = 0,
@@ -122,6 +123,39 @@ int vf_integrate_tsdf_camera_frame(float* tsdf, float* tsdf_weights, const float
  * (src/VertexManipulation.cu:73-87): mesh[i] += translation. */
 int vf_translate(vf_float3* mesh, uint32_t mesh_size, vf_float3 translation, vf_stream stream);
 
+/* ---- mesh points ---------------------------------------------------------- */
+/* af::runComputeWarpWeightsKernel, include/af/VertexManipulation.cuh:19-23
+ * (src/VertexManipulation.cu:105-145): for each point, all K slots,
+ * w_j = d_j >= r ? 0 : exp(-d_j^2 / (2 r^2)), normalised when the sum is not 0.
+ * A slot whose distance is FLT_MAX (a missing neighbour) weighs 0 without
+ * reading its radius.  graph_size > 0 also treats indices >= graph_size as
+ * missing; 0 leaves them unchecked, as the reference does. */
+int vf_compute_warp_weights(int k, float* knn_weights, const float* graph_radii, const uint32_t* knn_idxs,
+                            const float* knn_dists, size_t mesh_size, size_t graph_size, vf_stream stream);
+
+/* af::runWarpKernel, include/af/ComputeCorrespondence.cuh:11-16
+ * (src/ComputeCorrespondence.cu:5-43): p' = sum_j w_j (T_j [p; 1]).xyz over
+ * all K slots, p itself when every |w_j| <= 1e-5.  warped_mesh may alias mesh.
+ * Slots of weight exactly 0 are skipped (their index is not read). */
+int vf_warp_mesh(int k, vf_float3* warped_mesh, const vf_float3* mesh, const vf_mat4* motion_graph,
+                 const uint32_t* knn_idxs, const float* knn_weights, size_t mesh_size, size_t graph_size,
+                 vf_stream stream);
+
+/* af::getKnnData<K> (include/af/GraphBuilder.h:20-32; MotionGraph::knnSearch,
+ * include/af/MotionGraph.h:50-70) on the device: the K nearest nodes of each
+ * point by float L2 distance, slots ordered by (distance, node index);
+ * exclude_closest drops the nearest (node-to-node knn).  Missing slots get
+ * index UINT32_MAX and distance FLT_MAX.  Uses the workspace's scratch. */
+int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, const vf_float3* points, size_t n,
+                const vf_float3* nodes, size_t graph_size, int exclude_closest, vf_stream stream);
+
+/* The three above in one kernel (knn, weights, blend) for n points;
+ * blend VF_BLEND_LBS is the reference's warp, VF_BLEND_DQB blends the nodes'
+ * rigid parts as dual quaternions.  warped may alias points. */
+int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
+                   const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
+                   vf_stream stream);
+
 /* Gaussian warp weight of src/TSDFWarp.cu:17 element by element,
  * out[i] = exp(-(dist[i]^2) / (2 radius[i]^2)), unnormalised: the function
  * every weight above is built from, exposed for diagnostics. */
@@ -185,7 +219,6 @@ typedef struct vf_frame_stats {
     uint64_t overflow_bricks; /* bricks whose node list spilled            */
 } vf_frame_stats;
 
-typedef struct vf_workspace vf_workspace;
 
 /* Workspace for up to `max_volumes` volumes per call, each with at most
  * max_voxels voxels and max_nodes nodes (total over the batch).  Allocates

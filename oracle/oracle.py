@@ -120,6 +120,10 @@ def lib() -> C.CDLL:
                                            _f32p, _f32p, C.c_int, C.c_float, C.c_float, _i32p, _u32p, _i32p,
                                            C.c_int]
         L.vfo_warped_frame_f64.restype = C.c_int
+        L.vfo_mesh_knn.argtypes = [C.c_int, _u32p, _f32p, _f32p, C.c_size_t, _f32p, C.c_size_t, C.c_int]
+        L.vfo_compute_warp_weights.argtypes = [C.c_int, _f32p, _f32p, _u32p, _f32p, C.c_size_t, C.c_size_t]
+        L.vfo_warp_mesh.argtypes = [C.c_int, _f32p, _f32p, _f32p, _u32p, _f32p, C.c_size_t, C.c_size_t]
+        L.vfo_warp_points.argtypes = [C.c_int, _f32p, _f32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_size_t, C.c_int]
         L.vfo_mat4_to_dq.argtypes = [_f32p, _f32p]
         L.vfo_mat4_to_dq_f64.argtypes = [_f32p, _f64p]
         L.vfo_dqb_apply.argtypes = [C.POINTER(C.c_void_p), _f32p, C.c_int, _f32p, _f32p]
@@ -271,6 +275,38 @@ def mesh_warp(pts, nodes, radii, T, k, host_order=True, squared_dist=False) -> n
 
 
 # ---- whole frame ----------------------------------------------------------
+
+def mesh_knn(k, pts, nodes, exclude_closest=False):
+    """K nearest nodes of each point: (idx [n, k] uint32, dist [n, k] float32)."""
+    pts, nodes = _f32(pts).reshape(-1, 3), _f32(nodes).reshape(-1, 3)
+    idx = np.zeros((pts.shape[0], k), np.uint32)
+    dist = np.zeros((pts.shape[0], k), np.float32)
+    lib().vfo_mesh_knn(int(k), idx, dist, pts, pts.shape[0], nodes, nodes.shape[0], 1 if exclude_closest else 0)
+    return idx, dist
+
+
+def compute_warp_weights(k, radii, idx, dist) -> np.ndarray:
+    idx = np.ascontiguousarray(idx, np.uint32)
+    w = np.zeros(idx.shape, np.float32)
+    radii = _f32(radii)
+    lib().vfo_compute_warp_weights(int(k), w, radii, idx, _f32(dist), idx.shape[0], radii.shape[0])
+    return w
+
+
+def warp_mesh(k, pts, T, idx, w, G) -> np.ndarray:
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.zeros_like(pts)
+    lib().vfo_warp_mesh(int(k), out, pts, _f32(T), np.ascontiguousarray(idx, np.uint32), _f32(w), pts.shape[0],
+                        int(G))
+    return out
+
+
+def warp_points(k, pts, nodes, radii, T, blend=0) -> np.ndarray:
+    pts, nodes = _f32(pts).reshape(-1, 3), _f32(nodes).reshape(-1, 3)
+    out = np.zeros_like(pts)
+    lib().vfo_warp_points(int(k), out, pts, pts.shape[0], nodes, _f32(radii), _f32(T), nodes.shape[0], int(blend))
+    return out
+
 
 def mat4_to_dq(T16) -> np.ndarray:
     """Dual quaternion (real xyzw, dual xyzw) of a column-major Mat4f, float32."""

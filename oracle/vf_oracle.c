@@ -38,6 +38,7 @@
  */
 #include "vf_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -804,6 +805,157 @@ int64_t vfo_tsdfvolume_integrate(float* tsdf, float* wt, const float* depth, int
                 }
             }
     return updated;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mesh points: knn, weights, warp (the device kernels of the point path)    */
+/* ------------------------------------------------------------------------ */
+
+/* K nearest nodes of each point, MotionGraph::knnSearch<K>
+ * (include/af/MotionGraph.h:50-70; nanoflann's float squared distance
+ * ((dx^2 + dy^2) + dz^2), then sqrt).  Canonical order (distance, node index)
+ * where nanoflann keeps its tree's visit order among equal distances.  With
+ * exclude_closest the first result is dropped (node-to-node knn,
+ * src/WarpField.cu:31).  Missing slots: index UINT32_MAX, distance FLT_MAX
+ * (include/af/MotionGraph.h:66-69).  idx/dist are [n][K]. */
+void vfo_mesh_knn(int K, uint32_t* idx, float* dist, const float* pts, size_t n, const float* nodes, size_t G,
+                  int exclude_closest) {
+    const int KK = K + (exclude_closest ? 1 : 0);
+    float dl[65];
+    uint32_t il[65];
+    for (size_t i = 0; i < n; ++i) {
+        int c = 0;
+        for (size_t j = 0; j < G; ++j) {
+            const float dx = pts[3 * i + 0] - nodes[3 * j + 0];
+            const float dy = pts[3 * i + 1] - nodes[3 * j + 1];
+            const float dz = pts[3 * i + 2] - nodes[3 * j + 2];
+            const float s2 = (dx * dx + dy * dy) + dz * dz;
+            knn_insert(KK, dl, il, &c, sqrtf(s2), (uint32_t)j);
+        }
+        const int off = exclude_closest ? 1 : 0;
+        for (int s = 0; s < K; ++s) {
+            const int t = s + off;
+            idx[(size_t)K * i + s] = t < c ? il[t] : UINT32_MAX;
+            dist[(size_t)K * i + s] = t < c ? dl[t] : FLT_MAX;
+        }
+    }
+}
+
+/* computeWarpWeightsKernel, src/VertexManipulation.cu:105-136: every one of
+ * the K slots, Gaussian with the d >= r cutoff (device float semantics, the
+ * exponential as vfo_expw), normalised when the sum is not 0.  A slot whose
+ * distance is FLT_MAX (a missing neighbour) weighs 0 without reading the
+ * radius; indices >= G are treated the same way (the reference would read
+ * out of bounds). */
+void vfo_compute_warp_weights(int K, float* w, const float* radii, const uint32_t* idx, const float* dist, size_t n,
+                              size_t G) {
+    for (size_t i = 0; i < n; ++i) {
+        float sum = 0.0f;
+        float* wv = w + (size_t)K * i;
+        for (int s = 0; s < K; ++s) {
+            const uint32_t j = idx[(size_t)K * i + s];
+            const float d = dist[(size_t)K * i + s];
+            float x = 0.0f;
+            if (d < FLT_MAX && j < G) {
+                const float r = radii[j];
+                if (!(d >= r)) {
+                    const float r2 = r * r;
+                    const float den = 2.0f * r2;
+                    x = vfo_expw(-(d * d) / den);
+                }
+            }
+            wv[s] = x;
+            sum = sum + x;
+        }
+        if (sum != 0.0f)
+            for (int s = 0; s < K; ++s)
+                wv[s] = wv[s] / sum;
+    }
+}
+
+/* warpKernel, src/ComputeCorrespondence.cu:5-30: all K slots; the vertex is
+ * materialised as a Vec4f, so each row of T p is Eigen's four-term reduction
+ * (a0 + a1) + (a2 + a3) (Eigen/src/Core/Redux.h:99-113); weights all within
+ * 1e-5 of 0 leave the point as it is.  A slot of weight exactly 0 adds
+ * nothing: for a finite transform w t is a signed zero and the running sum,
+ * never -0, is unchanged; skipping it keeps a missing neighbour's index
+ * (UINT32_MAX) from being read.  Indices >= G contribute 0. */
+void vfo_warp_mesh(int K, float* out, const float* pts, const float* T, const uint32_t* idx, const float* w, size_t n,
+                   size_t G) {
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = pts + 3 * i;
+        const float* wv = w + (size_t)K * i;
+        if (weights_zero(wv, K)) {
+            out[3 * i + 0] = p[0];
+            out[3 * i + 1] = p[1];
+            out[3 * i + 2] = p[2];
+            continue;
+        }
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (int s = 0; s < K; ++s) {
+            const uint32_t j = idx[(size_t)K * i + s];
+            if (wv[s] == 0.0f)
+                continue;
+            float t[3] = {0.0f, 0.0f, 0.0f};
+            if (j < G) {
+                const float* M = T + 16 * (size_t)j;
+                for (int a = 0; a < 3; ++a) {
+                    const float a0 = M[0 * 4 + a] * p[0];
+                    const float a1 = M[1 * 4 + a] * p[1];
+                    const float a2 = M[2 * 4 + a] * p[2];
+                    const float a3 = M[3 * 4 + a] * 1.0f;
+                    t[a] = (a0 + a1) + (a2 + a3);
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float wt = wv[s] * t[a];
+                acc[a] = acc[a] + wt;
+            }
+        }
+        out[3 * i + 0] = acc[0];
+        out[3 * i + 1] = acc[1];
+        out[3 * i + 2] = acc[2];
+    }
+}
+
+/* The point path in one call: knn, weights, then the linear blend (blend 0)
+ * or DQB (blend 1, weights as above, missing slots skipped). */
+void vfo_warp_points(int K, float* out, const float* pts, size_t n, const float* nodes, const float* radii,
+                     const float* T, size_t G, int blend) {
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)K * (n ? n : 1));
+    float* dist = (float*)malloc(sizeof(float) * (size_t)K * (n ? n : 1));
+    float* w = (float*)malloc(sizeof(float) * (size_t)K * (n ? n : 1));
+    if (!idx || !dist || !w) {
+        free(idx); free(dist); free(w);
+        return;
+    }
+    vfo_mesh_knn(K, idx, dist, pts, n, nodes, G, 0);
+    vfo_compute_warp_weights(K, w, radii, idx, dist, n, G);
+    if (!blend) {
+        vfo_warp_mesh(K, out, pts, T, idx, w, n, G);
+    } else {
+        for (size_t i = 0; i < n; ++i) {
+            const float* wv = w + (size_t)K * i;
+            if (weights_zero(wv, K)) {
+                memcpy(out + 3 * i, pts + 3 * i, 3 * sizeof(float));
+                continue;
+            }
+            float dqs[64][8], ws[64];
+            const float* q[64];
+            int m = 0;
+            for (int s = 0; s < K; ++s) {
+                const uint32_t j = idx[(size_t)K * i + s];
+                if (j >= G)
+                    continue;
+                vfo_mat4_to_dq(T + 16 * (size_t)j, dqs[m]);
+                q[m] = dqs[m];
+                ws[m] = wv[s];
+                ++m;
+            }
+            vfo_dqb_apply(q, ws, m, pts + 3 * i, out + 3 * i);
+        }
+    }
+    free(idx); free(dist); free(w);
 }
 
 /* MotionGraph::warp, src/MotionGraph.cpp:62-90, host semantics: the K nearest

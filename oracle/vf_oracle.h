@@ -120,6 +120,16 @@ int64_t vfo_tsdfvolume_integrate(float* tsdf, float* wt, const float* depth, int
 void vfo_mesh_warp(float* out, const float* pts, size_t n, const float* nodes, const float* radii, const float* T,
                    size_t G, int K, int host_order, int squared_dist);
 
+/* ---- mesh points: knn, weights, warp -------------------------------------- */
+void vfo_mesh_knn(int K, uint32_t* idx, float* dist, const float* pts, size_t n, const float* nodes, size_t G,
+                  int exclude_closest);
+void vfo_compute_warp_weights(int K, float* w, const float* radii, const uint32_t* idx, const float* dist, size_t n,
+                              size_t G);
+void vfo_warp_mesh(int K, float* out, const float* pts, const float* T, const uint32_t* idx, const float* w, size_t n,
+                   size_t G);
+void vfo_warp_points(int K, float* out, const float* pts, size_t n, const float* nodes, const float* radii,
+                     const float* T, size_t G, int blend);
+
 /* ---- dual-quaternion blending (second blend mode; not in the reference) --- */
 /* dq: 8 floats, real (x, y, z, w) then dual (x, y, z, w). */
 void vfo_mat4_to_dq(const float* M, float* dq);

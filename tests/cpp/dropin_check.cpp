@@ -58,8 +58,13 @@ int host_checks() {
         &af::runComputeTsdfWarpWeightsKernel<K>;
     void (*warp)(Vec3f*, const Mat4f*, const Vecui*, const Vecf*, const int*, Vec3f, Vec3f, Vec3i) =
         &af::runWarpTsdfKernel<K>;
+    void (*mesh_w)(Vecf*, const float*, const Vecui*, const Vecf*, std::size_t) = &af::runComputeWarpWeightsKernel<K>;
+    void (*mesh_warp)(Vec3f*, const Vec3f*, const Mat4f*, const Vecui*, const Vecf*, std::size_t) =
+        &af::runWarpKernel<K>;
     (void)wts;
     (void)warp;
+    (void)mesh_w;
+    (void)mesh_warp;
     const Vec3f vs(0.01f, 0.01f, 0.01f), size(0.1f, 0.1f, 0.1f);
     const Vec3i dim(10, 10, 10);
     expect_throw([&] { af::runTranslateKernel(nullptr, 5, Vec3f(1, 2, 3)); }, "mesh is NULL",

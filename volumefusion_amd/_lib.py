@@ -171,6 +171,10 @@ _SIGS = {
                                        f32, vp],
     "vf_translate": [vp, u32, Float3, vp],
     "vf_eval_warp_weight": [vp, vp, vp, sz, vp],
+    "vf_compute_warp_weights": [C.c_int, vp, vp, vp, vp, sz, sz, vp],
+    "vf_warp_mesh": [C.c_int, vp, vp, vp, vp, vp, sz, sz, vp],
+    "vf_mesh_knn": [vp, C.c_int, vp, vp, vp, sz, vp, sz, C.c_int, vp],
+    "vf_warp_points": [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
     "vf_workspace_create": [C.POINTER(vp), C.c_int, u64, u64, vp],
     "vf_workspace_destroy": [vp],
     "vf_workspace_bytes": [vp],

@@ -791,6 +791,8 @@ struct vf_workspace {
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
     vf_tuning tuning = {0, 0, 0.0f};
+    void* scratch = nullptr;  // general scratch for other entry points (point path)
+    uint64_t scratch_bytes = 0;
 };
 
 namespace vf {
@@ -1375,6 +1377,30 @@ std::vector<vf_workspace*> g_knn_ws;
 
 using namespace vf;
 
+namespace vf {
+
+// At least `bytes` of device scratch owned by the workspace; grows after the
+// stream's earlier work is done.  Contents are not preserved.
+int workspace_scratch(vf_workspace* ws, uint64_t bytes, void** ptr, hipStream_t stream) {
+    if (!ws) return fail(VF_ERR_INVALID_ARGUMENT, "NULL workspace");
+    if (bytes > ws->scratch_bytes) {
+        VF_HIP(hipStreamSynchronize(stream));
+        if (ws->scratch) VF_HIP(hipFree(ws->scratch));
+        ws->scratch = nullptr;
+        ws->scratch_bytes = 0;
+        const uint64_t nb = std::max<uint64_t>(bytes, 1u << 16);
+        const hipError_t e = hipMalloc(&ws->scratch, nb);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? VF_ERR_OUT_OF_MEMORY : VF_ERR_HIP, "workspace scratch of %llu B: %s",
+                        (unsigned long long)nb, hipGetErrorString(e));
+        ws->scratch_bytes = nb;
+    }
+    *ptr = ws->scratch;
+    return VF_OK;
+}
+
+}  // namespace vf
+
 extern "C" {
 
 int vf_workspace_create(vf_workspace** out, int max_volumes, uint64_t max_total_voxels, uint64_t max_total_nodes,
@@ -1393,13 +1419,14 @@ int vf_workspace_create(vf_workspace** out, int max_volumes, uint64_t max_total_
 
 int vf_workspace_destroy(vf_workspace* ws) {
     if (!ws) return VF_OK;
+    if (ws->scratch) (void)hipFree(ws->scratch);
     for (auto& e : ws->events) (void)hipEventDestroy(e);
     free_buffers(ws->buf);
     delete ws;
     return VF_OK;
 }
 
-uint64_t vf_workspace_bytes(const vf_workspace* ws) { return ws ? ws->buf.bytes : 0; }
+uint64_t vf_workspace_bytes(const vf_workspace* ws) { return ws ? ws->buf.bytes + ws->scratch_bytes : 0; }
 
 int vf_warped_integrate(vf_workspace* ws, const vf_volume* volume, const vf_graph* graph, const vf_frame* frame,
                         const vf_warp_params* params, vf_stream stream) {

@@ -60,6 +60,9 @@ inline int check_k(int k) {
     return VF_OK;
 }
 
+// Device scratch owned by a workspace (defined with the workspace).
+int workspace_scratch(vf_workspace* ws, uint64_t bytes, void** ptr, hipStream_t stream);
+
 inline Camera make_camera(const vf_mat3& K, int W, int H, float delta, float maxw) {
     Camera c;
     c.fx = K.m[0];

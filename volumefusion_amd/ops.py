@@ -129,6 +129,51 @@ def run_translate_kernel(mesh: DeviceArray, mesh_size: int, translation, stream:
     _done(stream)
 
 
+def run_compute_warp_weights_kernel(knn_weights: DeviceArray, motion_graph_radiuses: DeviceArray,
+                                    knn_idxs: DeviceArray, knn_dists: DeviceArray, mesh_size: int,
+                                    graph_size: int = 0, stream: Stream | None = None) -> None:
+    """af::runComputeWarpWeightsKernel (include/af/VertexManipulation.cuh:19-23).
+    graph_size 0 leaves indices unchecked, as the reference does."""
+    L.call("vf_compute_warp_weights", _k_of(knn_idxs), knn_weights.ptr, motion_graph_radiuses.ptr, knn_idxs.ptr,
+           knn_dists.ptr, int(mesh_size), int(graph_size), _s(stream))
+    _done(stream)
+
+
+def run_warp_kernel(warped_mesh: DeviceArray, mesh: DeviceArray, motion_graph: DeviceArray, knn_idxs: DeviceArray,
+                    knn_weights: DeviceArray, mesh_size: int, graph_size: int = 0,
+                    stream: Stream | None = None) -> None:
+    """af::runWarpKernel (include/af/ComputeCorrespondence.cuh:11-16)."""
+    L.call("vf_warp_mesh", _k_of(knn_idxs), warped_mesh.ptr, mesh.ptr, motion_graph.ptr, knn_idxs.ptr,
+           knn_weights.ptr, int(mesh_size), int(graph_size), _s(stream))
+    _done(stream)
+
+
+def get_knn_data(ws: "Workspace", k: int, points: DeviceArray, nodes: DeviceArray, exclude_closest: bool = False,
+                 stream: Stream | None = None) -> tuple:
+    """af::getKnnData<K> (include/af/GraphBuilder.h:20-32) on the device:
+    (idx [n, k] uint32, dist [n, k] float32) device arrays."""
+    n = points.shape[0]
+    idx = DeviceArray((n, k), np.uint32, stream)
+    dist = DeviceArray((n, k), np.float32, stream)
+    L.call("vf_mesh_knn", ws.handle, int(k), idx.ptr, dist.ptr, points.ptr, int(n), nodes.ptr, int(nodes.shape[0]),
+           1 if exclude_closest else 0, _s(stream))
+    _done(stream)
+    return idx, dist
+
+
+def warp_points(ws: "Workspace", k: int, points: DeviceArray, graph: "DeviceMotionGraph",
+                out: DeviceArray | None = None, blend: int = L.BLEND_LBS, stream: Stream | None = None) -> DeviceArray:
+    """getKnnData + runComputeWarpWeightsKernel + runWarpKernel in one kernel;
+    out may be points itself."""
+    n = points.shape[0]
+    if out is None:
+        out = DeviceArray((n, 3), np.float32, stream)
+    L.call("vf_warp_points", ws.handle, int(k), int(blend), out.ptr, points.ptr, int(n), graph.points.ptr,
+           graph.radii.ptr, graph.transforms.ptr, int(graph.count), _s(stream))
+    _done(stream)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Reference-layout device containers
 # ---------------------------------------------------------------------------
