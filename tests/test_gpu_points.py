@@ -35,7 +35,8 @@ def random_graph(rng, G, n, spread=0.3):
         M[:3, 3] = rng.uniform(-0.02, 0.02, 3)
         T[j] = M.T.reshape(16)
     pts = rng.uniform(-spread, spread, (n, 3)).astype(F)
-    pts[:10] = nodes[:10]  # points exactly on nodes
+    m = min(10, G)
+    pts[:m] = nodes[:m]  # points exactly on nodes
     return nodes, radii, T, pts
 
 
