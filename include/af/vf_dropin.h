@@ -4,7 +4,8 @@
 // Replaces include/af/TSDFWarp.h and include/af/TSDFIntegration.cuh, and
 // af::runTranslateKernel, af::runComputeWarpWeightsKernel
 // (include/af/VertexManipulation.cuh:16, 19-23) and af::runWarpKernel
-// (include/af/ComputeCorrespondence.cuh:11-16), for callers
+// (include/af/ComputeCorrespondence.cuh:11-16) and the marching-cubes
+// kernels (include/af/MarchingCubes.cuh:10-28), for callers
 // such as af::runVolumeFusion (src/VolumeFusion.cu:334, 708-741) and
 // af::DeviceTsdf::integrate (include/af/DeviceTsdf.cuh:28-32).  Same names,
 // argument order and meaning, same Eigen types (spelled out, so this header
@@ -21,6 +22,7 @@
 
 #include <Eigen/Core>
 #include <cstddef>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 
@@ -201,6 +203,29 @@ void runWarpKernel(Eigen::Vector3f* warpedMesh, const Eigen::Vector3f* cMesh, co
                                  reinterpret_cast<const uint32_t*>(cKnnIdxs), reinterpret_cast<const float*>(cKnnWeights),
                                  cMeshSize, 0, nullptr),
                     "runWarpKernel");
+}
+
+// src/MarchingCubes.cu:510-522 (canonical vertex order; see volumefusion.h)
+inline void runMarchingCubesMeshKernel(Eigen::Vector3f* mesh, unsigned int* meshSize, const float* cTsdf,
+                                       const float* cTsdfWeights, const Eigen::Vector3i cDims,
+                                       const Eigen::Vector3f& cVoxelSize, const Eigen::Vector3f& cVolSize,
+                                       const float cIsoValue) {
+    using namespace vf_detail;
+    done(vf_marching_cubes_mesh(nullptr, reinterpret_cast<vf_float3*>(mesh), meshSize, UINT64_MAX, cTsdf, cTsdfWeights,
+                                i3(cDims), f3(cVoxelSize), f3(cVolSize), cIsoValue, nullptr),
+         "runMarchingCubesMeshKernel");
+}
+
+// src/MarchingCubes.cu:684-702
+inline void runMarchingCubesFullKernel(Eigen::Vector3f* mesh, unsigned int* meshSize, Eigen::Vector3i* faces,
+                                       unsigned int* facesSize, const float* cTsdf, const float* cTsdfWeights,
+                                       const Eigen::Vector3i cDims, const Eigen::Vector3f& cVoxelSize,
+                                       const Eigen::Vector3f& cVolSize, const float cIsoValue) {
+    using namespace vf_detail;
+    done(vf_marching_cubes_full(nullptr, reinterpret_cast<vf_float3*>(mesh), meshSize, UINT64_MAX,
+                                reinterpret_cast<vf_int3*>(faces), facesSize, UINT64_MAX, cTsdf, cTsdfWeights,
+                                i3(cDims), f3(cVoxelSize), f3(cVolSize), cIsoValue, nullptr),
+         "runMarchingCubesFullKernel");
 }
 
 // src/VertexManipulation.cu:81-87

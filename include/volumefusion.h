@@ -145,7 +145,8 @@ int vf_warp_mesh(int k, vf_float3* warped_mesh, const vf_float3* mesh, const vf_
  * include/af/MotionGraph.h:50-70) on the device: the K nearest nodes of each
  * point by float L2 distance, slots ordered by (distance, node index);
  * exclude_closest drops the nearest (node-to-node knn).  Missing slots get
- * index UINT32_MAX and distance FLT_MAX.  Uses the workspace's scratch. */
+ * index UINT32_MAX and distance FLT_MAX.  Uses the workspace's scratch (NULL:
+ * the device's default workspace). */
 int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, const vf_float3* points, size_t n,
                 const vf_float3* nodes, size_t graph_size, int exclude_closest, vf_stream stream);
 
@@ -155,6 +156,29 @@ int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, c
 int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
                    const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
                    vf_stream stream);
+
+/* ---- marching cubes ------------------------------------------------------- */
+/* af::runMarchingCubesMeshKernel, include/af/MarchingCubes.cuh:10-17
+ * (src/MarchingCubes.cu:422-522): one vertex per crossed edge of every cube
+ * (i < X-2, j < Y-2, k < Z-2; a cube with a weight-0 corner is skipped).
+ * Vertices are appended after *mesh_size (a device counter the caller zeroes,
+ * as the reference does) in canonical order (cubes by linear index, edges by
+ * number) where the reference's atomics give a racy order; *mesh_size
+ * advances by the full count, and entries at or beyond mesh_capacity are not
+ * written.  Uses the workspace's scratch (NULL: the device's default
+ * workspace, for callers without one; such calls must not overlap). */
+int vf_marching_cubes_mesh(vf_workspace* ws, vf_float3* mesh, uint32_t* mesh_size, uint64_t mesh_capacity,
+                           const float* tsdf, const float* weights, vf_int3 dims, vf_float3 voxel_size,
+                           vf_float3 volume_size, float iso, vf_stream stream);
+
+/* af::runMarchingCubesFullKernel, include/af/MarchingCubes.cuh:19-28
+ * (src/MarchingCubes.cu:526-702): three vertices and one face per
+ * non-degenerate triangle of every cube, triangles in table order; face
+ * entries are absolute vertex indices.  Same counters and capacities. */
+int vf_marching_cubes_full(vf_workspace* ws, vf_float3* mesh, uint32_t* mesh_size, uint64_t mesh_capacity,
+                           vf_int3* faces, uint32_t* faces_size, uint64_t faces_capacity, const float* tsdf,
+                           const float* weights, vf_int3 dims, vf_float3 voxel_size, vf_float3 volume_size, float iso,
+                           vf_stream stream);
 
 /* Gaussian warp weight of src/TSDFWarp.cu:17 element by element,
  * out[i] = exp(-(dist[i]^2) / (2 radius[i]^2)), unnormalised: the function

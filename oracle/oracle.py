@@ -124,6 +124,8 @@ def lib() -> C.CDLL:
         L.vfo_compute_warp_weights.argtypes = [C.c_int, _f32p, _f32p, _u32p, _f32p, C.c_size_t, C.c_size_t]
         L.vfo_warp_mesh.argtypes = [C.c_int, _f32p, _f32p, _f32p, _u32p, _f32p, C.c_size_t, C.c_size_t]
         L.vfo_warp_points.argtypes = [C.c_int, _f32p, _f32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_size_t, C.c_int]
+        L.vfo_marching_cubes.argtypes = [C.c_int, _f32p, _f32p, C.POINTER(Vol), C.c_float, _f32p, C.c_uint64,
+                                         _i32p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.vfo_mat4_to_dq.argtypes = [_f32p, _f32p]
         L.vfo_mat4_to_dq_f64.argtypes = [_f32p, _f64p]
         L.vfo_dqb_apply.argtypes = [C.POINTER(C.c_void_p), _f32p, C.c_int, _f32p, _f32p]
@@ -306,6 +308,21 @@ def warp_points(k, pts, nodes, radii, T, blend=0) -> np.ndarray:
     out = np.zeros_like(pts)
     lib().vfo_warp_points(int(k), out, pts, pts.shape[0], nodes, _f32(radii), _f32(T), nodes.shape[0], int(blend))
     return out
+
+
+def marching_cubes(tsdf, wt, dim, vs, size, iso=0.0, full=False, cap=None):
+    """(vertices [n, 3], faces [m, 3] or None) in canonical order."""
+    v = make_vol(dim, vs, size)
+    nv, nf = C.c_uint64(0), C.c_uint64(0)
+    dummy_v, dummy_f = np.zeros((1, 3), np.float32), np.zeros((1, 3), np.int32)
+    t, w = _f32(tsdf), _f32(wt)
+    lib().vfo_marching_cubes(1 if full else 0, t, w, C.byref(v), float(iso), dummy_v, 0, dummy_f, 0, C.byref(nv),
+                             C.byref(nf))
+    verts = np.zeros((max(nv.value, 1), 3), np.float32)
+    faces = np.zeros((max(nf.value, 1), 3), np.int32)
+    lib().vfo_marching_cubes(1 if full else 0, t, w, C.byref(v), float(iso), verts, nv.value, faces, nf.value,
+                             C.byref(nv), C.byref(nf))
+    return verts[:nv.value], (faces[:nf.value] if full else None)
 
 
 def mat4_to_dq(T16) -> np.ndarray:

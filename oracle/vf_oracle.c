@@ -37,6 +37,7 @@
  *    (float, int)).
  */
 #include "vf_oracle.h"
+#include "vf_mc_tables.h"
 
 #include <float.h>
 #include <math.h>
@@ -1440,4 +1441,133 @@ int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes
             }
     (void)W;
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* marching cubes over the TSDF                                             */
+/* ------------------------------------------------------------------------ */
+
+static const uint64_t kMcTri[256] = VF_MC_TRI_TABLE;
+static const int kMcEdge[12][2] = VF_MC_EDGE_CORNERS;
+static const int kMcCorner[8][3] = VF_MC_CORNER_OFFSETS;
+
+/* interpolate, src/MarchingCubes.cu:294-312: the ratio in float, widened to
+ * binary64, clamped, and the lerp in binary64 rounded to float. */
+static void mc_interp(float t0, float t1, const float* p0, const float* p1, float iso, float* out) {
+    if (fabs((double)(iso - t0)) < 1e-7) {
+        memcpy(out, p0, 3 * sizeof(float));
+        return;
+    }
+    if (fabs((double)(iso - t1)) < 1e-7) {
+        memcpy(out, p1, 3 * sizeof(float));
+        return;
+    }
+    if (fabs((double)(t0 - t1)) < 1e-7) {
+        memcpy(out, p0, 3 * sizeof(float));
+        return;
+    }
+    double mu = (double)((iso - t0) / (t1 - t0));
+    if (mu > 1.0)
+        mu = 1.0;
+    else if (mu < 0)
+        mu = 0.0;
+    for (int a = 0; a < 3; ++a)
+        out[a] = (float)((double)p0[a] + mu * (double)(p1[a] - p0[a]));
+}
+
+/* Case of the cube at (i, j, k), computeLutIndex (src/MarchingCubes.cu:336-377):
+ * 0 when any corner has weight 0. */
+static int mc_case(const geom_t* g, const float* tsdf, const float* wt, int64_t i, int64_t j, int64_t k, float iso) {
+    size_t off[8];
+    for (int c = 0; c < 8; ++c)
+        off[c] = lin(g, i + kMcCorner[c][0], j + kMcCorner[c][1], k + kMcCorner[c][2]);
+    for (int c = 0; c < 8; ++c)
+        if (wt[off[c]] == 0.0f)
+            return 0;
+    int idx = 0;
+    for (int c = 0; c < 8; ++c)
+        if (tsdf[off[c]] > iso)
+            idx |= 1 << c;
+    return idx;
+}
+
+static void mc_edge_point(const geom_t* g, const float* tsdf, int64_t i, int64_t j, int64_t k, int e, float iso,
+                          float* out) {
+    const int a = kMcEdge[e][0], b = kMcEdge[e][1];
+    const int64_t ia = i + kMcCorner[a][0], ja = j + kMcCorner[a][1], ka = k + kMcCorner[a][2];
+    const int64_t ib = i + kMcCorner[b][0], jb = j + kMcCorner[b][1], kb = k + kMcCorner[b][2];
+    float pa[3], pb[3];
+    voxel_to_world(g, ia, ja, ka, pa);
+    voxel_to_world(g, ib, jb, kb, pb);
+    mc_interp(tsdf[lin(g, ia, ja, ka)], tsdf[lin(g, ib, jb, kb)], pa, pb, iso, out);
+}
+
+static int mc_edge_mask(int c) {
+    int m = 0;
+    for (int e = 0; e < 12; ++e)
+        if (((c >> kMcEdge[e][0]) ^ (c >> kMcEdge[e][1])) & 1)
+            m |= 1 << e;
+    return m;
+}
+
+/* marchingCubesMeshKernel (full == 0, src/MarchingCubes.cu:422-508: one
+ * vertex per crossed edge) and marchingCubesFullKernel (full == 1,
+ * :526-682 with computeTriangles :395-420: three vertices and one face per
+ * non-degenerate triangle).  Cubes at i < X-2, j < Y-2, k < Z-2, as the
+ * reference launches them.  The reference appends with atomics, so its order
+ * is a race; here the order is canonical: cubes by (k, j, i), edges by
+ * index, triangles in table order.  Writes at most cap_verts vertices and
+ * cap_faces faces (face indices count from 0) and returns the full counts. */
+void vfo_marching_cubes(int full, const float* tsdf, const float* wt, const vfo_vol* vol, float iso, float* verts,
+                        uint64_t cap_verts, int32_t* faces, uint64_t cap_faces, uint64_t* n_verts,
+                        uint64_t* n_faces) {
+    geom_t g;
+    geom_init(&g, vol);
+    uint64_t nv = 0, nf = 0;
+    for (int64_t k = 0; k + 2 < g.dim[2]; ++k)
+        for (int64_t j = 0; j + 2 < g.dim[1]; ++j)
+            for (int64_t i = 0; i + 2 < g.dim[0]; ++i) {
+                const int c = mc_case(&g, tsdf, wt, i, j, k, iso);
+                if (c == 0 || c == 255)
+                    continue;
+                const int mask = mc_edge_mask(c);
+                float ep[12][3];
+                for (int e = 0; e < 12; ++e)
+                    if (mask & (1 << e))
+                        mc_edge_point(&g, tsdf, i, j, k, e, iso, ep[e]);
+                if (!full) {
+                    for (int e = 0; e < 12; ++e)
+                        if (mask & (1 << e)) {
+                            if (nv < cap_verts)
+                                memcpy(verts + 3 * nv, ep[e], 3 * sizeof(float));
+                            ++nv;
+                        }
+                    continue;
+                }
+                const uint64_t t = kMcTri[c];
+                const int nt = (int)(t & 15);
+                for (int q = 0; q < nt; ++q) {
+                    const float* p[3];
+                    for (int v = 0; v < 3; ++v)
+                        p[v] = ep[(t >> (4 + 4 * (3 * q + v))) & 15];
+                    const int same01 = p[0][0] == p[1][0] && p[0][1] == p[1][1] && p[0][2] == p[1][2];
+                    const int same02 = p[0][0] == p[2][0] && p[0][1] == p[2][1] && p[0][2] == p[2][2];
+                    const int same12 = p[1][0] == p[2][0] && p[1][1] == p[2][1] && p[1][2] == p[2][2];
+                    if (same01 || same02 || same12)
+                        continue;
+                    for (int v = 0; v < 3; ++v) {
+                        if (nv + v < cap_verts)
+                            memcpy(verts + 3 * (nv + v), p[v], 3 * sizeof(float));
+                    }
+                    if (nf < cap_faces) {
+                        faces[3 * nf + 0] = (int32_t)nv;
+                        faces[3 * nf + 1] = (int32_t)(nv + 1);
+                        faces[3 * nf + 2] = (int32_t)(nv + 2);
+                    }
+                    nv += 3;
+                    nf += 1;
+                }
+            }
+    *n_verts = nv;
+    *n_faces = nf;
 }

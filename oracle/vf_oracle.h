@@ -130,6 +130,11 @@ void vfo_warp_mesh(int K, float* out, const float* pts, const float* T, const ui
 void vfo_warp_points(int K, float* out, const float* pts, size_t n, const float* nodes, const float* radii,
                      const float* T, size_t G, int blend);
 
+/* ---- marching cubes (src/MarchingCubes.cu), canonical output order -------- */
+void vfo_marching_cubes(int full, const float* tsdf, const float* wt, const vfo_vol* vol, float iso, float* verts,
+                        uint64_t cap_verts, int32_t* faces, uint64_t cap_faces, uint64_t* n_verts,
+                        uint64_t* n_faces);
+
 /* ---- dual-quaternion blending (second blend mode; not in the reference) --- */
 /* dq: 8 floats, real (x, y, z, w) then dual (x, y, z, w). */
 void vfo_mat4_to_dq(const float* M, float* dq);

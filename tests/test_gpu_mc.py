@@ -1,0 +1,95 @@
+"""GPU marching cubes against the oracle, bit for bit in canonical order."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from test_gpu_fused import dev
+from volumefusion_amd import _lib as L
+from volumefusion_amd import ops, scenes
+from volumefusion_amd.device import DeviceArray
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def sphere(n, vs, R, center=(0.0, 0.0, 0.0), trunc=0.1):
+    size = n * vs
+    c = np.arange(n, dtype=np.float64) * vs - size / 2
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2)
+    tsdf = np.clip((r - R) / trunc, -1, 1).astype(F).ravel()
+    wt = np.where(r < R + 2 * trunc, 1.0, 0.0).astype(F).ravel()
+    return tsdf, wt, (n, n, n), (F(size / n),) * 3, (F(size),) * 3
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("n", [20, 70])
+def test_sphere(vf, oracle, n, full):
+    tsdf, wt, dims, vs, size = sphere(n, 0.8 / n, 0.27)
+    v, f = ops.marching_cubes(ops.Workspace(), dev(tsdf), dev(wt), dims, vs, size, full=full)
+    vo, fo = oracle.marching_cubes(tsdf, wt, dims, vs, size, full=full)
+    assert v.shape[0] > 100
+    np.testing.assert_array_equal(v, vo)
+    if full:
+        np.testing.assert_array_equal(f, fo)
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_integrated_volume(vf, oracle, full):
+    """The TSDF the fused integration leaves after two frames (weight-0 voxels
+    outside the observed band), at a non-zero iso value."""
+    s = scenes.small_scene(seed=30, k=4, nodes=70)
+    tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
+    graph = ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii)
+    ws = ops.Workspace()
+    for f in range(2):
+        graph.transforms.upload(s.transforms(f))
+        ops.warped_integrate(ws, tsdf, graph, dev(s.depth(f)), s.intrinsics(), s.k, s.delta, s.max_weight, s.cube)
+    t, w = tsdf.tsdf.download(), tsdf.weights.download()
+    assert (w == 0).any() and (w > 0).sum() > 1000
+    v, fc = ops.marching_cubes(ws, tsdf.tsdf, tsdf.weights, s.dim, s.voxel_size, s.size, iso=0.1, full=full)
+    vo, fo = oracle.marching_cubes(t, w, s.dim, s.voxel_size, s.size, iso=0.1, full=full)
+    assert v.shape[0] > 50
+    np.testing.assert_array_equal(v, vo)
+    if full:
+        np.testing.assert_array_equal(fc, fo)
+
+
+def test_append_after_count_and_capacity(vf, oracle):
+    """Vertices go after the caller's count (the reference's counter
+    semantics); the counters advance by the full count even past the
+    capacity, and nothing is written there."""
+    tsdf, wt, dims, vs, size = sphere(24, 0.8 / 24, 0.27)
+    vo, fo = oracle.marching_cubes(tsdf, wt, dims, vs, size, full=True)
+    cap = 5 + vo.shape[0] // 2
+    mesh = DeviceArray((cap, 3), F)
+    mesh.fill(7.0)
+    faces = DeviceArray((cap, 3), np.int32)
+    counts = DeviceArray((2,), np.uint32)
+    counts.upload(np.array([5, 2], np.uint32))
+    L.call("vf_marching_cubes_full", ops.Workspace().handle, mesh.ptr, counts.ptr, cap, faces.ptr, counts.ptr + 4, cap,
+           dev(tsdf).ptr, dev(wt).ptr, L.i3(dims), L.f3(vs), L.f3(size), 0.0, None)
+    L.call("vf_device_synchronize")
+    nv, nf = counts.download()
+    assert nv == 5 + vo.shape[0] and nf == 2 + fo.shape[0]
+    m = mesh.download()
+    assert (m[:5] == 7.0).all()
+    np.testing.assert_array_equal(m[5:], vo[:cap - 5])
+    fc = faces.download()
+    k = min(cap - 2, fo.shape[0])
+    np.testing.assert_array_equal(fc[2:2 + k], fo[:k] + 5)
+
+
+def test_default_workspace_entry(vf, oracle):
+    """A NULL workspace (the af:: shims' path) uses the device default."""
+    tsdf, wt, dims, vs, size = sphere(16, 0.8 / 16, 0.27)
+    vo, _ = oracle.marching_cubes(tsdf, wt, dims, vs, size)
+    mesh = DeviceArray((vo.shape[0] + 10, 3), F)
+    counts = DeviceArray((1,), np.uint32)
+    counts.upload(np.zeros(1, np.uint32))
+    L.call("vf_marching_cubes_mesh", None, mesh.ptr, counts.ptr, mesh.shape[0], dev(tsdf).ptr, dev(wt).ptr,
+           L.i3(dims), L.f3(vs), L.f3(size), 0.0, None)
+    L.call("vf_device_synchronize")
+    assert counts.download()[0] == vo.shape[0]
+    np.testing.assert_array_equal(mesh.download()[:vo.shape[0]], vo)

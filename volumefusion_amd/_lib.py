@@ -175,6 +175,8 @@ _SIGS = {
     "vf_warp_mesh": [C.c_int, vp, vp, vp, vp, vp, sz, sz, vp],
     "vf_mesh_knn": [vp, C.c_int, vp, vp, vp, sz, vp, sz, C.c_int, vp],
     "vf_warp_points": [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
+    "vf_marching_cubes_mesh": [vp, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
+    "vf_marching_cubes_full": [vp, vp, vp, u64, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
     "vf_workspace_create": [C.POINTER(vp), C.c_int, u64, u64, vp],
     "vf_workspace_destroy": [vp],
     "vf_workspace_bytes": [vp],

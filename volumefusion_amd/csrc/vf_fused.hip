@@ -1379,10 +1379,27 @@ using namespace vf;
 
 namespace vf {
 
+namespace {
+std::mutex g_default_ws_mutex;
+std::vector<vf_workspace*> g_default_ws;
+}  // namespace
+
 // At least `bytes` of device scratch owned by the workspace; grows after the
-// stream's earlier work is done.  Contents are not preserved.
+// stream's earlier work is done.  Contents are not preserved.  A NULL
+// workspace selects the current device's default one (for entry points with
+// the reference's signatures; calls that share it must not overlap).
 int workspace_scratch(vf_workspace* ws, uint64_t bytes, void** ptr, hipStream_t stream) {
-    if (!ws) return fail(VF_ERR_INVALID_ARGUMENT, "NULL workspace");
+    if (!ws) {
+        int dev = 0;
+        VF_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lock(g_default_ws_mutex);
+        if ((int)g_default_ws.size() <= dev) g_default_ws.resize(dev + 1, nullptr);
+        if (!g_default_ws[dev]) {
+            g_default_ws[dev] = new vf_workspace();
+            g_default_ws[dev]->device = dev;
+        }
+        ws = g_default_ws[dev];
+    }
     if (bytes > ws->scratch_bytes) {
         VF_HIP(hipStreamSynchronize(stream));
         if (ws->scratch) VF_HIP(hipFree(ws->scratch));

@@ -347,7 +347,7 @@ int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, c
     int st = check_k(k);
     if (st) return st;
     if (!n) return VF_OK;
-    if (!ws || !knn_idxs || !knn_dists || !points || (graph_size && !nodes))
+    if (!knn_idxs || !knn_dists || !points || (graph_size && !nodes))
         return fail(VF_ERR_INVALID_ARGUMENT, "vf_mesh_knn: NULL argument");
     if (graph_size > 0xffffffffull) return fail(VF_ERR_INVALID_ARGUMENT, "graph_size exceeds 2^32-1");
     const uint32_t G = (uint32_t)graph_size;
@@ -381,7 +381,7 @@ int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const 
     if (blend != VF_BLEND_LBS && blend != VF_BLEND_DQB)
         return fail(VF_ERR_INVALID_ARGUMENT, "blend mode %d: supported VF_BLEND_LBS (0), VF_BLEND_DQB (1)", blend);
     if (!n) return VF_OK;
-    if (!ws || !warped || !points || (graph_size && (!nodes || !radii || !transforms)))
+    if (!warped || !points || (graph_size && (!nodes || !radii || !transforms)))
         return fail(VF_ERR_INVALID_ARGUMENT, "vf_warp_points: NULL argument");
     if (graph_size > 0xffffffffull) return fail(VF_ERR_INVALID_ARGUMENT, "graph_size exceeds 2^32-1");
     const uint32_t G = (uint32_t)graph_size;

@@ -174,6 +174,36 @@ def warp_points(ws: "Workspace", k: int, points: DeviceArray, graph: "DeviceMoti
     return out
 
 
+def marching_cubes(ws: "Workspace", tsdf: DeviceArray, weights: DeviceArray, dims, voxel_size, volume_size,
+                   iso: float = 0.0, full: bool = False, capacity: int | None = None,
+                   stream: Stream | None = None) -> tuple:
+    """af::runMarchingCubesMeshKernel / runMarchingCubesFullKernel
+    (include/af/MarchingCubes.cuh:10-28) into fresh buffers: (vertices
+    [n, 3] float32, faces [m, 3] int32 or None) on the host, canonical order."""
+    X, Y, Z = (int(d) for d in dims)
+    if capacity is None:
+        capacity = 15 * max(X - 2, 0) * max(Y - 2, 0) * max(Z - 2, 0) if full else \
+            12 * max(X - 2, 0) * max(Y - 2, 0) * max(Z - 2, 0)
+        capacity = max(min(capacity, 1 << 26), 1)
+    mesh = DeviceArray((capacity, 3), np.float32, stream)
+    counts = DeviceArray((2,), np.uint32, stream)
+    counts.upload(np.zeros(2, np.uint32))
+    if full:
+        faces = DeviceArray((capacity // 3 + 1, 3), np.int32, stream)
+        L.call("vf_marching_cubes_full", ws.handle, mesh.ptr, counts.ptr, int(capacity), faces.ptr, counts.ptr + 4,
+               int(faces.shape[0]), tsdf.ptr, weights.ptr, L.i3(dims), L.f3(voxel_size), L.f3(volume_size),
+               float(iso), _s(stream))
+    else:
+        L.call("vf_marching_cubes_mesh", ws.handle, mesh.ptr, counts.ptr, int(capacity), tsdf.ptr, weights.ptr,
+               L.i3(dims), L.f3(voxel_size), L.f3(volume_size), float(iso), _s(stream))
+    _done(stream)
+    nv, nf = (int(x) for x in counts.download())
+    if nv > capacity:
+        raise L.VFError(6, "marching_cubes", f"{nv} vertices exceed the capacity {capacity}")
+    verts = mesh.download()[:nv]
+    return verts, (faces.download()[:nf] if full else None)
+
+
 # ---------------------------------------------------------------------------
 # Reference-layout device containers
 # ---------------------------------------------------------------------------
