@@ -68,8 +68,9 @@ def test_append_after_count_and_capacity(vf, oracle):
     faces = DeviceArray((cap, 3), np.int32)
     counts = DeviceArray((2,), np.uint32)
     counts.upload(np.array([5, 2], np.uint32))
-    L.call("vf_marching_cubes_full", ops.Workspace().handle, mesh.ptr, counts.ptr, cap, faces.ptr, counts.ptr + 4, cap,
-           dev(tsdf).ptr, dev(wt).ptr, L.i3(dims), L.f3(vs), L.f3(size), 0.0, None)
+    ws, t_d, w_d = ops.Workspace(), dev(tsdf), dev(wt)  # alive across the call
+    L.call("vf_marching_cubes_full", ws.handle, mesh.ptr, counts.ptr, cap, faces.ptr, counts.ptr + 4, cap, t_d.ptr,
+           w_d.ptr, L.i3(dims), L.f3(vs), L.f3(size), 0.0, None)
     L.call("vf_device_synchronize")
     nv, nf = counts.download()
     assert nv == 5 + vo.shape[0] and nf == 2 + fo.shape[0]
@@ -88,8 +89,9 @@ def test_default_workspace_entry(vf, oracle):
     mesh = DeviceArray((vo.shape[0] + 10, 3), F)
     counts = DeviceArray((1,), np.uint32)
     counts.upload(np.zeros(1, np.uint32))
-    L.call("vf_marching_cubes_mesh", None, mesh.ptr, counts.ptr, mesh.shape[0], dev(tsdf).ptr, dev(wt).ptr,
-           L.i3(dims), L.f3(vs), L.f3(size), 0.0, None)
+    t_d, w_d = dev(tsdf), dev(wt)
+    L.call("vf_marching_cubes_mesh", None, mesh.ptr, counts.ptr, mesh.shape[0], t_d.ptr, w_d.ptr, L.i3(dims),
+           L.f3(vs), L.f3(size), 0.0, None)
     L.call("vf_device_synchronize")
     assert counts.download()[0] == vo.shape[0]
     np.testing.assert_array_equal(mesh.download()[:vo.shape[0]], vo)
