@@ -4,9 +4,10 @@
 // Replaces include/af/TSDFWarp.h and include/af/TSDFIntegration.cuh, and
 // af::runTranslateKernel, af::runComputeWarpWeightsKernel
 // (include/af/VertexManipulation.cuh:16, 19-23) and af::runWarpKernel
-// (include/af/ComputeCorrespondence.cuh:11-16) and the marching-cubes
-// kernels (include/af/MarchingCubes.cuh:10-28), for callers
-// such as af::runVolumeFusion (src/VolumeFusion.cu:334, 708-741) and
+// (include/af/ComputeCorrespondence.cuh:11-16), the depth-ingest kernels
+// (include/af/VertexManipulation.cuh:11-32, include/af/BilateralFilter.cuh)
+// and the marching-cubes kernels (include/af/MarchingCubes.cuh:10-28), for
+// callers such as af::runVolumeFusion (src/VolumeFusion.cu:334, 708-741) and
 // af::DeviceTsdf::integrate (include/af/DeviceTsdf.cuh:28-32).  Same names,
 // argument order and meaning, same Eigen types (spelled out, so this header
 // coexists with the reference's eigen_extension.h aliases Vecui<K>/Vecf<K>),
@@ -23,6 +24,8 @@
 #include <Eigen/Core>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
+#include <utility>
 #include <stdexcept>
 #include <string>
 
@@ -226,6 +229,50 @@ inline void runMarchingCubesFullKernel(Eigen::Vector3f* mesh, unsigned int* mesh
                                 reinterpret_cast<vf_int3*>(faces), facesSize, UINT64_MAX, cTsdf, cTsdfWeights,
                                 i3(cDims), f3(cVoxelSize), f3(cVolSize), cIsoValue, nullptr),
          "runMarchingCubesFullKernel");
+}
+
+// src/VertexManipulation.cu:17-23
+inline void runFilterDepthKernel(float* depth, unsigned int depthSize, float threshold,
+                                 float fillValue = std::numeric_limits<float>::infinity()) {
+    vf_detail::done(vf_filter_depth(depth, depthSize, threshold, fillValue, nullptr), "runFilterDepthKernel");
+}
+
+// src/VertexManipulation.cu:34-40
+inline void runMaskDepthKernel(float* depth, const bool* cDepthMask, unsigned int depthSize) {
+    static_assert(sizeof(bool) == 1, "bool masks are one byte");
+    vf_detail::done(vf_mask_depth(depth, reinterpret_cast<const uint8_t*>(cDepthMask), depthSize, nullptr),
+                    "runMaskDepthKernel");
+}
+
+// src/VertexManipulation.cu:65-71
+inline void runDepthToMeshKernel(Eigen::Vector3f* mesh, const float* cDepth, unsigned int depthWidth,
+                                 unsigned int depthHeight, const Eigen::Matrix3f& K) {
+    using namespace vf_detail;
+    done(vf_depth_to_mesh(reinterpret_cast<vf_float3*>(mesh), cDepth, depthWidth, depthHeight, m3(K), nullptr),
+         "runDepthToMeshKernel");
+}
+
+// src/VertexManipulation.cu:205-217
+inline void runDepthVerticesNormalsKernel(Eigen::Vector3f* normals, Eigen::Vector3f* vertices, const float* depth,
+                                          int h, int w, std::pair<float, float> normExclAngleRange) {
+    vf_detail::done(vf_depth_normals(reinterpret_cast<vf_float3*>(normals), reinterpret_cast<vf_float3*>(vertices),
+                                     depth, h, w, normExclAngleRange.first, normExclAngleRange.second, nullptr),
+                    "runDepthVerticesNormalsKernel");
+}
+
+// src/VertexManipulation.cu:234-241
+inline void runRemoveDepthWithoutNormalsKernel(Eigen::Vector3f* normals, Eigen::Vector3f* vertices, float* depth,
+                                               int h, int w) {
+    vf_detail::done(vf_remove_depth_without_normals(reinterpret_cast<vf_float3*>(normals),
+                                                    reinterpret_cast<vf_float3*>(vertices), depth, h, w, nullptr),
+                    "runRemoveDepthWithoutNormalsKernel");
+}
+
+// src/BilateralFilter.cu:185-193
+inline void bilateralFilterTextureOpmShared(float* output, const float* input, int height, int width, int r,
+                                            double sI, double sS, float thr) {
+    vf_detail::done(vf_bilateral_filter(output, input, height, width, r, sI, sS, thr, nullptr),
+                    "bilateralFilterTextureOpmShared");
 }
 
 // src/VertexManipulation.cu:81-87

@@ -157,6 +157,52 @@ int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const 
                    const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
                    vf_stream stream);
 
+/* ---- depth-frame ingest ---------------------------------------------------- */
+/* af::runFilterDepthKernel, include/af/VertexManipulation.cuh:11
+ * (src/VertexManipulation.cu:6-23): values above threshold, below 1e-4 or NaN
+ * become fill_value (the reference's default: +infinity). */
+int vf_filter_depth(float* depth, uint32_t depth_size, float threshold, float fill_value, vf_stream stream);
+/* af::runMaskDepthKernel (:12; src/VertexManipulation.cu:25-40): mask[i] == 0
+ * sets depth[i] to +infinity. */
+int vf_mask_depth(float* depth, const uint8_t* mask, uint32_t depth_size, vf_stream stream);
+/* af::bilateralFilterTextureOpmShared, include/af/BilateralFilter.cuh
+ * (src/BilateralFilter.cu:52-127, 185-193): clamp-to-edge neighbourhood of
+ * radius r <= 9, spatial and quantised intensity weights as the reference
+ * tabulates them; infinite centres stay infinite, infinite neighbours and
+ * neighbours differing by more than threshold are skipped.  output must not
+ * alias input. */
+int vf_bilateral_filter(float* output, const float* input, int height, int width, int radius, double sigma_i,
+                        double sigma_s, float threshold, vf_stream stream);
+/* af::runDepthToMeshKernel (:14; src/VertexManipulation.cu:42-71). */
+int vf_depth_to_mesh(vf_float3* mesh, const float* depth, uint32_t width, uint32_t height, vf_mat3 intrinsics,
+                     vf_stream stream);
+/* af::runDepthVerticesNormalsKernel (:25-30; src/VertexManipulation.cu:147-217):
+ * interior pixels; a normal whose angle to +z, acos(n.z), lies in
+ * (angle_from, angle_to) is zeroed.  The test is n.z < cos(angle_from) and
+ * n.z > cos(angle_to) in binary64, the reference's float acos being
+ * unavailable. */
+int vf_depth_normals(vf_float3* normals, vf_float3* vertices, const float* depth, int height, int width,
+                     float angle_from, float angle_to, vf_stream stream);
+/* af::runRemoveDepthWithoutNormalsKernel (:32; src/VertexManipulation.cu:219-241). */
+int vf_remove_depth_without_normals(vf_float3* normals, vf_float3* vertices, float* depth, int height, int width,
+                                    vf_stream stream);
+/* The frame centroid of src/VolumeFusion.cu:216-219: sum of all points over
+ * the number with z > 1e-4; the sum in binary64 in a fixed order (256-point
+ * binary trees, then the blocks in order), rounded to float.  Blocks on
+ * `stream` and returns the centroid and count on the host.  NULL workspace:
+ * the device default. */
+int vf_depth_centroid(vf_workspace* ws, vf_float3* centroid, uint64_t* count, const vf_float3* mesh, size_t n,
+                      vf_stream stream);
+/* The whole ingest of src/VolumeFusion.cu:190-219 in three launches: filter
+ * (threshold), bilateral filter (bilateral_radius 0 skips it), back-projection,
+ * normals, removal of points without normals, centroid.  depth is updated in
+ * place as the reference leaves it; vertices and normals are W*H arrays
+ * (border normals are not written).  Blocks on `stream` for the centroid. */
+int vf_ingest_depth(vf_workspace* ws, float* depth, int width, int height, vf_mat3 intrinsics, float threshold,
+                    int bilateral_radius, double sigma_i, double sigma_s, float bilateral_threshold,
+                    float angle_from, float angle_to, vf_float3* vertices, vf_float3* normals, vf_float3* centroid,
+                    uint64_t* count, vf_stream stream);
+
 /* ---- marching cubes ------------------------------------------------------- */
 /* af::runMarchingCubesMeshKernel, include/af/MarchingCubes.cuh:10-17
  * (src/MarchingCubes.cu:422-522): one vertex per crossed edge of every cube

@@ -126,6 +126,14 @@ def lib() -> C.CDLL:
         L.vfo_warp_points.argtypes = [C.c_int, _f32p, _f32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_size_t, C.c_int]
         L.vfo_marching_cubes.argtypes = [C.c_int, _f32p, _f32p, C.POINTER(Vol), C.c_float, _f32p, C.c_uint64,
                                          _i32p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.vfo_filter_depth.argtypes = [_f32p, C.c_size_t, C.c_float, C.c_float]
+        L.vfo_mask_depth.argtypes = [_f32p, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"), C.c_size_t]
+        L.vfo_depth_to_mesh.argtypes = [_f32p, _f32p, C.c_int, C.c_int, _f32p]
+        L.vfo_depth_normals.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.vfo_remove_depth_without_normals.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int]
+        L.vfo_depth_centroid.argtypes = [_f32p, C.c_size_t, _f32p, C.POINTER(C.c_uint64)]
+        L.vfo_bilateral_tables.argtypes = [C.c_int, C.c_double, _f32p, _i32p]
+        L.vfo_bilateral.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_float]
         L.vfo_mat4_to_dq.argtypes = [_f32p, _f32p]
         L.vfo_mat4_to_dq_f64.argtypes = [_f32p, _f64p]
         L.vfo_dqb_apply.argtypes = [C.POINTER(C.c_void_p), _f32p, C.c_int, _f32p, _f32p]
@@ -323,6 +331,60 @@ def marching_cubes(tsdf, wt, dim, vs, size, iso=0.0, full=False, cap=None):
     lib().vfo_marching_cubes(1 if full else 0, t, w, C.byref(v), float(iso), verts, nv.value, faces, nf.value,
                              C.byref(nv), C.byref(nf))
     return verts[:nv.value], (faces[:nf.value] if full else None)
+
+
+def filter_depth(depth, threshold, fill=np.inf):
+    d = _f32(depth).copy()
+    lib().vfo_filter_depth(d, d.size, float(threshold), float(fill))
+    return d
+
+
+def mask_depth(depth, mask):
+    d = _f32(depth).copy()
+    lib().vfo_mask_depth(d, np.ascontiguousarray(mask, np.uint8), d.size)
+    return d
+
+
+def depth_to_mesh(depth, Kin):
+    H, W = depth.shape
+    mesh = np.zeros((H * W, 3), np.float32)
+    lib().vfo_depth_to_mesh(mesh, _f32(depth), W, H, _f32(Kin))
+    return mesh
+
+
+def depth_normals(verts, H, W, a0, a1, normals=None):
+    n = np.zeros((H * W, 3), np.float32) if normals is None else normals
+    lib().vfo_depth_normals(n, _f32(verts), H, W, float(a0), float(a1))
+    return n
+
+
+def remove_depth_without_normals(normals, verts, depth):
+    H, W = depth.shape
+    v, d = _f32(verts).copy(), _f32(depth).copy()
+    lib().vfo_remove_depth_without_normals(_f32(normals), v, d, H, W)
+    return v, d
+
+
+def depth_centroid(mesh):
+    out = np.zeros(3, np.float32)
+    cnt = C.c_uint64(0)
+    m = _f32(mesh).reshape(-1, 3)
+    lib().vfo_depth_centroid(m, m.shape[0], out, C.byref(cnt))
+    return out, cnt.value
+
+
+def bilateral_tables(r, sd):
+    g = np.zeros(400, np.float32)
+    c = np.zeros(31, np.int32)
+    lib().vfo_bilateral_tables(int(r), float(sd), g, c)
+    return g[:(2 * r + 1) ** 2], c
+
+
+def bilateral(depth, r, sI, sS, thr):
+    H, W = depth.shape
+    out = np.zeros_like(_f32(depth))
+    lib().vfo_bilateral(out, _f32(depth), H, W, int(r), float(sI), float(sS), float(thr))
+    return out
 
 
 def mat4_to_dq(T16) -> np.ndarray:

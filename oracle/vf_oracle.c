@@ -1571,3 +1571,201 @@ void vfo_marching_cubes(int full, const float* tsdf, const float* wt, const vfo_
     *n_verts = nv;
     *n_faces = nf;
 }
+
+/* ------------------------------------------------------------------------ */
+/* depth ingest (src/VertexManipulation.cu:6-71, 147-217; BilateralFilter.cu) */
+/* ------------------------------------------------------------------------ */
+
+/* filterDepthKernel, src/VertexManipulation.cu:6-15. */
+void vfo_filter_depth(float* depth, size_t n, float threshold, float fill) {
+    for (size_t i = 0; i < n; ++i) {
+        const float v = depth[i];
+        if (v > threshold || v < 1.e-4f || v != v)
+            depth[i] = fill;
+    }
+}
+
+/* maskDepthKernel, src/VertexManipulation.cu:25-32. */
+void vfo_mask_depth(float* depth, const uint8_t* mask, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!mask[i])
+            depth[i] = INFINITY;
+}
+
+/* depthToMeshKernel, src/VertexManipulation.cu:42-63: back-projection with
+ * the reciprocals 1/fx, 1/fy; invalid depth gives (0, 0, 0).  K column-major. */
+void vfo_depth_to_mesh(float* mesh, const float* depth, int W, int H, const float* K) {
+    const float rfx = 1 / K[0], rfy = 1 / K[4];
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const float d = depth[i];
+            if (isinf(d) || d < 1.e-4f) {
+                mesh[3 * i] = mesh[3 * i + 1] = mesh[3 * i + 2] = 0.0f;
+                continue;
+            }
+            const float x0 = ((float)x - K[6]) * rfx;
+            const float y0 = ((float)y - K[7]) * rfy;
+            mesh[3 * i] = x0 * d;
+            mesh[3 * i + 1] = y0 * d;
+            mesh[3 * i + 2] = d;
+        }
+}
+
+/* depthVerticesNormalsKernel, src/VertexManipulation.cu:147-203, interior
+ * pixels only.  The angle test acos(n.z) in (a0, a1) is evaluated as
+ * n.z < cos(a0) and n.z > cos(a1) in binary64 (acos is decreasing); the
+ * reference's float acos is not available, so its boundary ulp is not
+ * pinned. */
+void vfo_depth_normals(float* normals, const float* v, int H, int W, float a0, float a1) {
+    const double c0 = cos((double)a0), c1 = cos((double)a1);
+    for (int y = 1; y < H - 1; ++y)
+        for (int x = 1; x < W - 1; ++x) {
+            const size_t i = (size_t)y * W + x;
+            float* nrm = normals + 3 * i;
+            if (v[3 * i + 2] < 1.e-4f) {
+                nrm[0] = nrm[1] = nrm[2] = 0.0f;
+                continue;
+            }
+            const float* xr = v + 3 * (i + 1);
+            const float* xl = v + 3 * (i - 1);
+            const float* yb = v + 3 * (i + W);
+            const float* yt = v + 3 * (i - W);
+            if (xr[2] < 1.e-4f || xl[2] < 1.e-4f || yb[2] < 1.e-4f || yt[2] < 1.e-4f) {
+                nrm[0] = nrm[1] = nrm[2] = 0.0f;
+                continue;
+            }
+            float dx[3], dy[3], gx[3], gy[3], c[3];
+            for (int a = 0; a < 3; ++a) {
+                dx[a] = xr[a] - xl[a];
+                dy[a] = yb[a] - yt[a];
+            }
+            const float nx = sqrtf(dx[0] * dx[0] + dx[1] * dx[1]);
+            const float ny = sqrtf(dy[0] * dy[0] + dy[1] * dy[1]);
+            for (int a = 0; a < 3; ++a) {
+                gx[a] = dx[a] / nx;
+                gy[a] = dy[a] / ny;
+            }
+            /* gY.cross(gX) */
+            c[0] = gy[1] * gx[2] - gy[2] * gx[1];
+            c[1] = gy[2] * gx[0] - gy[0] * gx[2];
+            c[2] = gy[0] * gx[1] - gy[1] * gx[0];
+            const float sq = c[0] * c[0] + (c[1] * c[1] + c[2] * c[2]);
+            if (sq > 0.0f) {
+                const float len = sqrtf(sq);
+                for (int a = 0; a < 3; ++a)
+                    c[a] = c[a] / len;
+            }
+            /* (0, 0, 1) . n in Eigen's order: 0 n.x + (0 n.y + 1 n.z) */
+            const float dot = 0.0f * c[0] + (0.0f * c[1] + 1.0f * c[2]);
+            if ((double)dot < c0 && (double)dot > c1) {
+                nrm[0] = nrm[1] = nrm[2] = 0.0f;
+                continue;
+            }
+            nrm[0] = c[0];
+            nrm[1] = c[1];
+            nrm[2] = c[2];
+        }
+}
+
+/* removeDepthWithoutNormalsKernel, src/VertexManipulation.cu:219-232. */
+void vfo_remove_depth_without_normals(const float* normals, float* v, float* depth, int H, int W) {
+    for (int y = 1; y < H - 1; ++y)
+        for (int x = 1; x < W - 1; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const float* nrm = normals + 3 * i;
+            if (fabsf(nrm[0]) <= 1.e-4f && fabsf(nrm[1]) <= 1.e-4f && fabsf(nrm[2]) <= 1.e-4f) {
+                v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0f;
+                depth[i] = INFINITY;
+            }
+        }
+}
+
+/* Centroid of the back-projected points (src/VolumeFusion.cu:216-219): the
+ * sum of all points divided by the number with z > 1e-4.  The reference sums
+ * in float in thrust's order; here in binary64 in a fixed order (256-point
+ * blocks reduced as a binary tree, a[i] += a[i + s] for s = 128 .. 1, then the
+ * block sums in order), rounded once to float. */
+void vfo_depth_centroid(const float* mesh, size_t n, float* out, uint64_t* count) {
+    double tot[3] = {0.0, 0.0, 0.0};
+    uint64_t cnt = 0;
+    for (size_t b = 0; b < n; b += 256) {
+        double blk[256][3];
+        for (int t = 0; t < 256; ++t)
+            for (int a = 0; a < 3; ++a)
+                blk[t][a] = (b + t < n) ? (double)mesh[3 * (b + t) + a] : 0.0;
+        for (int s = 128; s >= 1; s >>= 1)
+            for (int t = 0; t < s; ++t)
+                for (int a = 0; a < 3; ++a)
+                    blk[t][a] = blk[t][a] + blk[t + s][a];
+        for (int a = 0; a < 3; ++a)
+            tot[a] = tot[a] + blk[0][a];
+        for (int t = 0; t < 256 && b + t < n; ++t)
+            cnt += mesh[3 * (b + t) + 2] > 1.e-4f;
+    }
+    for (int a = 0; a < 3; ++a)
+        out[a] = (float)tot[a] / (float)cnt;
+    *count = cnt;
+}
+
+/* updateGaussian16 (src/BilateralFilter.cu:12-47): spatial weights
+ * expf(-(x^2) / (2 sd^2)) per offset, their 2D products; intensity weights
+ * as integers (int)(e^(-(i/10)^2 / 2) / e^(-4.5)), i = 0..30. */
+void vfo_bilateral_tables(int r, double sd, float* g2d, int32_t* clr) {
+    float g[64];
+    for (int i = 0; i < 2 * r + 1; ++i) {
+        const float x = (float)(i - r);
+        g[i] = expf((float)(-(x * x) / (2 * sd * sd)));
+    }
+    int idx = 0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx)
+            g2d[idx++] = g[dy + r] * g[dx + r];
+    float fc[31];
+    for (int i = 0; i <= 30; ++i)
+        fc[i] = (float)exp(-(pow(i / 10.0, 2)) / 2);
+    for (int i = 0; i <= 30; ++i)
+        clr[i] = (int32_t)(fc[i] / fc[30]);
+}
+
+/* gpuFilterTextureOpmShared (src/BilateralFilter.cu:52-127): clamp-to-edge
+ * neighbourhood of radius r; an infinite centre stays infinite; neighbours
+ * that are infinite or differ by more than thr are skipped; the rest weigh
+ * spatial(b, a) * intensity((int)|diff| 30/(3 sI)), summed with a (x offset)
+ * outer and b (y offset) inner. */
+void vfo_bilateral(float* out, const float* in, int H, int W, int r, double sI, double sS, float thr) {
+    float g2d[400];
+    int32_t clr[31];
+    vfo_bilateral_tables(r, sS, g2d, clr);
+    const float norm = (float)(30 / (3 * sI));
+    const int kd = 2 * r + 1;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const float c = in[(size_t)y * W + x];
+            if (isinf(c)) {
+                out[(size_t)y * W + x] = INFINITY;
+                continue;
+            }
+            float acc = 0.0f, wsum = 0.0f;
+            int ig = 0;
+            for (int a = 0; a < kd; ++a)
+                for (int b = 0; b < kd; ++b, ++ig) {
+                    int xx = x + a - r, yy = y + b - r;
+                    xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+                    yy = yy < 0 ? 0 : (yy >= H ? H - 1 : yy);
+                    const float p = in[(size_t)yy * W + xx];
+                    if (isinf(p))
+                        continue;
+                    const float diff = fabsf(c - p);
+                    if (diff > thr)
+                        continue;
+                    int x0 = vfo_cvt_i32_rz(fabsf(diff * norm));
+                    if (x0 > 30 || x0 < 0)
+                        x0 = 30;
+                    const float w = g2d[ig] * (float)clr[x0];
+                    acc = acc + p * w;
+                    wsum = wsum + w;
+                }
+            out[(size_t)y * W + x] = acc / wsum;
+        }
+}

@@ -130,6 +130,16 @@ void vfo_warp_mesh(int K, float* out, const float* pts, const float* T, const ui
 void vfo_warp_points(int K, float* out, const float* pts, size_t n, const float* nodes, const float* radii,
                      const float* T, size_t G, int blend);
 
+/* ---- depth ingest ----------------------------------------------------------- */
+void vfo_filter_depth(float* depth, size_t n, float threshold, float fill);
+void vfo_mask_depth(float* depth, const uint8_t* mask, size_t n);
+void vfo_depth_to_mesh(float* mesh, const float* depth, int W, int H, const float* K);
+void vfo_depth_normals(float* normals, const float* v, int H, int W, float a0, float a1);
+void vfo_remove_depth_without_normals(const float* normals, float* v, float* depth, int H, int W);
+void vfo_depth_centroid(const float* mesh, size_t n, float* out, uint64_t* count);
+void vfo_bilateral_tables(int r, double sd, float* g2d, int32_t* clr);
+void vfo_bilateral(float* out, const float* in, int H, int W, int r, double sI, double sS, float thr);
+
 /* ---- marching cubes (src/MarchingCubes.cu), canonical output order -------- */
 void vfo_marching_cubes(int full, const float* tsdf, const float* wt, const vfo_vol* vol, float iso, float* verts,
                         uint64_t cap_verts, int32_t* faces, uint64_t cap_faces, uint64_t* n_verts,

@@ -175,6 +175,15 @@ _SIGS = {
     "vf_warp_mesh": [C.c_int, vp, vp, vp, vp, vp, sz, sz, vp],
     "vf_mesh_knn": [vp, C.c_int, vp, vp, vp, sz, vp, sz, C.c_int, vp],
     "vf_warp_points": [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
+    "vf_filter_depth": [vp, u32, f32, f32, vp],
+    "vf_mask_depth": [vp, vp, u32, vp],
+    "vf_bilateral_filter": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, f32, vp],
+    "vf_depth_to_mesh": [vp, vp, u32, u32, Mat3, vp],
+    "vf_depth_normals": [vp, vp, vp, C.c_int, C.c_int, f32, f32, vp],
+    "vf_remove_depth_without_normals": [vp, vp, vp, C.c_int, C.c_int, vp],
+    "vf_depth_centroid": [vp, C.POINTER(Float3), C.POINTER(C.c_uint64), vp, sz, vp],
+    "vf_ingest_depth": [vp, vp, C.c_int, C.c_int, Mat3, f32, C.c_int, C.c_double, C.c_double, f32, f32, f32, vp, vp,
+                        C.POINTER(Float3), C.POINTER(C.c_uint64), vp],
     "vf_marching_cubes_mesh": [vp, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
     "vf_marching_cubes_full": [vp, vp, vp, u64, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
     "vf_workspace_create": [C.POINTER(vp), C.c_int, u64, u64, vp],

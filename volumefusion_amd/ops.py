@@ -174,6 +174,76 @@ def warp_points(ws: "Workspace", k: int, points: DeviceArray, graph: "DeviceMoti
     return out
 
 
+def run_filter_depth_kernel(depth: DeviceArray, depth_size: int, threshold: float, fill_value: float = np.inf,
+                            stream: Stream | None = None) -> None:
+    """af::runFilterDepthKernel (include/af/VertexManipulation.cuh:11)."""
+    L.call("vf_filter_depth", depth.ptr, int(depth_size), float(threshold), float(fill_value), _s(stream))
+    _done(stream)
+
+
+def run_mask_depth_kernel(depth: DeviceArray, mask: DeviceArray, depth_size: int,
+                          stream: Stream | None = None) -> None:
+    """af::runMaskDepthKernel (include/af/VertexManipulation.cuh:12); mask is uint8."""
+    L.call("vf_mask_depth", depth.ptr, mask.ptr, int(depth_size), _s(stream))
+    _done(stream)
+
+
+def bilateral_filter_texture_opm_shared(output: DeviceArray, input: DeviceArray, height: int, width: int, r: int,
+                                        sigma_i: float, sigma_s: float, threshold: float,
+                                        stream: Stream | None = None) -> None:
+    """af::bilateralFilterTextureOpmShared (src/BilateralFilter.cu:185-193)."""
+    L.call("vf_bilateral_filter", output.ptr, input.ptr, int(height), int(width), int(r), float(sigma_i),
+           float(sigma_s), float(threshold), _s(stream))
+    _done(stream)
+
+
+def run_depth_to_mesh_kernel(mesh: DeviceArray, depth: DeviceArray, width: int, height: int, intrinsics,
+                             stream: Stream | None = None) -> None:
+    """af::runDepthToMeshKernel (include/af/VertexManipulation.cuh:14)."""
+    L.call("vf_depth_to_mesh", mesh.ptr, depth.ptr, int(width), int(height), L.m3(intrinsics), _s(stream))
+    _done(stream)
+
+
+def run_depth_vertices_normals_kernel(normals: DeviceArray, vertices: DeviceArray, depth: DeviceArray, h: int,
+                                      w: int, angle_range=(1.3, 2.1), stream: Stream | None = None) -> None:
+    """af::runDepthVerticesNormalsKernel (include/af/VertexManipulation.cuh:25-30)."""
+    L.call("vf_depth_normals", normals.ptr, vertices.ptr, depth.ptr, int(h), int(w), float(angle_range[0]),
+           float(angle_range[1]), _s(stream))
+    _done(stream)
+
+
+def run_remove_depth_without_normals_kernel(normals: DeviceArray, vertices: DeviceArray, depth: DeviceArray, h: int,
+                                            w: int, stream: Stream | None = None) -> None:
+    """af::runRemoveDepthWithoutNormalsKernel (include/af/VertexManipulation.cuh:32)."""
+    L.call("vf_remove_depth_without_normals", normals.ptr, vertices.ptr, depth.ptr, int(h), int(w), _s(stream))
+    _done(stream)
+
+
+def depth_centroid(ws: "Workspace | None", mesh: DeviceArray, stream: Stream | None = None) -> tuple:
+    """Frame centroid (src/VolumeFusion.cu:216-219): (float32[3], count)."""
+    c, n = L.Float3(), C.c_uint64(0)
+    L.call("vf_depth_centroid", ws.handle if ws is not None else None, C.byref(c), C.byref(n), mesh.ptr,
+           int(mesh.shape[0]), _s(stream))
+    return np.array([c.x, c.y, c.z], np.float32), int(n.value)
+
+
+def ingest_depth(ws: "Workspace", depth: DeviceArray, intrinsics, threshold: float = 1.0, bilateral=(5, 75.0, 75.0,
+                 0.025), angle_range=(1.3, 2.1), stream: Stream | None = None) -> dict:
+    """The frame ingest of src/VolumeFusion.cu:190-219 in three launches;
+    depth [H, W] is updated in place.  bilateral=None skips the filter."""
+    H, W = depth.shape
+    verts = DeviceArray((H * W, 3), np.float32, stream)
+    normals = DeviceArray((H * W, 3), np.float32, stream)
+    normals.fill(0.0)
+    r, si, ss, bt = bilateral if bilateral is not None else (0, 1.0, 1.0, 0.0)
+    c, n = L.Float3(), C.c_uint64(0)
+    L.call("vf_ingest_depth", ws.handle, depth.ptr, int(W), int(H), L.m3(intrinsics), float(threshold), int(r),
+           float(si), float(ss), float(bt), float(angle_range[0]), float(angle_range[1]), verts.ptr, normals.ptr,
+           C.byref(c), C.byref(n), _s(stream))
+    return {"vertices": verts, "normals": normals, "centroid": np.array([c.x, c.y, c.z], np.float32),
+            "count": int(n.value)}
+
+
 def marching_cubes(ws: "Workspace", tsdf: DeviceArray, weights: DeviceArray, dims, voxel_size, volume_size,
                    iso: float = 0.0, full: bool = False, capacity: int | None = None,
                    stream: Stream | None = None) -> tuple:
