@@ -203,6 +203,36 @@ int vf_ingest_depth(vf_workspace* ws, float* depth, int width, int height, vf_ma
                     float angle_from, float angle_to, vf_float3* vertices, vf_float3* normals, vf_float3* centroid,
                     uint64_t* count, vf_stream stream);
 
+/* ---- TSDF volumes on disk ------------------------------------------------- */
+/* VF_TSDF_FORMAT_NATIVE: a 64-byte header ("VFTSDF", version 1, header size,
+ * dims, size, centre, delta, max weight, flags) then tsdf[N] and weight[N],
+ * little-endian float32, z-major.  VF_TSDF_FORMAT_LEGACY: the reference's
+ * TSDFVolume::save layout (src/TSDFVolume.cpp:339-358) with its 24-byte size
+ * field written as the float size plus 12 zero bytes (the reference writes a
+ * Vec3f with sizeof(double) * 3, :344) and zero colour arrays; loading reads
+ * only the first 12 bytes of that field and skips the colours. */
+enum { VF_TSDF_FORMAT_NATIVE = 1, VF_TSDF_FORMAT_LEGACY = 2 };
+
+typedef struct vf_tsdf_info {
+    vf_int3 dim;
+    vf_float3 size;
+    vf_float3 center;     /* native only (0 from legacy files)              */
+    float delta;
+    float max_weight;     /* native only                                    */
+    int32_t format;       /* set by vf_tsdf_load                            */
+} vf_tsdf_info;
+
+/* Device arrays tsdf/weights of info->dim voxels to `path` (streamed through
+ * a bounded host buffer; blocks). */
+int vf_tsdf_save(const char* path, int format, const vf_tsdf_info* info, const float* tsdf, const float* weights);
+/* Reads the header into *info and, when tsdf and weights are not NULL, the
+ * volume into device arrays of at least `capacity` voxels. */
+int vf_tsdf_load(const char* path, vf_tsdf_info* info, float* tsdf, float* weights, uint64_t capacity);
+/* The same with host arrays (no device needed). */
+int vf_tsdf_save_host(const char* path, int format, const vf_tsdf_info* info, const float* tsdf,
+                      const float* weights);
+int vf_tsdf_load_host(const char* path, vf_tsdf_info* info, float* tsdf, float* weights, uint64_t capacity);
+
 /* ---- marching cubes ------------------------------------------------------- */
 /* af::runMarchingCubesMeshKernel, include/af/MarchingCubes.cuh:10-17
  * (src/MarchingCubes.cu:422-522): one vertex per crossed edge of every cube
@@ -469,6 +499,11 @@ int vf_server_drain(vf_server* server);
  * are done (any pointer may be NULL). */
 int vf_server_read(vf_server* server, int32_t stream_id, float* tsdf, float* weights, vf_float3* points);
 int vf_server_get_stats(vf_server* server, vf_server_stats* stats);
+/* Checkpoint a stream's volume to a TSDF file once its frames are done, and
+ * restore it from one (same dims) into an open stream: resume after a
+ * restart.  Node positions and transforms are not part of the file. */
+int vf_server_save(vf_server* server, int32_t stream_id, const char* path, int format);
+int vf_server_restore(vf_server* server, int32_t stream_id, const char* path);
 
 /* ------------------------------------------------------------------------ */
 /* Device plumbing for hosts without a GPU framework (Python via ctypes)    */

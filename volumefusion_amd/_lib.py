@@ -98,6 +98,14 @@ BRICK_SHAPES = [(16, 8, 4), (16, 4, 4), (8, 8, 4), (8, 8, 8), (32, 4, 4)]  # VF_
 PLAN_GRAPH = 1
 
 
+class TsdfInfo(C.Structure):
+    _fields_ = [("dim", Int3), ("size", Float3), ("center", Float3), ("delta", C.c_float), ("max_weight", C.c_float),
+                ("format", C.c_int32)]
+
+
+TSDF_FORMAT_NATIVE, TSDF_FORMAT_LEGACY = 1, 2
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("k", C.c_int32), ("max_batch", C.c_int32), ("lanes", C.c_int32),
                 ("device", C.c_int32)]
@@ -184,6 +192,10 @@ _SIGS = {
     "vf_depth_centroid": [vp, C.POINTER(Float3), C.POINTER(C.c_uint64), vp, sz, vp],
     "vf_ingest_depth": [vp, vp, C.c_int, C.c_int, Mat3, f32, C.c_int, C.c_double, C.c_double, f32, f32, f32, vp, vp,
                         C.POINTER(Float3), C.POINTER(C.c_uint64), vp],
+    "vf_tsdf_save": [C.c_char_p, C.c_int, C.POINTER(TsdfInfo), vp, vp],
+    "vf_tsdf_load": [C.c_char_p, C.POINTER(TsdfInfo), vp, vp, u64],
+    "vf_tsdf_save_host": [C.c_char_p, C.c_int, C.POINTER(TsdfInfo), vp, vp],
+    "vf_tsdf_load_host": [C.c_char_p, C.POINTER(TsdfInfo), vp, vp, u64],
     "vf_marching_cubes_mesh": [vp, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
     "vf_marching_cubes_full": [vp, vp, vp, u64, vp, vp, u64, vp, vp, Int3, Float3, Float3, f32, vp],
     "vf_workspace_create": [C.POINTER(vp), C.c_int, u64, u64, vp],
@@ -217,6 +229,8 @@ _SIGS = {
     "vf_server_drain": [vp],
     "vf_server_read": [vp, C.c_int32, vp, vp, vp],
     "vf_server_get_stats": [vp, C.POINTER(ServerStats)],
+    "vf_server_save": [vp, C.c_int32, C.c_char_p, C.c_int],
+    "vf_server_restore": [vp, C.c_int32, C.c_char_p],
     "vf_device_count": [C.POINTER(C.c_int)],
     "vf_set_device": [C.c_int],
     "vf_malloc": [C.POINTER(vp), u64],

@@ -598,6 +598,41 @@ int vf_server_read(vf_server* sv, int32_t sid, float* tsdf, float* weights, vf_f
     return VF_OK;
 }
 
+int vf_server_save(vf_server* sv, int32_t sid, const char* path, int format) {
+    if (!sv || !path) return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_save: NULL argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (sid < 0 || (size_t)sid >= sv->streams.size() || !sv->streams[sid]->open)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_save: stream %d is not open", sid);
+    sv->cv_done.wait(lk, [&] { return !sv->streams[sid]->busy && sv->streams[sid]->queue.empty(); });
+    const StreamState& s = *sv->streams[sid];
+    vf_tsdf_info info;
+    memset(&info, 0, sizeof(info));
+    info.dim = s.desc.dim;
+    info.size = s.desc.size;
+    info.center = s.desc.center;
+    info.delta = s.desc.delta;
+    info.max_weight = s.desc.max_weight;
+    VF_HIP(hipSetDevice(sv->cfg.device));
+    return vf_tsdf_save(path, format, &info, s.tsdf, s.wt);
+}
+
+int vf_server_restore(vf_server* sv, int32_t sid, const char* path) {
+    if (!sv || !path) return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_restore: NULL argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (sid < 0 || (size_t)sid >= sv->streams.size() || !sv->streams[sid]->open)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_restore: stream %d is not open", sid);
+    sv->cv_done.wait(lk, [&] { return !sv->streams[sid]->busy && sv->streams[sid]->queue.empty(); });
+    const StreamState& s = *sv->streams[sid];
+    vf_tsdf_info info;
+    int st = vf_tsdf_load_host(path, &info, nullptr, nullptr, 0);
+    if (st) return st;
+    if (info.dim.x != s.desc.dim.x || info.dim.y != s.desc.dim.y || info.dim.z != s.desc.dim.z)
+        return fail(VF_ERR_INVALID_ARGUMENT, "%s holds a %dx%dx%d volume; stream %d is %dx%dx%d", path, info.dim.x,
+                    info.dim.y, info.dim.z, sid, s.desc.dim.x, s.desc.dim.y, s.desc.dim.z);
+    VF_HIP(hipSetDevice(sv->cfg.device));
+    return vf_tsdf_load(path, &info, s.tsdf, s.wt, voxel_count(s.desc.dim));
+}
+
 int vf_server_get_stats(vf_server* sv, vf_server_stats* out) {
     if (!sv || !out) return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_get_stats: NULL argument");
     memset(out, 0, sizeof(*out));

@@ -305,6 +305,19 @@ class DeviceTsdf:
         self.tsdf.fill(-1.0)
         self.weights.fill(0.0)
 
+    def save(self, path: str, fmt: int = L.TSDF_FORMAT_NATIVE, max_weight: float = 15.0) -> None:
+        """Checkpoint to disk (the reference's TSDFVolume::save layout with
+        fmt=TSDF_FORMAT_LEGACY, src/TSDFVolume.cpp:339-358)."""
+        from . import tsdf_io
+
+        tsdf_io.save_device(path, self, fmt, max_weight)
+
+    def load(self, path: str) -> dict:
+        """Resume from disk; the file's dims must fit this volume."""
+        from . import tsdf_io
+
+        return tsdf_io.load_device(path, self)
+
     def integrate(self, depth: DeviceArray, intrinsics, max_weight: float = 15.0,
                   stream: Stream | None = None) -> None:
         """DeviceTsdf::integrate (include/af/DeviceTsdf.cuh:28-32)."""

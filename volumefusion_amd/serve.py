@@ -107,6 +107,12 @@ class Server:
         L.call("vf_server_read", self.handle, int(sid), t.ctypes.data, w.ctypes.data, p.ctypes.data)
         return t, w, p
 
+    def save(self, sid: int, path: str, fmt: int = L.TSDF_FORMAT_NATIVE) -> None:
+        L.call("vf_server_save", self.handle, int(sid), str(path).encode(), int(fmt))
+
+    def restore(self, sid: int, path: str) -> None:
+        L.call("vf_server_restore", self.handle, int(sid), str(path).encode())
+
     def stats(self) -> dict:
         st = L.ServerStats()
         L.call("vf_server_get_stats", self.handle, C.byref(st))
