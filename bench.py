@@ -19,6 +19,8 @@ JSON line:
   batching   config 4: 16 volumes of uneven size, one frame each: 16 plain
              per-volume calls per step against one autotuned frame plan
              replayed as a HIP graph (the same work, results identical).
+  extras     on the config-3 state: marching cubes of the integrated volume,
+             the fused depth ingest of a frame, the fused point warp.
   serving    config 5: 32 resident streams, Poisson frame arrivals at 30 fps
              each, continuous batching; p50/p99 frame latency (submission to
              completion) and aggregate updates/s; then the same streams
@@ -61,6 +63,7 @@ def parse_args():
     ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson arrival run")
     ap.add_argument("--no-tune", action="store_true", help="default brick shape instead of the autotuned one")
+    ap.add_argument("--no-extras", action="store_true", help="skip marching cubes, ingest and point warp")
     return ap.parse_args()
 
 
@@ -120,6 +123,89 @@ def _pct(values, q):
         return None
     i = min(len(v) - 1, max(0, int(math.ceil(q / 100.0 * len(v))) - 1))
     return v[i]
+
+
+def bench_extras(s, tsdf, graph, depth_np, stream) -> dict:
+    """The rows beside the path on the headline workload's state: marching
+    cubes over the integrated 512^3 volume, the fused depth ingest of a frame,
+    and the fused point warp of the frame's points through the motion graph."""
+    from volumefusion_amd import _lib as L
+    from volumefusion_amd import ops
+    from volumefusion_amd.device import DeviceArray, Event
+
+    out = {}
+    ws = ops.Workspace(stream=stream)
+    e0, e1 = Event(), Event()
+    reps = 3
+
+    # marching cubes (full: vertices and faces)
+    X, Y, Z = s.dim
+    cap = 64 << 20
+    mesh = DeviceArray((cap, 3), np.float32, stream)
+    faces = DeviceArray((cap // 3, 3), np.int32, stream)
+    counts = DeviceArray((2,), np.uint32, stream)
+
+    def mc():
+        counts.upload(np.zeros(2, np.uint32))
+        L.call("vf_marching_cubes_full", ws.handle, mesh.ptr, counts.ptr, cap, faces.ptr, counts.ptr + 4, cap // 3,
+               tsdf.tsdf.ptr, tsdf.weights.ptr, L.i3(s.dim), L.f3(s.voxel_size), L.f3(s.size), 0.0, stream.ptr)
+
+    mc()
+    stream.synchronize()
+    ms = 0.0
+    for _ in range(reps):
+        counts.upload(np.zeros(2, np.uint32))
+        e0.record(stream)
+        L.call("vf_marching_cubes_full", ws.handle, mesh.ptr, counts.ptr, cap, faces.ptr, counts.ptr + 4, cap // 3,
+               tsdf.tsdf.ptr, tsdf.weights.ptr, L.i3(s.dim), L.f3(s.voxel_size), L.f3(s.size), 0.0, stream.ptr)
+        e1.record(stream)
+        stream.synchronize()
+        ms += e0.elapsed_ms(e1)
+    ms /= reps
+    nv, nf = (int(x) for x in counts.download())
+    alg = 8.0 * X * Y * Z + 12.0 * nv + 12.0 * nf
+    out["marching_cubes"] = {"vertices": nv, "faces": nf, "ms": ms, "algorithmic_bytes": alg,
+                             "achieved_GBps": alg / (ms * 1e-3) / 1e9, "frac_hbm": alg / (ms * 1e-3) / HBM_PEAK}
+    del mesh, faces
+
+    # depth ingest (filter, bilateral r=5, back-projection, normals, removal, centroid)
+    H, W = depth_np.shape
+    d = DeviceArray((H, W), np.float32, stream)
+    verts = DeviceArray((H * W, 3), np.float32, stream)
+    normals = DeviceArray((H * W, 3), np.float32, stream)
+    cen, cnt = L.Float3(), C.c_uint64(0)
+    K = L.m3(s.intrinsics())
+    ms = 0.0
+    for r in range(reps + 1):
+        d.upload(depth_np)
+        t0 = time.perf_counter()
+        L.call("vf_ingest_depth", ws.handle, d.ptr, W, H, K, 2.0, 5, 75.0, 75.0, 0.025, 1.3, 2.1, verts.ptr,
+               normals.ptr, C.byref(cen), C.byref(cnt), stream.ptr)
+        if r:
+            ms += (time.perf_counter() - t0) * 1e3
+    ms /= reps
+    alg = 32.0 * H * W
+    out["depth_ingest"] = {"pixels": H * W, "points": int(cnt.value), "ms_wall_incl_centroid_readback": ms,
+                           "algorithmic_bytes": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9}
+
+    # point warp: the frame's points through the 4096-node graph, k=8
+    pts = verts
+    npts = H * W
+    warped = DeviceArray((npts, 3), np.float32, stream)
+    L.call("vf_warp_points", ws.handle, s.k, 0, warped.ptr, pts.ptr, npts, graph.points.ptr, graph.radii.ptr,
+           graph.transforms.ptr, graph.count, stream.ptr)
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        L.call("vf_warp_points", ws.handle, s.k, 0, warped.ptr, pts.ptr, npts, graph.points.ptr, graph.radii.ptr,
+               graph.transforms.ptr, graph.count, stream.ptr)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    out["point_warp"] = {"points": npts, "nodes": int(graph.count), "k": s.k, "ms": ms,
+                         "points_per_s": npts / (ms * 1e-3),
+                         "node_tests_per_s": npts * graph.count / (ms * 1e-3)}
+    return out
 
 
 def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
@@ -507,6 +593,12 @@ def main() -> int:
         },
         "dense_equivalent_voxels_per_s": s.n_voxels * args.steps / wall,
     }
+    if not args.no_extras:
+        result["extras"] = bench_extras(s, tsdf, graph, depths[0], stream)
+        ex = result["extras"]
+        print(f"extras: marching cubes {ex['marching_cubes']['ms']:.3f} ms, ingest "
+              f"{ex['depth_ingest']['ms_wall_incl_centroid_readback']:.3f} ms, point warp "
+              f"{ex['point_warp']['ms']:.3f} ms", file=sys.stderr, flush=True)
     del tsdf, graph, ws
     if c4 is not None:
         off = 2
