@@ -90,6 +90,37 @@ def test_knn_graph_spilled_brick_lists(vf, oracle):
     assert over > 100
 
 
+def test_knn_graph_nodes_on_voxels(vf, oracle):
+    """Nodes exactly on voxel positions and a hair away from them: squared
+    distances of 0, below 2^-96 and subnormal take the square root's slow
+    path; the distances and the order they give match bit for bit."""
+    dim, k = (24, 20, 16), 8
+    vsz = (0.005,) * 3
+    size = tuple(float(F(d) * F(v)) for d, v in zip(dim, vsz))
+    half = np.array(size, F) * F(0.5)
+    # voxel (12, 10, 8) sits at the origin; (5, 3, 9) at an exact float position
+    on = (np.array([5, 3, 9], F) * np.array(vsz, F)).astype(F) - half
+    nodes = np.array([[0, 0, 0], [1e-17, 0, 0], [0, -1e-20, 0], [0, 0, 3e-30], on,
+                      on + np.array([0, 0, 1e-19], F), [1e-10, 1e-10, 1e-10]], F)
+    rng = np.random.default_rng(4)
+    nodes = np.concatenate([nodes, rng.uniform(-0.04, 0.04, (20, 3)).astype(F)])
+    G = nodes.shape[0]
+    radii = np.full(G, 0.02, F)
+    cube = scenes.voxels_per_node(0.02, vsz)
+    idx, dist, cnt, _ = oracle.knn_graph(k, nodes, radii, dim, vsz, size, cube)
+    n = int(np.prod(dim))
+    gi, gd, gc = DeviceArray((n, k), np.uint32), DeviceArray((n, k), F), DeviceArray((n,), np.int32)
+    gi.fill(0), gd.fill(0), gc.fill(0)
+    g_nodes, g_radii = dev(nodes), dev(radii)
+    ops.run_tsdf_knn_graph(gi, gd, gc, g_nodes, g_radii, G, vsz, size, dim, cube)
+    np.testing.assert_array_equal(gc.download(), cnt)
+    mask = np.arange(k)[None, :] < cnt[:, None]
+    np.testing.assert_array_equal(gi.download()[mask], idx[mask])
+    np.testing.assert_array_equal(gd.download()[mask], dist[mask])
+    d = dist[mask]
+    assert (d == 0).any() and ((d > 0) & (d < 2.0 ** -48)).any()
+
+
 def test_knn_graph_empty_and_outside(vf, oracle):
     s = _setup(4)
     n = s.n_voxels

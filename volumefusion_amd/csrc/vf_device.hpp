@@ -22,6 +22,26 @@ __device__ __forceinline__ int32_t cvt_i32_rz(float f) {
     return (int32_t)f;
 }
 
+// Correctly rounded square root, the same bits as __builtin_sqrtf.  For
+// s >= 2^-96 the compiler's expansion of __builtin_sqrtf on gfx950 is
+// v_sqrt_f32 (within 1 ulp) followed by a one-ulp correction: with
+// r- = r - ulp and r+ = r + ulp, the residuals fma(-r-, r, s) and
+// fma(-r+, r, s) decide between r-, r and r+.  This is that sequence without
+// the scaling by 2^32 that only inputs below 2^-96 need and without the
+// zero/infinity class test (the correction already leaves 0 and +inf
+// unchanged).  Inputs below 2^-96, zero and NaN take __builtin_sqrtf itself.
+__device__ __forceinline__ float sqrt_rn(float s) {
+    float r = __builtin_amdgcn_sqrtf(s);
+    const float dn = __int_as_float(__float_as_int(r) - 1);
+    const float up = __int_as_float(__float_as_int(r) + 1);
+    const float vd = __builtin_fmaf(-dn, r, s);
+    const float vu = __builtin_fmaf(-up, r, s);
+    r = (vd <= 0.0f) ? dn : r;
+    r = (vu > 0.0f) ? up : r;
+    if (__builtin_expect(!(s >= 0x1p-96f), 0)) r = __builtin_sqrtf(s);
+    return r;
+}
+
 // 2^(j/32), j = 0..31, correctly rounded to binary64.
 #define VF_EXP2_TABLE                                                                                  \
     0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,          \
@@ -168,13 +188,38 @@ struct KnnSlots {
         }
     }
 
-    // Sorted selection into registers: d[s], i[s] for s < n.
+    // The first Q slots, empty ones as ~0 (after every real key), sorted.
+    template <int Q, int P>
+    __device__ __forceinline__ void load_sort(unsigned long long* k) const {
+#pragma unroll
+        for (int s = 0; s < Q; ++s) k[s] = (s < K && s < n) ? slot[s * STRIDE] : ~0ull;
+#pragma unroll
+        for (int s = Q; s < P; ++s) k[s] = ~0ull;
+        sort_keys<Q>(k);
+    }
+
+    // Sorted selection into registers: d[s], i[s] for s < n.  The network is
+    // the smallest power of two that holds the largest n of the wave; slots
+    // at or beyond a lane's n are never read by the caller.  Called by whole
+    // waves (the size test is a ballot).
     __device__ __forceinline__ void finish(float* d, uint32_t* i) const {
         constexpr int P = pow2_at_least(K);
         unsigned long long k[P];
-#pragma unroll
-        for (int s = 0; s < P; ++s) k[s] = (s < K && s < n) ? slot[s * STRIDE] : ~0ull;
-        sort_keys<P>(k);
+        if constexpr (P == 8) {
+            if (__ballot(n > 4))
+                load_sort<8, P>(k);
+            else if (__ballot(n > 2))
+                load_sort<4, P>(k);
+            else
+                load_sort<2, P>(k);
+        } else if constexpr (P == 4) {
+            if (__ballot(n > 2))
+                load_sort<4, P>(k);
+            else
+                load_sort<2, P>(k);
+        } else {
+            load_sort<P, P>(k);
+        }
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             d[s] = __uint_as_float((uint32_t)(k[s] >> 32));

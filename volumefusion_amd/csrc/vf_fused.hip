@@ -265,8 +265,8 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     constexpr int kWinTiles = S::kWinTiles;
     __shared__ uint32_t s_raw[kCap];
     __shared__ uint32_t s_ids[kCap];
-    __shared__ float4 s_m[kCap];
-    __shared__ int4 s_lo[kCap];
+    __shared__ float4 s_m[kCap + 1];
+    __shared__ int4 s_lo[kCap + 1];
     __shared__ float4 s_t0[kCap];
     __shared__ float4 s_t1[kCap];
     __shared__ float4 s_t2[kCap];
@@ -379,19 +379,23 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 s_w[rank] = make_float2(pw.x, pw.y);
             }
             __syncthreads();
-            if (valid) {
-                for (uint32_t e = 0; e < cnt; ++e) {
-                    const int4 l = s_lo[e];
-                    const bool inside = ((uint32_t)(x - l.x) < cx) & ((uint32_t)(y - l.y) < cy) &
-                                        ((uint32_t)(z - l.z) < cz);
-                    if (inside) {
-                        const float4 nm = s_m[e];
-                        const float dx = p[0] - nm.x, dy = p[1] - nm.y, dz = p[2] - nm.z;
-                        const float s = dx * dx + (dy * dy + dz * dz);
-                        const float d = __builtin_sqrtf(s);
-                        if (!(d > nm.w)) ks.offer(d, e);
-                    }
-                }
+            // Branch-free test of every list entry, the next entry's LDS
+            // reads issued before this one's arithmetic (the arrays hold one
+            // slot beyond kCap, so the read after the last entry stays inside).
+            int4 l = s_lo[0];
+            float4 nm = s_m[0];
+#pragma unroll 2
+            for (uint32_t e = 0; e < cnt; ++e) {
+                const int4 lc = l;
+                const float4 mc = nm;
+                l = s_lo[e + 1];
+                nm = s_m[e + 1];
+                const bool inside = valid & ((uint32_t)(x - lc.x) < cx) & ((uint32_t)(y - lc.y) < cy) &
+                                    ((uint32_t)(z - lc.z) < cz);
+                const float dx = p[0] - mc.x, dy = p[1] - mc.y, dz = p[2] - mc.z;
+                const float s = dx * dx + (dy * dy + dz * dz);
+                const float d = sqrt_rn(s);
+                if (inside & !(d > mc.w)) ks.offer(d, e);
             }
         } else {
             // the list spilled: walk every node of the volume in chunks
@@ -416,25 +420,13 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                             const float4 nm = s_m[e];
                             const float dx = p[0] - nm.x, dy = p[1] - nm.y, dz = p[2] - nm.z;
                             const float s = dx * dx + (dy * dy + dz * dz);
-                            const float d = __builtin_sqrtf(s);
+                            const float d = sqrt_rn(s);
                             if (!(d > nm.w)) ks.offer(d, base + e);
                         }
                     }
                 }
             }
         }
-        // next brick's node record for this thread's list slot
-        float4 pm_n = z4, plo_n = z4, pt0_n = z4, pt1_n = z4, pt2_n = z4, pw_n = z4;
-        if (item_n != kNone && cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) {
-            const float4* r = (const float4*)(nrec + nbase_n + id_n);
-            pm_n = r[0];
-            plo_n = r[1];
-            pt0_n = r[2];
-            pt1_n = r[3];
-            pt2_n = r[4];
-            pw_n = r[5];
-        }
-
         if (tid == 0) {
             brick_count[gb] = 0u;
             if (spill) s_stat[3] += 1ull;
@@ -448,7 +440,23 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         } kl;
         ks.finish(kl.d, kl.i);
 
+        // the next brick's node record for this thread's list slot
+        float4 pm_n = z4, plo_n = z4, pt0_n = z4, pt1_n = z4, pt2_n = z4, pw_n = z4;
+#define VF_FETCH_NEXT_RECORD()                                                         \
+    do {                                                                               \
+        if (item_n != kNone && cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) {     \
+            const float4* r = (const float4*)(nrec + nbase_n + id_n);                  \
+            pm_n = r[0];                                                               \
+            plo_n = r[1];                                                              \
+            pt0_n = r[2];                                                              \
+            pt1_n = r[3];                                                              \
+            pt2_n = r[4];                                                              \
+            pw_n = r[5];                                                               \
+        }                                                                              \
+    } while (0)
+
         if (KNN_ONLY) {
+            VF_FETCH_NEXT_RECORD();
             if (n > 0) {
                 kout.cnt[vidx] = n;
 #pragma unroll
@@ -585,6 +593,9 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 }
             }
         }
+        // next brick's node record for this thread's list slot, loaded once
+        // this brick's knn state is no longer live
+        VF_FETCH_NEXT_RECORD();
         const unsigned long long msup = __ballot(n > 0);
         const unsigned long long minb = __ballot(inb);
         const unsigned long long memit = __ballot(emit);
@@ -642,6 +653,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
             }
         }
         }  // !KNN_ONLY
+#undef VF_FETCH_NEXT_RECORD
         __syncthreads();
         a = a_n;
         item = item_n;
