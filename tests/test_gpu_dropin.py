@@ -178,6 +178,53 @@ def test_weights_warp_collisions_integrate(vf, oracle, k):
     assert upd > 500
 
 
+def test_collisions_and_integration_special_vertices(vf, oracle):
+    """Warped vertices at NaN, infinities, beyond the int32 range and just
+    outside the volume: the float -> int conversions saturate (NaN gives 0)
+    in the target voxel and in the pixel, bit for bit with the oracle."""
+    s = _setup(4, nodes=40)
+    c = np.array(s.center, F)
+    n = s.n_voxels
+    rng = np.random.default_rng(8)
+    verts = oracle.translate(oracle.generate_vertices(s.dim, s.voxel_size, s.size), c)
+    cnt = np.ones(n, np.int32)
+    cnt[rng.random(n) < 0.1] = 0
+    half = np.array(s.size, F) * F(0.5)
+    special = np.array([np.nan, np.inf, -np.inf, 3e9, -3e9, 2.0 ** 31, -(2.0 ** 31), 1e30, -1e30, 0.0, -0.0],
+                       np.float64).astype(F)
+    m = 4000
+    idx = rng.choice(n, m, replace=False)
+    for j, i in enumerate(idx):
+        a = j % 3
+        if j % 4 == 0:
+            verts[i, a] = special[(j // 4) % special.shape[0]]
+        elif j % 4 == 1:  # just below the volume's low face: truncation toward zero keeps voxel 0
+            verts[i, a] = c[a] - half[a] - F(s.voxel_size[a]) * F(0.5)
+        elif j % 4 == 2:
+            verts[i, a] = c[a] + half[a] + F(1e-4)
+        else:
+            verts[i, :] = special[(j // 4) % special.shape[0]]
+    col = oracle.count_collisions(verts, cnt, s.dim, s.voxel_size, s.size, I3, c)
+    g_verts, g_cnt = dev(verts), dev(cnt)
+    gcol = DeviceArray((n,), np.int32)
+    gcol.fill(0)
+    ops.run_count_collisions_per_voxel_kernel(gcol, g_verts, g_cnt, s.voxel_size, s.size, s.dim, I3, c)
+    np.testing.assert_array_equal(gcol.download(), col)
+    assert col[0] > 1  # the NaN vertices all land on voxel 0
+    depth = s.depth(0)
+    tsdf0 = np.full(n, -1, F)
+    wt0 = np.zeros(n, F)
+    t, wt, upd = oracle.integrate_warped(tsdf0, wt0, verts, col, cnt, depth, s.delta, s.dim, s.voxel_size, s.size,
+                                         I3, c, s.intrinsics(), s.max_weight)
+    gt, gwt, g_col, g_depth = dev(tsdf0), dev(wt0), dev(col), dev(depth)
+    H, W = depth.shape
+    ops.run_integrate_warped_tsdf_from_frame_kernel(gt, gwt, g_verts, g_col, g_cnt, g_depth, (W, H), s.delta,
+                                                    s.voxel_size, s.size, s.dim, I3, c, s.intrinsics(), s.max_weight)
+    np.testing.assert_array_equal(gt.download(), t)
+    np.testing.assert_array_equal(gwt.download(), wt)
+    assert upd > 100
+
+
 def test_reference_sequence_chained(vf, oracle):
     """src/VolumeFusion.cu:708-741 through the drop-in entry points, GPU buffers
     only, equals the oracle frame (tsdf, weights, node round trip)."""

@@ -14,12 +14,34 @@ namespace vf {
 
 constexpr float kInf = __builtin_huge_valf();
 
-// float -> int32, truncation toward zero, saturating, NaN -> 0.
+// float -> int32, truncation toward zero, saturating, NaN -> 0: exactly what
+// v_cvt_i32_f32 does (out-of-range inputs clamp to INT32_MIN / INT32_MAX, NaN
+// gives 0), so one instruction.  Written as asm because the C cast leaves
+// out-of-range inputs undefined.
 __device__ __forceinline__ int32_t cvt_i32_rz(float f) {
-    if (f != f) return 0;
-    if (f >= 2147483648.0f) return 2147483647;
-    if (f < -2147483648.0f) return (-2147483647 - 1);
-    return (int32_t)f;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
+
+// A load that the compiler may issue as a global (not flat) memory access:
+// the pointer is known to address device global memory.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// a / b for a divisor b shared by many quotients, from rb = RN64(1 / b):
+// RN32(RN64(a * rb)) is the IEEE binary32 quotient whenever that quotient is
+// a normal number.  Proof: the binary64 value is within 2^-52 (relative) of
+// a / b; a quotient of two binary32 numbers is never a binary32 midpoint and,
+// when not one, lies at least 2^-49 (relative) from every midpoint, so both
+// round to the same binary32 number.  Results below 2^-126 in magnitude
+// (zero, subnormal, or underflowed) take the IEEE division itself.
+__device__ __forceinline__ float div_by_recip64(float a, float b, double rb) {
+    const float q = (float)((double)a * rb);
+    if (__builtin_expect(!(__builtin_fabsf(q) >= 0x1p-126f), 0)) return a / b;
+    return q;
 }
 
 // Correctly rounded square root, the same bits as __builtin_sqrtf.  For
@@ -410,6 +432,7 @@ struct Camera {
     float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)
     int32_t W, H;
     float delta, maxw;
+    double rdelta;  // RN64(1 / delta)
 };
 
 // project3dToFrame + the tests of updateTsdf up to the read-modify-write,
@@ -427,11 +450,11 @@ __device__ __forceinline__ int64_t project_sample(const Camera& cam, const float
     const int32_t py = cvt_i32_rz(vf + 0.5f);
     if (px < 0 || px >= cam.W || py < 0 || py >= cam.H) return -1;
     const int64_t pi = (int64_t)py * cam.W + px;
-    const float d = depth[pi];
+    const float d = gload(depth + pi);
     if (__builtin_isinf(d) || d < 1.e-4f) return -1;
     const float sdf = p[2] - d;
     if (sdf > cam.delta) return -1;
-    *tn = (sdf < -cam.delta) ? -1.0f : sdf / cam.delta;
+    *tn = (sdf < -cam.delta) ? -1.0f : div_by_recip64(sdf, cam.delta, cam.rdelta);
     return pi;
 }
 

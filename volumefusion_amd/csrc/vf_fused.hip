@@ -145,16 +145,17 @@ __device__ __forceinline__ void stat_add(unsigned long long* stat, unsigned long
     atomicAdd(totals + slot, v);
 }
 
+constexpr int kPrepThreads = 1024;
+
 template <class S>
-__global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vols, int nvol, uint32_t total,
-                                                  NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
-                                                  uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active2,
-                                                  Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap,
-                                                  uint32_t* __restrict__ tile_flag,
-                                                  const uint32_t* __restrict__ touched2, uint64_t brick_cap,
-                                                  uint64_t tile_cap, int parity_arg,
-                                                  unsigned long long* __restrict__ stat,
-                                                  unsigned long long* __restrict__ totals) {
+__global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
+    const VolDesc* __restrict__ vols, int nvol, uint32_t total, NodeRec* __restrict__ nrec,
+    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active2,
+    Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag,
+    const uint32_t* __restrict__ touched2, uint64_t brick_cap, uint64_t tile_cap, int parity_arg,
+    unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals) {
+    __shared__ uint32_t s_wcnt[kPrepThreads / 64];
+    __shared__ uint32_t s_base;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
@@ -173,83 +174,110 @@ __global__ __launch_bounds__(256) void k_prep_bin(const VolDesc* __restrict__ vo
     }
     if (gid < kStCount * kStatSlots) stat[gid] = 0ull;
     if (gid == 0) ctrl->overflow_records = 0u;
-    if (gid >= total) return;
+    __syncthreads();
 
-    int v = 0;
-    while (v + 1 < nvol && vols[v + 1].k1_base <= gid) ++v;
-    const VolDesc& vd = vols[v];
-    const uint32_t local = gid - vd.k1_base;
-    if (local >= vd.k1_count) return;
-    const uint32_t nb = (uint32_t)(vd.maxb[0] * vd.maxb[1] * vd.maxb[2]);
-    const uint32_t j = local / nb;
-    const uint32_t o = local - j * nb;
+    // This thread's (node, brick) pair; `act` when it is the brick's first
+    // node, which makes the brick active.
+    bool act = false;
+    uint32_t entry = 0;
+    do {
+        if (gid >= total) break;
+        int v = 0;
+        while (v + 1 < nvol && vols[v + 1].k1_base <= gid) ++v;
+        const VolDesc& vd = vols[v];
+        const uint32_t local = gid - vd.k1_base;
+        if (local >= vd.k1_count) break;
+        const uint32_t nb = (uint32_t)(vd.maxb[0] * vd.maxb[1] * vd.maxb[2]);
+        const uint32_t j = local / nb;
+        const uint32_t o = local - j * nb;
 
-    // translateKernel(nodes, -c), src/VolumeFusion.cu:708
-    const float* np = vd.nodes + 3ull * j;
-    float m[3];
-    m[0] = np[0] + (-vd.c[0]);
-    m[1] = np[1] + (-vd.c[1]);
-    m[2] = np[2] + (-vd.c[2]);
-    // cube corner, src/TSDFWarp.cu:46-48
-    int32_t nv[3];
-    world_to_voxel(vd.g, m, nv);
-    int32_t lo[3];
+        // translateKernel(nodes, -c), src/VolumeFusion.cu:708
+        const float* np = vd.nodes + 3ull * j;
+        float m[3];
+        m[0] = np[0] + (-vd.c[0]);
+        m[1] = np[1] + (-vd.c[1]);
+        m[2] = np[2] + (-vd.c[2]);
+        // cube corner, src/TSDFWarp.cu:46-48
+        int32_t nv[3];
+        world_to_voxel(vd.g, m, nv);
+        int32_t lo[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        long long l = (long long)nv[a] - (long long)((vd.cube[a] - 1) / 2);
-        if (l < -(1ll << 30)) l = -(1ll << 30);
-        if (l > (1ll << 30)) l = (1ll << 30);
-        lo[a] = (int32_t)l;
-    }
-    if (o == 0) {
-        const float r = vd.radii[j];
-        const float rr = r * r;
-        const float den = 2.0f * rr;
-        NodeRec rec;
-        rec.m = make_float4(m[0], m[1], m[2], r * 2.0f);
-        rec.lo = make_int4(lo[0], lo[1], lo[2], 0);
-        if (vd.T && vd.blend == VF_BLEND_DQB) {
-            // t0, t1: the node's dual quaternion (real, dual)
-            mat4_to_dq(vd.T + 16ull * j, &rec.t0, &rec.t1);
-            rec.t2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else if (vd.T) {  // the knn-only entry point has no transforms
-            const float* M = vd.T + 16ull * j;
-            rec.t0 = make_float4(M[0], M[4], M[8], M[12]);
-            rec.t1 = make_float4(M[1], M[5], M[9], M[13]);
-            rec.t2 = make_float4(M[2], M[6], M[10], M[14]);
-        } else {
-            rec.t0 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-            rec.t1 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-            rec.t2 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        for (int a = 0; a < 3; ++a) {
+            long long l = (long long)nv[a] - (long long)((vd.cube[a] - 1) / 2);
+            if (l < -(1ll << 30)) l = -(1ll << 30);
+            if (l > (1ll << 30)) l = (1ll << 30);
+            lo[a] = (int32_t)l;
         }
-        rec.w = make_float4(den, 1.0f / den, 0.0f, 0.0f);
-        nrec[vd.node_base + j] = rec;
-    }
-    // bricks overlapped by the cube clipped to the volume
-    int32_t blo[3], bhi[3];
-    const int32_t bs[3] = {S::BX, S::BY, S::BZ};
+        if (o == 0) {
+            const float r = vd.radii[j];
+            const float rr = r * r;
+            const float den = 2.0f * rr;
+            NodeRec rec;
+            rec.m = make_float4(m[0], m[1], m[2], r * 2.0f);
+            rec.lo = make_int4(lo[0], lo[1], lo[2], 0);
+            if (vd.T && vd.blend == VF_BLEND_DQB) {
+                // t0, t1: the node's dual quaternion (real, dual)
+                mat4_to_dq(vd.T + 16ull * j, &rec.t0, &rec.t1);
+                rec.t2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            } else if (vd.T) {  // the knn-only entry point has no transforms
+                const float* M = vd.T + 16ull * j;
+                rec.t0 = make_float4(M[0], M[4], M[8], M[12]);
+                rec.t1 = make_float4(M[1], M[5], M[9], M[13]);
+                rec.t2 = make_float4(M[2], M[6], M[10], M[14]);
+            } else {
+                rec.t0 = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+                rec.t1 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+                rec.t2 = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+            }
+            rec.w = make_float4(den, 1.0f / den, 0.0f, 0.0f);
+            nrec[vd.node_base + j] = rec;
+        }
+        // bricks overlapped by the cube clipped to the volume
+        int32_t blo[3], bhi[3];
+        const int32_t bs[3] = {S::BX, S::BY, S::BZ};
+        bool empty = false;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const long long l = lo[a] < 0 ? 0 : lo[a];
-        long long h = (long long)lo[a] + vd.cube[a] - 1;
-        if (h > vd.g.dim[a] - 1) h = vd.g.dim[a] - 1;
-        if (l > h) return;
-        blo[a] = (int32_t)(l / bs[a]);
-        bhi[a] = (int32_t)(h / bs[a]);
+        for (int a = 0; a < 3; ++a) {
+            const long long l = lo[a] < 0 ? 0 : lo[a];
+            long long h = (long long)lo[a] + vd.cube[a] - 1;
+            if (h > vd.g.dim[a] - 1) h = vd.g.dim[a] - 1;
+            empty = empty || l > h;
+            blo[a] = (int32_t)(l / bs[a]);
+            bhi[a] = (int32_t)(h / bs[a]);
+        }
+        if (empty) break;
+        const int32_t ox = (int32_t)(o % (uint32_t)vd.maxb[0]);
+        const int32_t oy = (int32_t)((o / (uint32_t)vd.maxb[0]) % (uint32_t)vd.maxb[1]);
+        const int32_t oz = (int32_t)(o / (uint32_t)(vd.maxb[0] * vd.maxb[1]));
+        const int32_t bx = blo[0] + ox, by = blo[1] + oy, bz = blo[2] + oz;
+        if (bx > bhi[0] || by > bhi[1] || bz > bhi[2]) break;
+        const uint32_t b = ((uint32_t)bz * (uint32_t)vd.bdim[1] + (uint32_t)by) * (uint32_t)vd.bdim[0] + (uint32_t)bx;
+        const uint32_t gb = vd.brick_base + b;
+        const uint32_t slot = atomicAdd(brick_count + gb, 1u);
+        if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
+        act = slot == 0;
+        entry = ((uint32_t)v << 24) | b;
+    } while (false);
+
+    // Active bricks: one append per workgroup to the frame's list, in thread
+    // order (a single global counter taken once per brick measured as a
+    // serialising hot spot).
+    const unsigned long long mact = __ballot(act);
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(mact);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < kPrepThreads / 64; ++w) {
+            const uint32_t c = s_wcnt[w];
+            s_wcnt[w] = sum;
+            sum += c;
+        }
+        s_base = sum ? atomicAdd(&ctrl->active_count[parity], sum) : 0u;
     }
-    const int32_t ox = (int32_t)(o % (uint32_t)vd.maxb[0]);
-    const int32_t oy = (int32_t)((o / (uint32_t)vd.maxb[0]) % (uint32_t)vd.maxb[1]);
-    const int32_t oz = (int32_t)(o / (uint32_t)(vd.maxb[0] * vd.maxb[1]));
-    const int32_t bx = blo[0] + ox, by = blo[1] + oy, bz = blo[2] + oz;
-    if (bx > bhi[0] || by > bhi[1] || bz > bhi[2]) return;
-    const uint32_t b = ((uint32_t)bz * (uint32_t)vd.bdim[1] + (uint32_t)by) * (uint32_t)vd.bdim[0] + (uint32_t)bx;
-    const uint32_t gb = vd.brick_base + b;
-    const uint32_t slot = atomicAdd(brick_count + gb, 1u);
-    if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
-    if (slot == 0) {
-        const uint32_t a = atomicAdd(&ctrl->active_count[parity], 1u);
-        active[a] = ((uint32_t)v << 24) | b;
-    }
+    __syncthreads();
+    if (act) active[s_base + s_wcnt[wave] + (uint32_t)__popcll(mact & ((1ull << lane) - 1ull))] = entry;
 }
 
 template <int K, bool KNN_ONLY, class S>
@@ -588,7 +616,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 // projective tests and the new sample (src/TSDFIntegration.cu:32-64)
                 float tn;
                 if (project_sample(vd.cam, vd.depth, acc, &tn) >= 0) {
-                    blend_sample(vd.tsdf[vidx], vd.wt[vidx], tn, vd.cam.maxw, &tnew, &wnew);
+                    blend_sample(gload(vd.tsdf + vidx), gload(vd.wt + vidx), tn, vd.cam.maxw, &tnew, &wnew);
                     emit = true;
                 }
             }
@@ -1096,7 +1124,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->shape = shape;
     fs->knn_only = knn_only;
     fs->k1 = (uint32_t)k1;
-    fs->blocks_prep = std::max<uint64_t>((k1 + 255) / 256, 1024);
+    fs->blocks_prep = std::max<uint64_t>((k1 + kPrepThreads - 1) / kPrepThreads, 256);
     fs->blocks_a = blocks_a;
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
     fs->arena_cap = arena_cap;
@@ -1107,9 +1135,9 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
 template <class S>
 int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOut kout, hipEvent_t* ev,
                          hipStream_t stream) {
-    hipLaunchKernelGGL(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(256), 0, stream, B.vols, fs.nvol, fs.k1,
-                       B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag, B.touched,
-                       B.brick_cap, B.tile_cap, parity_arg, B.stat, B.totals);
+    hipLaunchKernelGGL(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), 0, stream, B.vols,
+                       fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
+                       B.touched, B.brick_cap, B.tile_cap, parity_arg, B.stat, B.totals);
     VF_LAUNCH_CHECK("k_prep_bin");
     if (ev) VF_HIP(hipEventRecord(ev[1], stream));
     Arena arena;

@@ -73,6 +73,7 @@ inline Camera make_camera(const vf_mat3& K, int W, int H, float delta, float max
     c.H = H;
     c.delta = delta;
     c.maxw = maxw;
+    c.rdelta = 1.0 / (double)delta;
     return c;
 }
 
