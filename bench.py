@@ -63,6 +63,8 @@ def parse_args():
     ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson arrival run")
     ap.add_argument("--no-tune", action="store_true", help="default brick shape instead of the autotuned one")
+    ap.add_argument("--tuning", default=None, metavar="SHAPE,WG",
+                    help="fixed brick shape index and workgroups per CU instead of autotuning (e.g. 2,0)")
     ap.add_argument("--no-extras", action="store_true", help="skip marching cubes, ingest and point warp")
     return ap.parse_args()
 
@@ -506,7 +508,11 @@ def main() -> int:
     # brick shape and workgroups per CU from the autotuner (scratch copies of
     # the volume; results do not depend on the choice)
     tuning = {"brick": [16, 8, 4], "wg_per_cu": 0, "tuned_ms_per_frame": None}
-    if not args.no_tune:
+    if args.tuning:
+        shape, wg = (int(x) for x in args.tuning.split(","))
+        ws.set_tuning(shape, wg)
+        tuning = {"brick": list(L.BRICK_SHAPES[shape]), "wg_per_cu": wg, "tuned_ms_per_frame": None}
+    elif not args.no_tune:
         plan = ops.Plan([tsdf], [graph], [d_dev[0]], [s.intrinsics()], s.k, s.delta, s.max_weight, [s.cube],
                         transforms=[T_dev[0]], graph=False, stream=stream)
         t = plan.autotune(frames=3, stream=stream)

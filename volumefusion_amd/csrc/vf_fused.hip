@@ -108,9 +108,14 @@ struct NodeRec {
     float4 w;   // .x = 2*(r*r) (weight denominator), .y = RN(1 / .x)
 };
 
+constexpr uint32_t kQueues = 8;       // pass-A brick queues, one per XCD
+constexpr uint32_t kQueueChunk = 16;  // list entries per queue chunk
+constexpr uint32_t kQueueStride = 64; // counters 256 bytes apart
+
 struct Ctrl {
     uint32_t active_count[2];  // by frame parity
     uint32_t work_next[2];
+    uint32_t queue[2][kQueues * kQueueStride];  // pass-A queue counters by frame parity
     uint32_t touched_count[2];
     uint32_t overflow_records;
     uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
@@ -154,8 +159,6 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag,
     const uint32_t* __restrict__ touched2, uint64_t brick_cap, uint64_t tile_cap, int parity_arg,
     unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals) {
-    __shared__ uint32_t s_wcnt[kPrepThreads / 64];
-    __shared__ uint32_t s_base;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
@@ -259,9 +262,35 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
         entry = ((uint32_t)v << 24) | b;
     } while (false);
 
-    // Active bricks: one append per workgroup to the frame's list, in thread
-    // order (a single global counter taken once per brick measured as a
-    // serialising hot spot).
+    (void)act;
+    (void)entry;
+}
+
+// The frame's active bricks (those with a non-empty node list), in brick
+// order, one thread per brick of the batch; one list append per workgroup.
+__global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restrict__ vols, int nvol,
+                                                          uint32_t nbricks, const uint32_t* __restrict__ brick_count,
+                                                          uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
+                                                          uint64_t brick_cap, int parity_arg) {
+    __shared__ uint32_t s_wcnt[kPrepThreads / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t parity = frame_parity(ctrl, parity_arg);
+    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
+    bool act = false;
+    uint32_t entry = 0;
+    if (gb < nbricks && brick_count[gb] != 0u) {
+        int lo = 0, hi = nvol - 1;  // last volume whose first brick is <= gb
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (vols[mid].brick_base <= gb)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        act = true;
+        entry = ((uint32_t)lo << 24) | (gb - vols[lo].brick_base);
+    }
     const unsigned long long mact = __ballot(act);
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
@@ -304,6 +333,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     __shared__ uint32_t s_bm[KNN_ONLY ? 1 : kWinTiles * 32];
     __shared__ uint32_t s_wave_emit[kPassAThreads / 64];
     __shared__ unsigned long long s_stat[4];
+    __shared__ uint32_t s_next, s_first;
 
     const int tid = threadIdx.x;
     const int lane = __lane_id();
@@ -321,6 +351,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         // whose kernels have finished
         ctrl->active_count[parity ^ 1u] = 0u;
         ctrl->work_next[parity ^ 1u] = 0u;
+        for (uint32_t q = 0; q < kQueues; ++q) ctrl->queue[parity ^ 1u][q * kQueueStride] = 0u;
         if (!KNN_ONLY) ctrl->touched_count[parity ^ 1u] = 0u;
     }
     const uint32_t nactive = ctrl->active_count[parity];
@@ -330,12 +361,27 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     const int lz = tid / (kBX * kBY);
     uint32_t fill = 0;  // records in this workgroup's arena (uniform)
 
-    // Static schedule: this workgroup takes active bricks blockIdx.x,
-    // blockIdx.x + gridDim.x, ...  The brick after the current one is fetched
-    // while the current one computes: its entry, its list count and, for this
-    // thread's list slot, the node id and the node record.
-    const uint32_t stride = gridDim.x;
-    uint32_t a = blockIdx.x;
+    // Dynamic schedule over kQueues queues: queue x owns the list chunks
+    // x, x + kQueues, ... of kQueueChunk entries and hands out its entries in
+    // order from a counter of its own.  Workgroup w draws from queue w % 8,
+    // the queue of its XCD (workgroups are dispatched to the 8 XCDs
+    // round-robin), so an XCD works through runs of neighbouring list entries
+    // and no workgroup waits on a fixed share; one counter for all workgroups
+    // measured as a serialising same-address hot spot.  Each grab is issued
+    // one brick ahead and read a brick later.  The brick after the current
+    // one is fetched while the current one computes: its entry, its list
+    // count and, for this thread's list slot, the node id and the node record.
+    const uint32_t qx = blockIdx.x % kQueues;
+    uint32_t* const qctr = &ctrl->queue[parity][qx * kQueueStride];
+    auto queue_pos = [qx](uint32_t j) -> uint32_t {
+        return ((j / kQueueChunk) * kQueues + qx) * kQueueChunk + j % kQueueChunk;
+    };
+    if (tid == 0) {
+        s_first = queue_pos(atomicAdd(qctr, 1u));
+        s_next = queue_pos(atomicAdd(qctr, 1u));
+    }
+    __syncthreads();
+    uint32_t a = s_first;
     uint32_t item = a < nactive ? active[a] : kNone;
     uint32_t cnt = 0, pre_id = 0;
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -356,6 +402,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         }
     }
     __syncthreads();
+    uint32_t a_n = s_next;
 
     while (item != kNone) {
         item = __builtin_amdgcn_readfirstlane(item);
@@ -374,9 +421,11 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         const int32_t x = bxi * kBX + lx, y = byi * kBY + ly, z = bzi * kBZ + lz;
         const bool valid = x < dimx && y < dimy && z < dimz;
 
-        // next brick: entry, count and list ids (all kCap slots, in bounds)
-        const uint32_t a_n = a + stride;
+        // next brick: entry, count and list ids (all kCap slots, in bounds);
+        // the grab of the one after it
         const uint32_t item_n = a_n < nactive ? active[a_n] : kNone;
+        uint32_t a_nn = 0;
+        if (tid == 0 && item_n != kNone) a_nn = queue_pos(atomicAdd(qctr, 1u));
         uint32_t cnt_n = 0, id_n = 0, nbase_n = 0;
         if (item_n != kNone) {
             const VolDesc& vn = vols[item_n >> 24];
@@ -682,8 +731,11 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         }
         }  // !KNN_ONLY
 #undef VF_FETCH_NEXT_RECORD
+        // every thread read s_next before this brick's first barrier
+        if (tid == 0) s_next = a_nn;
         __syncthreads();
         a = a_n;
+        a_n = s_next;
         item = item_n;
         cnt = cnt_n;
         pre_id = id_n;
@@ -715,13 +767,17 @@ struct VolPtr {
     uint32_t pad;
 };
 
-// One pending record: write it unless its target was reached twice.
-__device__ __forceinline__ bool apply_record(const uint4 rec, const VolPtr* s_vp, int nvol,
-                                             const uint32_t* __restrict__ bitmap) {
+// The collision word of a pending record's target: bit (target & 31) is set
+// when the target was reached twice.
+__device__ __forceinline__ uint32_t record_word(const uint4 rec, const uint32_t* __restrict__ bitmap) {
     const uint32_t tile = rec.y >> 9;
     const uint32_t bit = rec.y & 511u;
-    const uint32_t word = bitmap[(uint64_t)tile * 32u + 16u + (bit >> 5)];
-    if ((word >> (bit & 31u)) & 1u) return false;
+    return bitmap[(uint64_t)tile * 32u + 16u + (bit >> 5)];
+}
+
+// One pending record: write it unless its target was reached twice.
+__device__ __forceinline__ bool apply_record(const uint4 rec, uint32_t word, const VolPtr* s_vp, int nvol) {
+    if ((word >> (rec.y & 31u)) & 1u) return false;
     // volume of the voxel: last volume whose first voxel is <= rec.x
     int lo = 0, hi = nvol - 1;
     while (lo < hi) {
@@ -750,22 +806,40 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
     if (tid < 2) s_cnt[tid] = 0ull;
     __syncthreads();
     unsigned long long collided = 0, updated = 0;
-    // this workgroup's share of the per-workgroup arenas
+    // this workgroup's share of the per-workgroup arenas, kUnrollB records per
+    // thread in flight (their loads issued before any is applied)
+    constexpr uint32_t kUnrollB = 4;
     for (uint32_t a = blockIdx.x; a < arena.n_arenas; a += gridDim.x) {
         const uint32_t n = arena.counts[a];
         const uint4* recs = arena.records + (uint64_t)a * arena.cap;
-        for (uint32_t r = tid; r < n; r += blockDim.x) {
-            if (apply_record(recs[r], s_vp, nvol, bitmap))
-                ++updated;
-            else
-                ++collided;
+        for (uint32_t r0 = 0; r0 < n; r0 += kUnrollB * blockDim.x) {
+            uint4 rec[kUnrollB];
+            uint32_t word[kUnrollB];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnrollB; ++u) {
+                const uint32_t r = r0 + u * blockDim.x + tid;
+                rec[u] = r < n ? recs[r] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnrollB; ++u)
+                word[u] = r0 + u * blockDim.x + tid < n ? record_word(rec[u], bitmap) : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kUnrollB; ++u) {
+                if (r0 + u * blockDim.x + tid < n) {
+                    if (apply_record(rec[u], word[u], s_vp, nvol))
+                        ++updated;
+                    else
+                        ++collided;
+                }
+            }
         }
     }
     // the shared overflow arena
     uint32_t no = ctrl->overflow_records;
     if (no > arena.overflow_cap) no = arena.overflow_cap;
     for (uint32_t r = blockIdx.x * blockDim.x + tid; r < no; r += gridDim.x * blockDim.x) {
-        if (apply_record(arena.overflow[r], s_vp, nvol, bitmap))
+        const uint4 rec = arena.overflow[r];
+        if (apply_record(rec, record_word(rec, bitmap), s_vp, nvol))
             ++updated;
         else
             ++collided;
@@ -996,7 +1070,7 @@ struct FrameSetup {
     int shape = 0;
     bool knn_only = false;
     uint32_t k1 = 0;
-    uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0;
+    uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
 };
 
 template <class S>
@@ -1126,6 +1200,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->k1 = (uint32_t)k1;
     fs->blocks_prep = std::max<uint64_t>((k1 + kPrepThreads - 1) / kPrepThreads, 256);
     fs->blocks_a = blocks_a;
+    fs->bricks = need.bricks;
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
     fs->arena_cap = arena_cap;
     fs->overflow_cap = overflow_cap;
@@ -1139,6 +1214,10 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
                        fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
                        B.touched, B.brick_cap, B.tile_cap, parity_arg, B.stat, B.totals);
     VF_LAUNCH_CHECK("k_prep_bin");
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads - 1) / kPrepThreads)), dim3(kPrepThreads),
+                       0, stream, B.vols, fs.nvol, (uint32_t)fs.bricks, B.brick_count, B.active, B.ctrl, B.brick_cap,
+                       parity_arg);
+    VF_LAUNCH_CHECK("k_compact");
     if (ev) VF_HIP(hipEventRecord(ev[1], stream));
     Arena arena;
     arena.records = B.records;
