@@ -114,9 +114,7 @@ constexpr uint32_t kQueueStride = 64; // counters 256 bytes apart
 
 struct Ctrl {
     uint32_t active_count[2];  // by frame parity
-    uint32_t work_next[2];
     uint32_t queue[2][kQueues * kQueueStride];  // pass-A queue counters by frame parity
-    uint32_t touched_count[2];
     uint32_t overflow_records;
     uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
 };
@@ -157,23 +155,22 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     const VolDesc* __restrict__ vols, int nvol, uint32_t total, NodeRec* __restrict__ nrec,
     uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active2,
     Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag,
-    const uint32_t* __restrict__ touched2, uint64_t brick_cap, uint64_t tile_cap, int parity_arg,
+    uint32_t clear_tiles, uint64_t brick_cap, int parity_arg,
     unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
-    const uint32_t* __restrict__ touched_prev = touched2 + (uint64_t)(parity ^ 1u) * tile_cap;
 
-    // Clear the collision tiles the previous frame on this workspace touched.
-    const uint32_t nt = ctrl->touched_count[parity ^ 1u];
-    for (uint32_t w = gid; w < nt * 33u; w += nthreads) {
-        const uint32_t tile = touched_prev[w / 33u];
-        const uint32_t k = w % 33u;
-        if (k < 32u)
-            bitmap[(uint64_t)tile * 32u + k] = 0u;
-        else
-            tile_flag[tile] = 0u;
+    // Clear the collision tiles the previous frame on this workspace flagged
+    // (tiles [0, clear_tiles) hold every flag it can have set).
+    for (uint32_t t = gid; t < clear_tiles; t += nthreads) {
+        if (tile_flag[t]) {
+            uint4* w = (uint4*)(bitmap + (uint64_t)t * 32u);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w[k] = make_uint4(0u, 0u, 0u, 0u);
+            tile_flag[t] = 0u;
+        }
     }
     if (gid < kStCount * kStatSlots) stat[gid] = 0ull;
     if (gid == 0) ctrl->overflow_records = 0u;
@@ -313,8 +310,8 @@ template <int K, bool KNN_ONLY, class S>
 __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
     const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
-    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t* __restrict__ touched2,
-    uint64_t brick_cap, uint64_t tile_cap, int parity_arg, Arena arena, unsigned long long* __restrict__ stat,
+    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint64_t brick_cap, int parity_arg,
+    Arena arena, unsigned long long* __restrict__ stat,
     unsigned long long* __restrict__ totals, KnnOut kout) {
     constexpr int kPassAThreads = S::kThreads;
     constexpr int kBX = S::BX, kBY = S::BY, kBZ = S::BZ;
@@ -340,7 +337,6 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     const int wave = tid >> 6;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     const uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
-    uint32_t* __restrict__ touched_cur = touched2 + (uint64_t)parity * tile_cap;
     if (tid < 4) s_stat[tid] = 0ull;
     if (tid < 32) s_exp2[tid] = kExp2Table[tid];
     if (!KNN_ONLY) {
@@ -350,9 +346,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         // the other parity's list and counters belong to the previous frame,
         // whose kernels have finished
         ctrl->active_count[parity ^ 1u] = 0u;
-        ctrl->work_next[parity ^ 1u] = 0u;
         for (uint32_t q = 0; q < kQueues; ++q) ctrl->queue[parity ^ 1u][q * kQueueStride] = 0u;
-        if (!KNN_ONLY) ctrl->touched_count[parity ^ 1u] = 0u;
     }
     const uint32_t nactive = ctrl->active_count[parity];
 
@@ -658,9 +652,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                     uint32_t* blk = bitmap + (uint64_t)tile * 32u;
                     const uint32_t old = atomicOr(blk + (bit >> 5), mask);
                     if (old & mask) atomicOr(blk + 16 + (bit >> 5), mask);
-                    if (__atomic_load_n(tile_flag + tile, __ATOMIC_RELAXED) == 0u &&
-                        atomicCAS(tile_flag + tile, 0u, 1u) == 0u)
-                        touched_cur[atomicAdd(&ctrl->touched_count[parity], 1u)] = tile;
+                    tile_flag[tile] = 1u;
                 }
                 // projective tests and the new sample (src/TSDFIntegration.cu:32-64)
                 float tn;
@@ -722,9 +714,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 const uint32_t old = atomicOr(g + wi, l1);
                 const uint32_t two = l2 | (old & l1);
                 if (two) atomicOr(g + 16 + wi, two);
-                if (__atomic_load_n(tile_flag + tile, __ATOMIC_RELAXED) == 0u &&
-                    atomicCAS(tile_flag + tile, 0u, 1u) == 0u)
-                    touched_cur[atomicAdd(&ctrl->touched_count[parity], 1u)] = tile;
+                tile_flag[tile] = 1u;
                 blk[wi] = 0u;
                 blk[16 + wi] = 0u;
             }
@@ -880,7 +870,6 @@ struct Buffers {
     uint32_t* active = nullptr;  // 2 x brick_cap (frame parity)
     uint32_t* bitmap = nullptr;
     uint32_t* tile_flag = nullptr;
-    uint32_t* touched = nullptr;  // 2 x tile_cap
     uint4* records = nullptr;     // arena_slots x arena_cap, then the overflow arena
     uint32_t* arena_counts = nullptr;
     unsigned long long* stat = nullptr;  // kStCount x kStatSlots, this frame
@@ -896,7 +885,8 @@ struct Buffers {
 struct vf_workspace {
     int device = 0;
     vf::Buffers buf;
-    uint64_t frame = 0;  // frame parity selects the active and touched-tile lists
+    uint64_t frame = 0;  // frame parity selects the active-brick list
+    uint64_t dirty_tiles = 0;  // collision tiles [0, dirty_tiles) may hold flags from the last frame
     std::vector<vf::VolDesc> vols_h;
     std::vector<vf::VolPtr> vptr_h;
     // optional per-kernel timing: 4 events per frame (before k_prep_bin, after
@@ -915,8 +905,8 @@ namespace {
 constexpr int kMaxTimed = 4096;
 
 void free_buffers(Buffers& b) {
-    void* ptrs[] = {b.ctrl, b.nrec, b.brick_count, b.brick_nodes, b.active, b.bitmap, b.tile_flag,
-                    b.touched, b.records, b.arena_counts, b.stat, b.totals, b.vols, b.vptr};
+    void* ptrs[] = {b.ctrl,    b.nrec,         b.brick_count, b.brick_nodes, b.active, b.bitmap, b.tile_flag,
+                    b.records, b.arena_counts, b.stat,        b.totals,      b.vols,   b.vptr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = Buffers();
@@ -976,7 +966,6 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     if (!st) st = dev_alloc(&nbuf.active, nbuf.brick_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.bitmap, nbuf.tile_cap * 32, &bytes);
     if (!st) st = dev_alloc(&nbuf.tile_flag, nbuf.tile_cap, &bytes);
-    if (!st) st = dev_alloc(&nbuf.touched, nbuf.tile_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.records, nbuf.record_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.arena_counts, nbuf.arena_slots, &bytes);
     if (!st) st = dev_alloc(&nbuf.stat, (uint64_t)kStCount * kStatSlots, &bytes);
@@ -1002,6 +991,7 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     VF_HIP(hipStreamSynchronize(stream));
     b = nbuf;
     ws->frame = 0;
+    ws->dirty_tiles = 0;
     return VF_OK;
 }
 
@@ -1071,6 +1061,8 @@ struct FrameSetup {
     bool knn_only = false;
     uint32_t k1 = 0;
     uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
+    uint64_t tiles = 0;        // collision tiles of this batch
+    uint64_t clear_tiles = 0;  // tiles whose flags prep clears: every tile a previous frame or a replay may flag
 };
 
 template <class S>
@@ -1201,6 +1193,8 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->blocks_prep = std::max<uint64_t>((k1 + kPrepThreads - 1) / kPrepThreads, 256);
     fs->blocks_a = blocks_a;
     fs->bricks = need.bricks;
+    fs->tiles = need.tiles;
+    fs->clear_tiles = std::max<uint64_t>(ws->dirty_tiles, need.tiles);
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
     fs->arena_cap = arena_cap;
     fs->overflow_cap = overflow_cap;
@@ -1212,7 +1206,7 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
                          hipStream_t stream) {
     hipLaunchKernelGGL(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), 0, stream, B.vols,
                        fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
-                       B.touched, B.brick_cap, B.tile_cap, parity_arg, B.stat, B.totals);
+                       (uint32_t)fs.clear_tiles, B.brick_cap, parity_arg, B.stat, B.totals);
     VF_LAUNCH_CHECK("k_prep_bin");
     hipLaunchKernelGGL(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads - 1) / kPrepThreads)), dim3(kPrepThreads),
                        0, stream, B.vols, fs.nvol, (uint32_t)fs.bricks, B.brick_count, B.active, B.ctrl, B.brick_cap,
@@ -1230,11 +1224,11 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     if (fs.knn_only)                                                                                                 \
         hipLaunchKernelGGL((k_pass_a<KK, true, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,       \
                            B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
-                           B.touched, B.brick_cap, B.tile_cap, parity_arg, arena, B.stat, B.totals, kout);           \
+                           B.brick_cap, parity_arg, arena, B.stat, B.totals, kout);                                  \
     else                                                                                                             \
         hipLaunchKernelGGL((k_pass_a<KK, false, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,      \
                            B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
-                           B.touched, B.brick_cap, B.tile_cap, parity_arg, arena, B.stat, B.totals, kout)
+                           B.brick_cap, parity_arg, arena, B.stat, B.totals, kout)
     switch (fs.k) {
         case 1: VF_PASS_A(1); break;
         case 2: VF_PASS_A(2); break;
@@ -1291,6 +1285,7 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
     st = launch_kernels(ws, fs, (int)(ws->frame & 1u), kout, ev, stream);
     if (st) return st;
     ws->frame++;
+    ws->dirty_tiles = knn_only ? 0 : fs.tiles;
     return VF_OK;
 }
 
