@@ -47,6 +47,9 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "warped voxel updates/s + % HBM roofline; p50/p99 frame latency @32 streams"
 HBM_PEAK = 8.0e12
+SIMDS = 1024          # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9      # MI355X peak engine clock
+VALU_CYCLES = 4       # a wave64 VALU instruction on a 16-lane SIMD
 
 
 def parse_args():
@@ -546,13 +549,25 @@ def main() -> int:
     t_a = kms[1] / nk * 1e-3
     t_frame = sum(kms) / nk * 1e-3
     achieved = ab["k_pass_a"] / t_a
+    # HBM bytes and VALU instructions per k_pass_a launch, from the committed
+    # rocprofv3 counter runs of this workload (valid for the brick shape they
+    # were taken with)
     traffic = None
+    valu = None
     tfile = REPO / "profiles" / f"traffic_{args.workload}.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("k_pass_a", {}).get("hbm_bytes_per_launch")
-        except (ValueError, AttributeError):
-            traffic = None
+            prof = json.loads(tfile.read_text())
+            ka = prof.get("k_pass_a", {})
+            if prof.get("brick") == tuning["brick"]:
+                traffic = ka.get("hbm_bytes_per_launch")
+                if ka.get("valu_insts_per_launch"):
+                    issue = ka["valu_insts_per_launch"] * VALU_CYCLES / (SIMDS * CLOCK_HZ)
+                    valu = {"bound": "valu", "kernel": "k_pass_a", "insts_per_launch": ka["valu_insts_per_launch"],
+                            "issue_ms": issue * 1e3, "avg_launch_ms": t_a * 1e3, "frac": issue / t_a,
+                            "model": "wave64 VALU instruction = 4 cycles on a 16-lane SIMD, 1024 SIMDs at 2.4 GHz"}
+        except (ValueError, AttributeError, TypeError):
+            traffic = valu = None
 
     if args.copy_calibration:
         # a plain device-to-device copy of known size, for scaling HBM counters
@@ -591,6 +606,7 @@ def main() -> int:
             "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
             "algorithmic_bytes_per_launch": ab["k_pass_a"], "avg_launch_ms": t_a * 1e3,
         },
+        "valu_roofline": valu,
         "frame_roofline": {
             "achieved_GBps": ab["frame"] / t_frame / 1e9, "frac": ab["frame"] / t_frame / HBM_PEAK,
             "algorithmic_bytes": ab["frame"], "kernel_ms": {"k_prep_bin": kms[0] / nk, "k_pass_a": kms[1] / nk,
