@@ -267,7 +267,7 @@ int vf_eval_warp_weight(const float* radius, const float* dist, float* out, size
 /* One call performs the whole TSDF-integration block of the reference's frame
  * loop, src/VolumeFusion.cu:702-742 (translate nodes by -c, knn, weights,
  * vertices, translate back, warp, count collisions, integrate, with volume
- * rotation I and translation c), in three kernels and with no per-voxel
+ * rotation I and translation c), in four kernels and with no per-voxel
  * scratch arrays.  The tsdf/weight arrays are updated in place; when
  * graph.points_out is not NULL it receives the node positions as the
  * reference's two translations leave them, fl(fl(p - c) + c). */
@@ -346,9 +346,10 @@ int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream s
 int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream);
 
 /* Per-kernel timing: when enabled, each call records HIP events on its stream
- * around its three kernels.  vf_workspace_kernel_ms waits for the last one,
- * returns the summed milliseconds of k_prep_bin, k_pass_a and k_pass_b over
- * the timed calls since enabling (or the previous query) and starts over. */
+ * around its kernels.  vf_workspace_kernel_ms waits for the last one,
+ * returns the summed milliseconds of k_prep_bin with k_compact, k_pass_a and
+ * k_pass_b over the timed calls since enabling (or the previous query) and
+ * starts over. */
 int vf_workspace_set_timing(vf_workspace* ws, int enable);
 int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames);
 
@@ -380,7 +381,7 @@ int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning);
 /* A plan fixes a batch of volumes and the device buffers of their graphs and
  * frames; their contents (depth, transforms, node positions, tsdf) may change
  * between launches, the pointers may not.  The plan owns its workspace, and
- * with VF_PLAN_GRAPH it records its three kernels into a HIP graph on the
+ * with VF_PLAN_GRAPH it records its four kernels into a HIP graph on the
  * first launch and replays that graph afterwards (the kernels read the frame
  * parity from device memory, so the recorded launches stay valid).  Creating a
  * plan uploads its descriptors on `stream` and waits for them.  A plan's

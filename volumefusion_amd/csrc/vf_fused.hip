@@ -1,5 +1,5 @@
 // vf_fused.hip -- the warped TSDF integration of one frame (or a batch of
-// independent volumes, one frame each) in three kernels.
+// independent volumes, one frame each) in four kernels.
 //
 // The reference runs nine dense launches with ~100 B of per-voxel scratch
 // (src/VolumeFusion.cu:699-742, buffers at :252-260).  Here:
@@ -7,17 +7,20 @@
 //   k_prep_bin  one thread per (node, brick overlapped by the node's cube):
 //               node record (translated position, cube corner, 2r, 2r^2 and
 //               its reciprocal, transform rows); append the node to the
-//               brick's list; the first append activates the brick.  Also
-//               clears the collision tiles the previous frame touched.
-//   k_pass_a    one 512-thread workgroup per active brick (16 x 8 x 4 voxels,
-//               x fastest; one voxel per thread): stage the brick's node list
-//               in LDS sorted by node index, per voxel select the canonical
-//               top-K, weights, blend, target voxel.  Collisions are counted
-//               in an LDS window of 8^3 tiles around the brick (2 bits per
-//               voxel: seen once / seen twice) and merged into the global
-//               bitmap once per brick, a word at a time.  Voxels that pass the
-//               projective tests append a 16 B pending record (voxel, target,
-//               new tsdf, new weight) to the workgroup's own record arena.
+//               brick's list.  Also clears the collision tiles the previous
+//               frame flagged.
+//   k_compact   one thread per brick: the active list (bricks with a
+//               non-empty node list) in brick order.
+//   k_pass_a    one workgroup per active brick at a time, bricks drawn from
+//               per-XCD queues (x fastest; one voxel per thread): stage the
+//               brick's node list in LDS sorted by node index, per voxel
+//               select the canonical top-K, weights, blend, target voxel.
+//               Collisions are counted in an LDS window of 8^3 tiles around
+//               the brick (2 bits per voxel: seen once / seen twice) and
+//               merged into the global bitmap once per brick, a word at a
+//               time.  Voxels that pass the projective tests append a 16 B
+//               pending record (voxel, target, new tsdf, new weight) to the
+//               workgroup's own record arena.
 //   k_pass_b    one thread per pending record: drop it if its target was
 //               reached twice, else write tsdf and weight.
 //
@@ -890,7 +893,7 @@ struct vf_workspace {
     std::vector<vf::VolDesc> vols_h;
     std::vector<vf::VolPtr> vptr_h;
     // optional per-kernel timing: 4 events per frame (before k_prep_bin, after
-    // each of the three kernels)
+    // the prep-and-list kernels, pass A and pass B)
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;

@@ -476,7 +476,7 @@ def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list
 
 class Plan:
     """A fixed batch of volumes replayed every frame (vf_plan): descriptors are
-    uploaded once and, with graph=True, the three kernels run as one recorded
+    uploaded once and, with graph=True, the four kernels run as one recorded
     HIP graph.  The device buffers of every argument must stay alive and in
     place; their contents may change between launches."""
 
