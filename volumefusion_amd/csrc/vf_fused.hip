@@ -575,10 +575,14 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 for (int s = 0; s < K; ++s)
                     if (s < n) w[s] = div_rcp(w[s], sum, rs);
             }
-            bool zero = true;
+            // every selected weight at most 1e-5 in magnitude (src/TSDFWarp.cu:179-181).
+            // Slots at or beyond n hold 0.  A NaN weight makes the sum NaN and
+            // then every selected weight NaN, so the sum's NaN test stands in
+            // for the per-weight comparisons a maximum would skip.
+            float wmax = 0.0f;
 #pragma unroll
-            for (int s = 0; s < K; ++s)
-                if (s < n) zero = zero && (__builtin_fabsf(w[s]) <= 1e-5f);
+            for (int s = 0; s < K; ++s) wmax = __builtin_fmaxf(wmax, __builtin_fabsf(w[s]));
+            const bool zero = (sum == sum) && (wmax <= 1e-5f);
             // vertex + c (src/VolumeFusion.cu:723), warp (src/TSDFWarp.cu:183-189)
             float pc[3];
             pc[0] = p[0] + vd.c[0];
