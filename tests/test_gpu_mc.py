@@ -35,6 +35,46 @@ def test_sphere(vf, oracle, n, full):
         np.testing.assert_array_equal(f, fo)
 
 
+@pytest.mark.parametrize("iso", [0.0, 0.5])
+def test_degenerate_triangles(vf, oracle, iso):
+    """TSDF values on a coarse grid with the iso value among them: edge points
+    snap to cube corners, so triangles with two equal points occur and are
+    dropped, exactly as the oracle drops them."""
+    rng = np.random.default_rng(21)
+    n = 24
+    tsdf = rng.choice(np.array([-1.0, -0.5, 0.0, 0.5, 1.0], F), n ** 3).astype(F)
+    wt = np.ones(n ** 3, F)
+    dims, vs, size = (n, n, n), (F(0.01),) * 3, (F(0.24),) * 3
+    v, f = ops.marching_cubes(ops.Workspace(), dev(tsdf), dev(wt), dims, vs, size, iso=iso, full=True)
+    vo, fo = oracle.marching_cubes(tsdf, wt, dims, vs, size, iso=iso, full=True)
+    np.testing.assert_array_equal(v, vo)
+    np.testing.assert_array_equal(f, fo)
+    # the cases' triangles outnumber the kept ones: degenerate ones were dropped
+    from test_oracle_mc import tri_table
+    ntri = np.array([len(t) for t in tri_table()])
+    g = tsdf.reshape(n, n, n)  # [z, y, x]
+    corner = [(1, 1, 0), (1, 0, 0), (0, 0, 0), (0, 1, 0), (1, 1, 1), (1, 0, 1), (0, 0, 1), (0, 1, 1)]
+    case = np.zeros((n - 2, n - 2, n - 2), np.int64)
+    for b, (dx, dy, dz) in enumerate(corner):
+        case |= (g[dz:dz + n - 2, dy:dy + n - 2, dx:dx + n - 2] > F(iso)).astype(np.int64) << b
+    assert fo.shape[0] > 1000 and fo.shape[0] < ntri[case].sum()
+
+
+def test_coinciding_corner_positions(vf, oracle):
+    """A volume size far larger than dims x voxel size puts every corner at
+    the same binary32 position: the exact point comparison decides."""
+    tsdf, wt, dims, vs, size = sphere(20, 0.04, 0.27)
+    size = (F(1e9),) * 3
+    v, f = ops.marching_cubes(ops.Workspace(), dev(tsdf), dev(wt), dims, vs, size, full=True)
+    vo, fo = oracle.marching_cubes(tsdf, wt, dims, vs, size, full=True)
+    np.testing.assert_array_equal(v, vo)
+    np.testing.assert_array_equal(f, fo)
+    vm, _ = ops.marching_cubes(ops.Workspace(), dev(tsdf), dev(wt), dims, vs, size, full=False)
+    vmo, _ = oracle.marching_cubes(tsdf, wt, dims, vs, size, full=False)
+    np.testing.assert_array_equal(vm, vmo)
+    assert vmo.shape[0] > 100
+
+
 @pytest.mark.parametrize("full", [False, True])
 def test_integrated_volume(vf, oracle, full):
     """The TSDF the fused integration leaves after two frames (weight-0 voxels

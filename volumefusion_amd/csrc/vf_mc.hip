@@ -5,12 +5,15 @@
 // counter, so its output order is a race and the counter a hot spot.  Here
 // the output is compacted in a canonical order (cubes by linear index, edges
 // by number, triangles in table order) with two sweeps:
-//   k_mc_count   per cube: case and output count; per 256-cube block: sum
-//   (scan)       exclusive scan of the block sums (rocPRIM)
-//   k_mc_emit    per cube again: block-local prefix, then the writes
+//   k_mc_count   per cube: case and output count; per run of 63 cubes of a
+//                row: sum
+//   (scan)       exclusive scan of the run sums (rocPRIM)
+//   k_mc_emit    per cube again, only in runs with output: wave-local
+//                prefix, then the writes
 //   k_mc_finish  advances the caller's counters
-// Each sweep reads the 8 corners of every cube (tsdf and weight), so the
-// pair is HBM-bound: about 2 x 8 B per voxel.
+// The count sweep reads every voxel's tsdf and weight about once (waves march
+// along z and share x neighbours across lanes); the emit sweep reads only the
+// runs on the surface.  HBM-bound: about 8 B per voxel.
 #include <hipcub/hipcub.hpp>
 
 #include "vf_host.hpp"
@@ -24,6 +27,33 @@ constexpr int kMcBlock = 256;
 __constant__ unsigned long long kMcTri[256] = VF_MC_TRI_TABLE;
 __constant__ int kMcEdge[12][2] = VF_MC_EDGE_CORNERS;
 __constant__ int kMcCorner[8][3] = VF_MC_CORNER_OFFSETS;
+constexpr int kCornerOff[8][3] = VF_MC_CORNER_OFFSETS;  // compile-time copies for register selection
+constexpr int kMcEdgeC[12][2] = VF_MC_EDGE_CORNERS;
+constexpr unsigned long long kMcTriC[256] = VF_MC_TRI_TABLE;
+
+// Case lookup of the count sweep.  Index: bits 0-3 the above-iso flags of
+// this lane's corners (row j and j + 1 of slice k, then of slice k + 1),
+// bits 4-7 the same flags of the next lane's (x + 1) corners.  Entry: the
+// case (bits 0-7), its crossed-edge count (8-15), its triangle count (16-23).
+struct McCaseLut {
+    uint32_t v[256];
+    constexpr McCaseLut() : v() {
+        for (int idx = 0; idx < 256; ++idx) {
+            int cs = 0;
+            for (int c = 0; c < 8; ++c) {
+                const int dx = kCornerOff[c][0], dy = kCornerOff[c][1], dz = kCornerOff[c][2];
+                const int bit = (idx >> (4 * dx + dy + 2 * dz)) & 1;
+                cs |= bit << c;
+            }
+            int nedge = 0;
+            for (int e = 0; e < 12; ++e) nedge += ((cs >> kMcEdgeC[e][0]) ^ (cs >> kMcEdgeC[e][1])) & 1;
+            const int ntri = (cs == 0 || cs == 255) ? 0 : (int)(kMcTriC[cs] & 15);
+            if (cs == 0 || cs == 255) nedge = 0;
+            v[idx] = (uint32_t)cs | ((uint32_t)nedge << 8) | ((uint32_t)ntri << 16);
+        }
+    }
+};
+__device__ const McCaseLut kMcCaseLut = McCaseLut();
 
 struct McGeom {
     int32_t X, Y, Z;
@@ -32,6 +62,8 @@ struct McGeom {
     float halfs[3];
     float iso;
     uint64_t ncubes;
+    int32_t cz;  // cube layers (Z - 2)
+    int32_t nr;  // runs of kRun cubes per row
 };
 
 // interpolate, src/MarchingCubes.cu:294-312: ratio in float, widened,
@@ -71,29 +103,6 @@ struct Cube {
     float t[8];      // corner tsdf
 };
 
-__device__ __forceinline__ void load_cube(const McGeom& g, uint64_t c, const float* __restrict__ tsdf,
-                                          const float* __restrict__ wt, Cube* cb) {
-    cb->i = (int32_t)(c % (uint64_t)g.cx);
-    cb->j = (int32_t)((c / (uint64_t)g.cx) % (uint64_t)g.cy);
-    cb->k = (int32_t)(c / ((uint64_t)g.cx * (uint64_t)g.cy));
-    bool zero = false;
-    uint64_t off[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v)
-        off[v] = ((uint64_t)(cb->k + kMcCorner[v][2]) * (uint64_t)g.Y + (uint64_t)(cb->j + kMcCorner[v][1])) *
-                     (uint64_t)g.X +
-                 (uint64_t)(cb->i + kMcCorner[v][0]);
-#pragma unroll
-    for (int v = 0; v < 8; ++v) zero = zero || (wt[off[v]] == 0.0f);
-    int cs = 0;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        cb->t[v] = tsdf[off[v]];
-        if (cb->t[v] > g.iso) cs |= 1 << v;
-    }
-    cb->cs = zero ? 0 : cs;
-}
-
 __device__ __forceinline__ int edge_mask(int cs) {
     int m = 0;
 #pragma unroll
@@ -114,119 +123,359 @@ __device__ __forceinline__ bool same3(const float* a, const float* b) {
     return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
 }
 
-// Output vertex count of a cube; with FULL, its edge points are left in ep.
+// The cube's corner positions differ in every axis (always, unless the
+// voxel grid is too fine for binary32 where the cube sits).
+__device__ __forceinline__ bool corners_distinct(const McGeom& g, const Cube& cb) {
+    float p0[3], p1[3];
+    mc_world(g, cb.i, cb.j, cb.k, p0);
+    mc_world(g, cb.i + 1, cb.j + 1, cb.k + 1, p1);
+    return p0[0] != p1[0] && p0[1] != p1[1] && p0[2] != p1[2];
+}
+
+// For each crossed edge e, four bits at 4 e: 1 + the corner its point equals
+// exactly, or 0.  An edge point keeps the edge line's two constant
+// coordinates exactly and its third between the corners', so two points of
+// different edges are equal only where the two edge lines meet: a corner
+// both edges end at, when the corner positions differ in every axis.  Two
+// points are then equal exactly when their codes are equal and non-zero.
+__device__ __forceinline__ unsigned long long edge_codes(const McGeom& g, const Cube& cb, int mask) {
+    unsigned long long c = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        if (mask & (1 << e)) {
+            const int a = kMcEdgeC[e][0], b = kMcEdgeC[e][1];
+            float pa[3], pb[3], o[3];
+            mc_world(g, cb.i + kCornerOff[a][0], cb.j + kCornerOff[a][1], cb.k + kCornerOff[a][2], pa);
+            mc_world(g, cb.i + kCornerOff[b][0], cb.j + kCornerOff[b][1], cb.k + kCornerOff[b][2], pb);
+            mc_interp(cb.t[a], cb.t[b], pa, pb, g.iso, o);
+            const unsigned long long code = same3(o, pa) ? (unsigned long long)(a + 1)
+                                          : same3(o, pb) ? (unsigned long long)(b + 1)
+                                                         : 0ull;
+            c |= code << (4 * e);
+        }
+    }
+    return c;
+}
+
+// Output vertex count of a cube: 3 per triangle of its case (FULL, dropping
+// triangles with two equal points, src/MarchingCubes.cu:400-410) or 1 per
+// crossed edge.  With FULL, bit q of keep marks the kept triangles.
 template <bool FULL>
-__device__ __forceinline__ int cube_outputs(const McGeom& g, const Cube& cb, float ep[12][3], unsigned& keep) {
+__device__ __forceinline__ int cube_outputs(const McGeom& g, const Cube& cb, unsigned& keep) {
     keep = 0;
     if (cb.cs == 0 || cb.cs == 255) return 0;
     const int mask = edge_mask(cb.cs);
     if (!FULL) return __popc(mask);
-#pragma unroll
-    for (int e = 0; e < 12; ++e)
-        if (mask & (1 << e)) edge_point(g, cb, e, ep[e]);
     const unsigned long long t = kMcTri[cb.cs];
     const int nt = (int)(t & 15);
     int n = 0;
-    for (int q = 0; q < nt; ++q) {
-        const float* p0 = ep[(t >> (4 + 12 * q)) & 15];
-        const float* p1 = ep[(t >> (8 + 12 * q)) & 15];
-        const float* p2 = ep[(t >> (12 + 12 * q)) & 15];
-        if (!same3(p0, p1) && !same3(p0, p2) && !same3(p1, p2)) {
-            keep |= 1u << q;
-            n += 3;
+    if (corners_distinct(g, cb)) {
+        const unsigned long long codes = edge_codes(g, cb, mask);
+        for (int q = 0; q < nt; ++q) {
+            const int c0 = (int)(codes >> (4 * ((t >> (4 + 12 * q)) & 15))) & 15;
+            const int c1 = (int)(codes >> (4 * ((t >> (8 + 12 * q)) & 15))) & 15;
+            const int c2 = (int)(codes >> (4 * ((t >> (12 + 12 * q)) & 15))) & 15;
+            const bool degenerate = (c0 && (c0 == c1 || c0 == c2)) || (c1 && c1 == c2);
+            if (!degenerate) {
+                keep |= 1u << q;
+                n += 3;
+            }
+        }
+    } else {
+        // corners that share a coordinate: compare the points themselves
+        for (int q = 0; q < nt; ++q) {
+            float p0[3], p1[3], p2[3];
+            edge_point(g, cb, (int)((t >> (4 + 12 * q)) & 15), p0);
+            edge_point(g, cb, (int)((t >> (8 + 12 * q)) & 15), p1);
+            edge_point(g, cb, (int)((t >> (12 + 12 * q)) & 15), p2);
+            if (!same3(p0, p1) && !same3(p0, p2) && !same3(p1, p2)) {
+                keep |= 1u << q;
+                n += 3;
+            }
         }
     }
     return n;
 }
 
-template <bool FULL>
-__global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __restrict__ tsdf,
-                                                        const float* __restrict__ wt, uint32_t* __restrict__ sums) {
-    typedef hipcub::BlockReduce<uint32_t, kMcBlock> Reduce;
-    __shared__ typename Reduce::TempStorage tmp;
-    const uint64_t c = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
-    uint32_t n = 0;
-    if (c < g.ncubes) {
-        Cube cb;
-        load_cube(g, c, tsdf, wt, &cb);
-        float ep[12][3];
-        unsigned keep;
-        n = (uint32_t)cube_outputs<FULL>(g, cb, ep, keep);
+// The sweeps march along z.  A wave owns a run of kRun cubes of one row
+// (j, k): lane l holds voxel column x = r * kRun + l, and the cube at x takes
+// its x + 1 corners from lane l + 1 (lane 63 only supplies corners).  Each
+// step down z loads one new slice: 4 coalesced loads per lane (tsdf and
+// weight of rows j and j + 1).  Runs are numbered q = (k * cy + j) * nr + r,
+// which is the cubes' linear order, so per-run sums scanned in q order give
+// every cube's output offset.
+constexpr int kRun = 63;
+constexpr int kMarch = 16;  // z steps per wave
+static_assert(kMarch <= 64, "one lane per step holds the run sums");
+
+struct Slice {
+    float t0, t1;  // tsdf at rows j, j + 1 of this lane's column
+    float w0, w1;  // weights
+};
+
+// Columns beyond the row's end read the last column instead: only lanes
+// that own no cube see them, through their own values or their left
+// neighbour's, which owns none either.
+__device__ __forceinline__ Slice load_slice(const McGeom& g, const float* __restrict__ tsdf,
+                                            const float* __restrict__ wt, int32_t x, int32_t j, int32_t k) {
+    const int32_t xc = x < g.X ? x : g.X - 1;
+    const uint64_t o0 = ((uint64_t)k * (uint64_t)g.Y + (uint64_t)j) * (uint64_t)g.X + (uint64_t)xc;
+    const uint64_t o1 = o0 + (uint64_t)g.X;
+    Slice s;
+    s.t0 = tsdf[o0];
+    s.t1 = tsdf[o1];
+    s.w0 = wt[o0];
+    s.w1 = wt[o1];
+    return s;
+}
+
+// The value of lane l + 1 (DPP wave_shl:1, one VALU instruction); lane 63
+// gets 0.
+__device__ __forceinline__ float from_next_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// The cube at (x, j, k) from this lane's slices k (lo) and k + 1 (hi) and the
+// next lane's.  kCornerOff gives each corner's (dx, dy, dz).
+__device__ __forceinline__ void make_cube(const McGeom& g, int32_t x, int32_t j, int32_t k, const Slice& lo,
+                                          const Slice& hi, Cube* cb) {
+    const float nt0 = from_next_lane(lo.t0), nt1 = from_next_lane(lo.t1);
+    const float nw0 = from_next_lane(lo.w0), nw1 = from_next_lane(lo.w1);
+    const float mt0 = from_next_lane(hi.t0), mt1 = from_next_lane(hi.t1);
+    const float mw0 = from_next_lane(hi.w0), mw1 = from_next_lane(hi.w1);
+    // [dz][dy][dx]
+    const float T[2][2][2] = {{{lo.t0, nt0}, {lo.t1, nt1}}, {{hi.t0, mt0}, {hi.t1, mt1}}};
+    const float W[2][2][2] = {{{lo.w0, nw0}, {lo.w1, nw1}}, {{hi.w0, mw0}, {hi.w1, mw1}}};
+    cb->i = x;
+    cb->j = j;
+    cb->k = k;
+    bool zero = false;
+    int cs = 0;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const int dx = kCornerOff[v][0], dy = kCornerOff[v][1], dz = kCornerOff[v][2];
+        zero = zero || (W[dz][dy][dx] == 0.0f);
+        cb->t[v] = T[dz][dy][dx];
+        if (cb->t[v] > g.iso) cs |= 1 << v;
     }
-    const uint32_t total = Reduce(tmp).Sum(n);
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+    cb->cs = zero ? 0 : cs;
+}
+
+// Inclusive prefix sum over the wave in DPP moves (VALU only): shifts by 1,
+// 2, 4, 8 within each row of 16 lanes, then row 15's and row 31's lane added
+// to the rows above them.
+#define VF_DPP(x, ctrl, rows) (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(x), ctrl, rows, 0xf, false)
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += VF_DPP(x, 0x111, 0xf);  // row_shr:1
+    x += VF_DPP(x, 0x112, 0xf);  // row_shr:2
+    x += VF_DPP(x, 0x114, 0xf);  // row_shr:4
+    x += VF_DPP(x, 0x118, 0xf);  // row_shr:8
+    x += VF_DPP(x, 0x142, 0xa);  // row_bcast:15 into rows 1 and 3
+    x += VF_DPP(x, 0x143, 0xc);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+#undef VF_DPP
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_sum(v), 63);
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, int) { return wave_inclusive_sum(v) - v; }
+
+struct RunPos {
+    int32_t r, j, k0;
+    bool live;
+};
+
+// A workgroup's four waves take four consecutive rows j of the same run and
+// z segment, so the voxel row two waves share (j + 1 of one, j of the next)
+// is fetched from HBM once and read again from the CU's caches.
+__device__ __forceinline__ RunPos run_pos(const McGeom& g) {
+    const uint32_t groups = ((uint32_t)g.cy + (kMcBlock / 64) - 1) / (kMcBlock / 64);
+    const uint32_t b = blockIdx.x;
+    RunPos p;
+    p.r = (int32_t)(b % (uint32_t)g.nr);
+    p.j = (int32_t)(((b / (uint32_t)g.nr) % groups) * (kMcBlock / 64) + (threadIdx.x >> 6));
+    p.k0 = (int32_t)(b / ((uint32_t)g.nr * groups)) * kMarch;
+    p.live = p.k0 < g.cz && p.j < g.cy;
+    return p;
+}
+
+// Above-iso flags of a lane's two rows (bits 0, 1) and whether either weight
+// is 0 (bit 2), for one slice.
+__device__ __forceinline__ uint32_t slice_bits(const Slice& s, float iso) {
+    return (s.t0 > iso ? 1u : 0u) | (s.t1 > iso ? 2u : 0u) | ((s.w0 == 0.0f || s.w1 == 0.0f) ? 4u : 0u);
+}
+
+// FULL needs many registers only for surface cubes (edge points of the
+// degenerate-triangle test); capping it keeps the z-march at full occupancy.
+template <bool FULL>
+__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_mc_count(
+    McGeom g, const float* __restrict__ tsdf, const float* __restrict__ wt, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_lut[256];
+    static_assert(kMcBlock == 256, "one lookup entry per thread");
+    s_lut[threadIdx.x] = kMcCaseLut.v[threadIdx.x];
+    __syncthreads();
+    const RunPos rp = run_pos(g);
+    if (!rp.live) return;
+    const int lane = __lane_id();
+    const int32_t x = rp.r * kRun + lane;
+    const bool own = lane < kRun && x < g.cx;
+    const int32_t k1 = rp.k0 + kMarch < g.cz ? rp.k0 + kMarch : g.cz;
+    // slices k .. k + 3 are loaded or in flight while step k computes; near
+    // the segment's end the prefetch re-reads slice k1 (the last one used)
+    Slice lo = load_slice(g, tsdf, wt, x, rp.j, rp.k0);
+    Slice hi = load_slice(g, tsdf, wt, x, rp.j, rp.k0 + 1);
+    Slice n1 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 2, k1));
+    Slice n2 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 3, k1));
+    uint32_t blo = slice_bits(lo, g.iso);
+    uint32_t pending = 0;  // the previous step's count, 16 bits, reduced with this one
+    for (int32_t k = rp.k0; k < k1; ++k) {
+        const Slice n3 = load_slice(g, tsdf, wt, x, rp.j, min(k + 4, k1));
+        const uint32_t bhi = slice_bits(hi, g.iso);
+        // this lane's corner flags and the next lane's (x + 1), one DPP move
+        const uint32_t mine = (blo & 3u) | ((bhi & 3u) << 2) | (((blo | bhi) & 4u) << 2);
+        const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mine, 0x130, 0xf, 0xf, false);
+        const bool zero = ((mine | next) & 16u) != 0u;
+        const uint32_t e = s_lut[(mine & 15u) | ((next & 15u) << 4)];
+        const uint32_t cs = zero ? 0u : (e & 255u);
+        uint32_t n = 0;
+        if (!FULL) {
+            n = own && !zero ? (e >> 8) & 255u : 0u;
+        } else {
+            const bool surface = own && cs != 0u && cs != 255u;
+            if (__ballot(surface)) {
+                Cube cb;
+                make_cube(g, x, rp.j, k, lo, hi, &cb);
+                unsigned keep;
+                if (surface) n = (uint32_t)cube_outputs<true>(g, cb, keep);
+            }
+        }
+        // two steps per wave reduction: a run's count is below 2^16
+        // (63 cubes x at most 15 vertices)
+        const bool odd = ((k - rp.k0) & 1) != 0;
+        if (!odd && k + 1 < k1) {
+            pending = n;
+        } else {
+            const uint32_t both = wave_sum(odd ? (pending | (n << 16)) : n);
+            if (lane == 0) {
+                const uint64_t q = ((uint64_t)k * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
+                if (odd) {
+                    sums[q - (uint64_t)g.cy * (uint64_t)g.nr] = both & 0xffffu;
+                    sums[q] = both >> 16;
+                } else {
+                    sums[q] = both;
+                }
+            }
+        }
+        lo = hi;
+        hi = n1;
+        n1 = n2;
+        n2 = n3;
+        blo = bhi;
+    }
 }
 
 template <bool FULL>
 __global__ __launch_bounds__(kMcBlock) void k_mc_emit(McGeom g, const float* __restrict__ tsdf,
                                                        const float* __restrict__ wt,
+                                                       const uint32_t* __restrict__ sums,
                                                        const uint32_t* __restrict__ offsets,
                                                        const uint32_t* __restrict__ mesh_size_in,
                                                        const uint32_t* __restrict__ faces_size_in,
                                                        float* __restrict__ mesh, uint64_t mesh_cap,
                                                        int32_t* __restrict__ faces, uint64_t faces_cap) {
-    typedef hipcub::BlockScan<uint32_t, kMcBlock> Scan;
-    __shared__ typename Scan::TempStorage tmp;
-    const uint64_t c = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
-    Cube cb;
-    cb.cs = 0;
-    float ep[12][3];
-    unsigned keep = 0;
-    uint32_t n = 0;
-    if (c < g.ncubes) {
-        load_cube(g, c, tsdf, wt, &cb);
-        n = (uint32_t)cube_outputs<FULL>(g, cb, ep, keep);
+    // FULL: this thread's edge points, [edge][axis][thread]
+    __shared__ float s_ep[FULL ? 12 * 3 * kMcBlock : 1];
+    const RunPos rp = run_pos(g);
+    if (!rp.live) return;
+    const int lane = __lane_id();
+    const int32_t x = rp.r * kRun + lane;
+    const bool own = lane < kRun && x < g.cx;
+    const int32_t k1 = rp.k0 + kMarch < g.cz ? rp.k0 + kMarch : g.cz;
+    // the column's run sums and offsets, lane l holding step k0 + l; most
+    // runs hold no surface and are skipped without reading the volume
+    const uint64_t kstride = (uint64_t)g.cy * (uint64_t)g.nr;
+    const uint64_t q0 = ((uint64_t)rp.k0 * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
+    uint32_t my_sum = 0, my_off = 0;
+    if (lane < k1 - rp.k0) {
+        my_sum = sums[q0 + (uint64_t)lane * kstride];
+        my_off = offsets[q0 + (uint64_t)lane * kstride];
     }
-    uint32_t prefix;
-    Scan(tmp).ExclusiveSum(n, prefix);
-    if (!n) return;
-    const uint64_t rel = (uint64_t)offsets[blockIdx.x] + prefix;  // this call's vertex index
-    const uint64_t v0 = (uint64_t)*mesh_size_in + rel;            // appended after the caller's count
-    if (!FULL) {
+    unsigned long long steps = __ballot(my_sum != 0u);
+    while (steps) {
+        const int kk = __builtin_ctzll(steps);
+        steps &= steps - 1ull;
+        const int32_t k = rp.k0 + kk;
+        const uint32_t run_off = __shfl(my_off, kk);
+        const Slice lo = load_slice(g, tsdf, wt, x, rp.j, k);
+        const Slice hi = load_slice(g, tsdf, wt, x, rp.j, k + 1);
+        Cube cb;
+        make_cube(g, x, rp.j, k, lo, hi, &cb);
+        unsigned keep = 0;
+        uint32_t n = 0;
+        if (own) n = (uint32_t)cube_outputs<FULL>(g, cb, keep);
+        const uint32_t prefix = wave_exclusive_sum(n, lane);
+        if (!n) continue;
+        const uint64_t rel = (uint64_t)run_off + prefix;  // this call's vertex index
+        const uint64_t v0 = (uint64_t)*mesh_size_in + rel;  // appended after the caller's count
+        if (!FULL) {
+            const int mask = edge_mask(cb.cs);
+            uint64_t v = v0;
+            for (int e = 0; e < 12; ++e) {
+                if (!(mask & (1 << e))) continue;
+                float o[3];
+                edge_point(g, cb, e, o);
+                if (v < mesh_cap) {
+                    mesh[3 * v] = o[0];
+                    mesh[3 * v + 1] = o[1];
+                    mesh[3 * v + 2] = o[2];
+                }
+                ++v;
+            }
+            continue;
+        }
+        const unsigned long long t = kMcTri[cb.cs];
+        const int nt = (int)(t & 15);
         const int mask = edge_mask(cb.cs);
-        uint64_t v = v0;
-        for (int e = 0; e < 12; ++e) {
-            if (!(mask & (1 << e))) continue;
-            float o[3];
-            edge_point(g, cb, e, o);
-            if (v < mesh_cap) {
-                mesh[3 * v] = o[0];
-                mesh[3 * v + 1] = o[1];
-                mesh[3 * v + 2] = o[2];
-            }
-            ++v;
-        }
-        return;
-    }
-    const unsigned long long t = kMcTri[cb.cs];
-    const int nt = (int)(t & 15);
-    uint64_t v = v0;
-    uint64_t f = (uint64_t)*faces_size_in + rel / 3;
-    for (int q = 0; q < nt; ++q) {
-        if (!(keep & (1u << q))) continue;
+        float* ep = s_ep + threadIdx.x;
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const float* p = ep[(t >> (4 + 4 * (3 * q + s))) & 15];
-            if (v + s < mesh_cap) {
-                mesh[3 * (v + s)] = p[0];
-                mesh[3 * (v + s) + 1] = p[1];
-                mesh[3 * (v + s) + 2] = p[2];
+        for (int e = 0; e < 12; ++e) {
+            if (mask & (1 << e)) {
+                float o[3];
+                edge_point(g, cb, e, o);
+                ep[(3 * e + 0) * kMcBlock] = o[0];
+                ep[(3 * e + 1) * kMcBlock] = o[1];
+                ep[(3 * e + 2) * kMcBlock] = o[2];
             }
         }
-        if (f < faces_cap) {
-            faces[3 * f] = (int32_t)v;
-            faces[3 * f + 1] = (int32_t)(v + 1);
-            faces[3 * f + 2] = (int32_t)(v + 2);
+        uint64_t v = v0;
+        uint64_t f = (uint64_t)*faces_size_in + rel / 3;
+        for (int qq = 0; qq < nt; ++qq) {
+            if (!(keep & (1u << qq))) continue;
+            for (int s = 0; s < 3; ++s) {
+                const int e = (int)((t >> (4 + 4 * (3 * qq + s))) & 15);
+                if (v + s < mesh_cap) {
+                    mesh[3 * (v + s)] = ep[(3 * e + 0) * kMcBlock];
+                    mesh[3 * (v + s) + 1] = ep[(3 * e + 1) * kMcBlock];
+                    mesh[3 * (v + s) + 2] = ep[(3 * e + 2) * kMcBlock];
+                }
+            }
+            if (f < faces_cap) {
+                faces[3 * f] = (int32_t)v;
+                faces[3 * f + 1] = (int32_t)(v + 1);
+                faces[3 * f + 2] = (int32_t)(v + 2);
+            }
+            v += 3;
+            f += 1;
         }
-        v += 3;
-        f += 1;
     }
 }
 
-__global__ void k_mc_finish(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ sums, uint32_t nb,
+__global__ void k_mc_finish(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ sums, uint64_t nq,
                             uint32_t* mesh_size, uint32_t* faces_size) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t total = offsets[nb - 1] + sums[nb - 1];
+    const uint32_t total = offsets[nq - 1] + sums[nq - 1];
     *mesh_size = *mesh_size + total;
     if (faces_size) *faces_size = *faces_size + total / 3;
 }
@@ -254,19 +503,24 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     g.halfs[2] = size.z * 0.5f;
     g.iso = iso;
     g.ncubes = (uint64_t)g.cx * (uint64_t)g.cy * (uint64_t)(dims.z - 2);
-    const uint64_t nb64 = (g.ncubes + kMcBlock - 1) / kMcBlock;
-    if (nb64 > 0x7fffffffull) return fail(VF_ERR_CAPACITY, "marching cubes: %llu blocks", (unsigned long long)nb64);
+    g.cz = dims.z - 2;
+    g.nr = (g.cx + kRun - 1) / kRun;
+    const uint64_t nq = (uint64_t)g.cz * (uint64_t)g.cy * (uint64_t)g.nr;  // runs
+    const uint64_t groups = ((uint64_t)g.cy + kMcBlock / 64 - 1) / (kMcBlock / 64);
+    const uint64_t nb64 = (uint64_t)g.nr * groups * (uint64_t)((g.cz + kMarch - 1) / kMarch);
+    if (nb64 > 0x7fffffffull || nq > 0x7fffffffull)
+        return fail(VF_ERR_CAPACITY, "marching cubes: %llu runs", (unsigned long long)nq);
     const uint32_t nb = (uint32_t)nb64;
     size_t scan_bytes = 0;
-    VF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nb,
+    VF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nq,
                                             stream));
-    const uint64_t sums_bytes = (uint64_t)nb * 4;
+    const uint64_t sums_bytes = nq * 4;
     const uint64_t scan_off = (2 * sums_bytes + 255) & ~255ull;
     void* scratch = nullptr;
     st = workspace_scratch(ws, scan_off + scan_bytes, &scratch, stream);
     if (st) return st;
     uint32_t* sums = (uint32_t*)scratch;
-    uint32_t* offsets = sums + nb;
+    uint32_t* offsets = sums + nq;
     void* scan_tmp = (char*)scratch + scan_off;
     if (full) {
         hipLaunchKernelGGL(k_mc_count<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums);
@@ -274,16 +528,16 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
         hipLaunchKernelGGL(k_mc_count<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums);
     }
     VF_LAUNCH_CHECK("k_mc_count");
-    VF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, sums, offsets, (int)nb, stream));
+    VF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, sums, offsets, (int)nq, stream));
     if (full) {
-        hipLaunchKernelGGL(k_mc_emit<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, offsets, mesh_size,
-                           faces_size, mesh, mesh_cap, faces, faces_cap);
+        hipLaunchKernelGGL(k_mc_emit<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums, offsets,
+                           mesh_size, faces_size, mesh, mesh_cap, faces, faces_cap);
     } else {
-        hipLaunchKernelGGL(k_mc_emit<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, offsets, mesh_size,
-                           (const uint32_t*)nullptr, mesh, mesh_cap, (int32_t*)nullptr, (uint64_t)0);
+        hipLaunchKernelGGL(k_mc_emit<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums, offsets,
+                           mesh_size, (const uint32_t*)nullptr, mesh, mesh_cap, (int32_t*)nullptr, (uint64_t)0);
     }
     VF_LAUNCH_CHECK("k_mc_emit");
-    hipLaunchKernelGGL(k_mc_finish, dim3(1), dim3(64), 0, stream, offsets, sums, nb, mesh_size,
+    hipLaunchKernelGGL(k_mc_finish, dim3(1), dim3(64), 0, stream, offsets, sums, nq, mesh_size,
                        full ? faces_size : (uint32_t*)nullptr);
     VF_LAUNCH_CHECK("k_mc_finish");
     return VF_OK;
