@@ -51,6 +51,48 @@ def test_mesh_knn(vf, oracle, k, exclude, G):
     np.testing.assert_array_equal(dist.download(), d_or)
 
 
+def _knn_case(oracle, k, pts, nodes, exclude=False):
+    idx, dist = ops.get_knn_data(ops.Workspace(), k, dev(pts), dev(nodes), exclude_closest=exclude)
+    i_or, d_or = oracle.mesh_knn(k, pts, nodes, exclude_closest=exclude)
+    np.testing.assert_array_equal(idx.download(), i_or)
+    np.testing.assert_array_equal(dist.download(), d_or)
+
+
+@pytest.mark.parametrize("exclude", [False, True])
+def test_mesh_knn_grid_geometry(vf, oracle, exclude):
+    """The node grid's search against the brute force: points far outside
+    the nodes' box (many rings), nodes on a plane and on a line (zero extent
+    in some axes), all nodes at one position, a dense cluster beside sparse
+    nodes, and points and nodes at infinity."""
+    rng = np.random.default_rng(77)
+    k = 8
+    # far points
+    nodes = rng.uniform(-0.3, 0.3, (3000, 3)).astype(F)
+    far = rng.uniform(-1, 1, (500, 3)).astype(F) * F(20.0)
+    _knn_case(oracle, k, np.concatenate([far, nodes[:50] + F(1e-3)]), nodes, exclude)
+    # a plane and a line of nodes
+    plane = nodes.copy()
+    plane[:, 2] = F(0.1)
+    _knn_case(oracle, k, rng.uniform(-0.4, 0.4, (2000, 3)).astype(F), plane, exclude)
+    line = nodes.copy()
+    line[:, 1:] = F(-0.2)
+    _knn_case(oracle, k, rng.uniform(-0.4, 0.4, (2000, 3)).astype(F), line, exclude)
+    # all nodes at one position: every distance ties, index order decides
+    same = np.tile(np.array([[0.05, -0.1, 0.2]], F), (500, 1))
+    _knn_case(oracle, k, rng.uniform(-0.3, 0.3, (500, 3)).astype(F), same, exclude)
+    # a dense cluster and sparse nodes
+    cluster = np.concatenate([rng.normal(0, 0.002, (4000, 3)), rng.uniform(-1, 1, (200, 3))]).astype(F)
+    pts = np.concatenate([rng.normal(0, 0.01, (1500, 3)), rng.uniform(-1, 1, (1500, 3))]).astype(F)
+    _knn_case(oracle, k, pts, cluster, exclude)
+    # infinite coordinates: such nodes sit outside the grid, such points see every node at infinity
+    inf_nodes = nodes[:600].copy()
+    inf_nodes[::50, 0] = np.inf
+    inf_nodes[1::50, 2] = -np.inf
+    inf_pts = rng.uniform(-0.3, 0.3, (800, 3)).astype(F)
+    inf_pts[::40, 1] = np.inf
+    _knn_case(oracle, k, inf_pts, inf_nodes, exclude)
+
+
 @pytest.mark.parametrize("k", [3, 6, 8])
 def test_weights_and_warp_one_by_one(vf, oracle, k):
     rng = np.random.default_rng(k)
