@@ -1687,24 +1687,43 @@ void vfo_remove_depth_without_normals(const float* normals, float* v, float* dep
  * blocks reduced as a binary tree, a[i] += a[i + s] for s = 128 .. 1, then the
  * block sums in order), rounded once to float. */
 void vfo_depth_centroid(const float* mesh, size_t n, float* out, uint64_t* count) {
-    double tot[3] = {0.0, 0.0, 0.0};
+    /* block totals: a binary tree over each 256-point block */
+    const size_t nb = (n + 255) / 256;
+    double* bt = (double*)calloc(3 * (nb ? nb : 1), sizeof(double));
     uint64_t cnt = 0;
-    for (size_t b = 0; b < n; b += 256) {
+    for (size_t b = 0; b < nb; ++b) {
         double blk[256][3];
         for (int t = 0; t < 256; ++t)
             for (int a = 0; a < 3; ++a)
-                blk[t][a] = (b + t < n) ? (double)mesh[3 * (b + t) + a] : 0.0;
+                blk[t][a] = (256 * b + t < n) ? (double)mesh[3 * (256 * b + t) + a] : 0.0;
         for (int s = 128; s >= 1; s >>= 1)
             for (int t = 0; t < s; ++t)
                 for (int a = 0; a < 3; ++a)
                     blk[t][a] = blk[t][a] + blk[t + s][a];
         for (int a = 0; a < 3; ++a)
-            tot[a] = tot[a] + blk[0][a];
-        for (int t = 0; t < 256 && b + t < n; ++t)
-            cnt += mesh[3 * (b + t) + 2] > 1.e-4f;
+            bt[3 * b + a] = blk[0][a];
+        for (int t = 0; t < 256 && 256 * b + t < n; ++t)
+            cnt += mesh[3 * (256 * b + t) + 2] > 1.e-4f;
     }
+    /* the totals: 1024 chunks of ceil(nb / 1024) consecutive blocks, each
+     * summed in order from 0, then a binary tree over the chunks */
+    double part[1024][3];
+    const size_t c = (nb + 1023) / 1024;
+    for (size_t t = 0; t < 1024; ++t) {
+        double q[3] = {0.0, 0.0, 0.0};
+        for (size_t b = t * c; b < (t + 1) * c && b < nb; ++b)
+            for (int a = 0; a < 3; ++a)
+                q[a] = q[a] + bt[3 * b + a];
+        for (int a = 0; a < 3; ++a)
+            part[t][a] = q[a];
+    }
+    for (int s = 512; s >= 1; s >>= 1)
+        for (int t = 0; t < s; ++t)
+            for (int a = 0; a < 3; ++a)
+                part[t][a] = part[t][a] + part[t + s][a];
+    free(bt);
     for (int a = 0; a < 3; ++a)
-        out[a] = (float)tot[a] / (float)cnt;
+        out[a] = (float)part[0][a] / (float)cnt;
     *count = cnt;
 }
 

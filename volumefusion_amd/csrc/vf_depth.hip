@@ -244,22 +244,46 @@ __global__ __launch_bounds__(256) void k_centroid_blocks(const float* __restrict
     centroid_block(p, i < n, s_sum, &s_cnt, block_sums, block_counts);
 }
 
-// Block sums in order; centroid = float(sum) / float(count).
-__global__ void k_centroid_final(const double* __restrict__ block_sums, const uint32_t* __restrict__ block_counts,
-                                 uint32_t nb, float* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    uint64_t c = 0;
-    for (uint32_t b = 0; b < nb; ++b) {
-        s0 = s0 + block_sums[3 * b];
-        s1 = s1 + block_sums[3 * b + 1];
-        s2 = s2 + block_sums[3 * b + 2];
-        c += block_counts[b];
+// The frame total from the block sums in a fixed order: 1024 chunks of
+// ceil(nb / 1024) consecutive blocks, each summed in order from 0, then a
+// binary tree over the chunks; centroid = float(total) / float(count).
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void k_centroid_final(const double* __restrict__ block_sums,
+                                                                  const uint32_t* __restrict__ block_counts,
+                                                                  uint32_t nb, float* __restrict__ out) {
+    __shared__ double s_part[3][kFinalThreads];
+    __shared__ unsigned long long s_cnt;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_cnt = 0ull;
+    const uint32_t c = (nb + kFinalThreads - 1) / kFinalThreads;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+    uint64_t cnt = 0;
+    for (uint32_t b = t * c; b < (t + 1) * c && b < nb; ++b) {
+        q0 = q0 + block_sums[3 * b];
+        q1 = q1 + block_sums[3 * b + 1];
+        q2 = q2 + block_sums[3 * b + 2];
+        cnt += block_counts[b];
     }
-    out[0] = (float)s0 / (float)c;
-    out[1] = (float)s1 / (float)c;
-    out[2] = (float)s2 / (float)c;
-    out[3] = __uint_as_float((uint32_t)c);  // count, bit pattern (frames are < 2^32 pixels)
+    s_part[0][t] = q0;
+    s_part[1][t] = q1;
+    s_part[2][t] = q2;
+    __syncthreads();
+    if (cnt) atomicAdd(&s_cnt, (unsigned long long)cnt);
+    for (uint32_t s = kFinalThreads / 2; s >= 1; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s_part[a][t] = s_part[a][t] + s_part[a][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint64_t total = s_cnt;
+        out[0] = (float)s_part[0][0] / (float)total;
+        out[1] = (float)s_part[1][0] / (float)total;
+        out[2] = (float)s_part[2][0] / (float)total;
+        out[3] = __uint_as_float((uint32_t)total);  // count, bit pattern (frames are < 2^32 pixels)
+    }
 }
 
 // Fused back-projection, normal, removal and centroid block sums: each pixel
@@ -417,7 +441,8 @@ int vf_depth_centroid(vf_workspace* ws, vf_float3* centroid, uint64_t* count, co
     hipLaunchKernelGGL(k_centroid_blocks, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const float*)mesh,
                        (uint64_t)n, sums, counts);
     VF_LAUNCH_CHECK("k_centroid_blocks");
-    hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(64), 0, (hipStream_t)stream, sums, counts, nb, out);
+    hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(kFinalThreads), 0, (hipStream_t)stream, sums, counts, nb,
+                       out);
     VF_LAUNCH_CHECK("k_centroid_final");
     float h[4];
     VF_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
@@ -470,7 +495,7 @@ int vf_ingest_depth(vf_workspace* ws, float* depth, int width, int height, vf_ma
                        (float*)normals, depth, sums, counts);
     VF_LAUNCH_CHECK("k_ingest_points");
     // 3. the centroid
-    hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(64), 0, s, sums, counts, nb, out);
+    hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(kFinalThreads), 0, s, sums, counts, nb, out);
     VF_LAUNCH_CHECK("k_centroid_final");
     float h[4];
     VF_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
