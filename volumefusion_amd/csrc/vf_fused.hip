@@ -569,20 +569,15 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                     sum = sum + w[s];
                 }
             }
-            if (sum != 0.0f) {
-                const float rs = 1.0f / sum;
-#pragma unroll
-                for (int s = 0; s < K; ++s)
-                    if (s < n) w[s] = div_rcp(w[s], sum, rs);
-            }
-            // every selected weight at most 1e-5 in magnitude (src/TSDFWarp.cu:179-181).
-            // Slots at or beyond n hold 0.  A NaN weight makes the sum NaN and
-            // then every selected weight NaN, so the sum's NaN test stands in
-            // for the per-weight comparisons a maximum would skip.
-            float wmax = 0.0f;
-#pragma unroll
-            for (int s = 0; s < K; ++s) wmax = __builtin_fmaxf(wmax, __builtin_fabsf(w[s]));
-            const bool zero = (sum == sum) && (wmax <= 1e-5f);
+            // isZero() of the normalised weights (src/TSDFWarp.cu:179-181):
+            // every weight at most 1e-5 in magnitude.  Each weight is NaN or
+            // in [-0, 1] (exp of a non-positive argument), so a zero sum means
+            // every weight is zero, a NaN sum makes every normalised weight
+            // NaN (not zero), and a positive sum of n <= 8 terms leaves the
+            // largest normalised weight at least 1/8: the test is sum == 0.
+            // The division by the sum (src/TSDFWarp.cu:112-113) is applied
+            // to each selected weight where it is used below.
+            const bool zero = sum == 0.0f;
             // vertex + c (src/VolumeFusion.cu:723), warp (src/TSDFWarp.cu:183-189)
             float pc[3];
             pc[0] = p[0] + vd.c[0];
@@ -593,7 +588,9 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 acc[0] = pc[0];
                 acc[1] = pc[1];
                 acc[2] = pc[2];
-            } else if (vd.blend == VF_BLEND_DQB) {
+            } else {
+            const float rs = 1.0f / sum;
+            if (vd.blend == VF_BLEND_DQB) {
                 // dual-quaternion blend of the selected nodes' rigid parts
                 float4 br = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bd = br;
                 const float4 f = spill ? nrec[vd.node_base + kl.i[0]].t0 : s_t0[kl.i[0]];
@@ -609,7 +606,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                             qr = s_t0[kl.i[s]];
                             qd = s_t1[kl.i[s]];
                         }
-                        dqb_accumulate(qr, qd, f, w[s], &br, &bd);
+                        dqb_accumulate(qr, qd, f, div_rcp(w[s], sum, rs), &br, &bd);
                     }
                 }
                 dqb_transform(br, bd, pc, acc);
@@ -628,15 +625,17 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                             r1 = s_t1[kl.i[s]];
                             r2 = s_t2[kl.i[s]];
                         }
+                        const float ws = div_rcp(w[s], sum, rs);
                         float t[3];
                         affine_apply(r0, r1, r2, pc, t);
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
-                            const float wt = w[s] * t[a];
+                            const float wt = ws * t[a];
                             acc[a] = acc[a] + wt;
                         }
                     }
                 }
+            }
             }
             // collision target (src/TSDFIntegration.cu:272-282)
             int32_t u[3];
