@@ -39,6 +39,9 @@ namespace vf {
 namespace {
 
 constexpr int kCap = 128;       // node slots per brick list (and chunk size when a list spills)
+constexpr int kSlotBits = 7;    // list slot bits of a knn id (node << kSlotBits | slot)
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+static_assert(kCap == 1 << kSlotBits, "a list slot fits the id's low bits");
 constexpr int kMargin = 8;      // collision window reaches 8 voxels beyond the brick
 constexpr int kStatSlots = 64;  // statistics counters are spread over 64 addresses
 constexpr uint32_t kNone = 0xffffffffu;
@@ -320,8 +323,6 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
     constexpr int kBX = S::BX, kBY = S::BY, kBZ = S::BZ;
     constexpr int kWX = S::WX, kWY = S::WY, kWZ = S::WZ;
     constexpr int kWinTiles = S::kWinTiles;
-    __shared__ uint32_t s_raw[kCap];
-    __shared__ uint32_t s_ids[kCap];
     __shared__ float4 s_m[kCap + 1];
     __shared__ int4 s_lo[kCap + 1];
     __shared__ float4 s_t0[kCap];
@@ -438,19 +439,19 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
         ks.init(s_keys, tid);
 
         if (!spill) {
-            // this brick's list, sorted by node index, with its node records
-            if ((uint32_t)tid < cnt) s_raw[tid] = pre_id;
-            __syncthreads();
+            // This brick's list with its node records, in list order (the
+            // order the binning appended them).  Each entry carries the knn
+            // id (node << 7) | slot: ids order as node indices do, so ties
+            // in distance fall to the smaller node index, and the low bits
+            // locate the entry's records here.
             if ((uint32_t)tid < cnt) {
-                uint32_t rank = 0;
-                for (uint32_t e = 0; e < cnt; ++e) rank += s_raw[e] < pre_id ? 1u : 0u;
-                s_ids[rank] = pre_id;
-                s_m[rank] = pm;
-                s_lo[rank] = make_int4(__float_as_int(plo.x), __float_as_int(plo.y), __float_as_int(plo.z), 0);
-                s_t0[rank] = pt0;
-                s_t1[rank] = pt1;
-                s_t2[rank] = pt2;
-                s_w[rank] = make_float2(pw.x, pw.y);
+                s_m[tid] = pm;
+                s_lo[tid] = make_int4(__float_as_int(plo.x), __float_as_int(plo.y), __float_as_int(plo.z),
+                                      (int)((pre_id << kSlotBits) | (uint32_t)tid));
+                s_t0[tid] = pt0;
+                s_t1[tid] = pt1;
+                s_t2[tid] = pt2;
+                s_w[tid] = make_float2(pw.x, pw.y);
             }
             __syncthreads();
             // Branch-free test of every list entry, the next entry's LDS
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 const float dx = p[0] - mc.x, dy = p[1] - mc.y, dz = p[2] - mc.z;
                 const float s = dx * dx + (dy * dy + dz * dz);
                 const float d = sqrt_rn(s);
-                if (inside & !(d > mc.w)) ks.offer(d, e);
+                if (inside & !(d > mc.w)) ks.offer(d, (uint32_t)lc.w);
             }
         } else {
             // the list spilled: walk every node of the volume in chunks
@@ -480,7 +481,6 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                 __syncthreads();
                 if ((uint32_t)tid < m) {
                     const NodeRec rec = nrec[vd.node_base + base + tid];
-                    s_ids[tid] = base + tid;
                     s_m[tid] = rec.m;
                     s_lo[tid] = rec.lo;
                 }
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
 #pragma unroll
                 for (int s = 0; s < K; ++s) {
                     if (s < n) {
-                        kout.idx[(uint64_t)K * vidx + s] = spill ? kl.i[s] : s_ids[kl.i[s]];
+                        kout.idx[(uint64_t)K * vidx + s] = spill ? kl.i[s] : kl.i[s] >> kSlotBits;
                         kout.dist[(uint64_t)K * vidx + s] = kl.d[s];
                     }
                 }
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                         const float4 nw = nrec[vd.node_base + kl.i[s]].w;
                         dw = make_float2(nw.x, nw.y);
                     } else {
-                        dw = s_w[kl.i[s]];
+                        dw = s_w[kl.i[s] & kSlotMask];
                     }
                     w[s] = warp_weight_rcp(dw.x, dw.y, kl.d[s], s_exp2);
                     sum = sum + w[s];
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
             if (vd.blend == VF_BLEND_DQB) {
                 // dual-quaternion blend of the selected nodes' rigid parts
                 float4 br = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bd = br;
-                const float4 f = spill ? nrec[vd.node_base + kl.i[0]].t0 : s_t0[kl.i[0]];
+                const float4 f = spill ? nrec[vd.node_base + kl.i[0]].t0 : s_t0[kl.i[0] & kSlotMask];
 #pragma unroll
                 for (int s = 0; s < K; ++s) {
                     if (s < n) {
@@ -603,8 +603,8 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                             qr = rec.t0;
                             qd = rec.t1;
                         } else {
-                            qr = s_t0[kl.i[s]];
-                            qd = s_t1[kl.i[s]];
+                            qr = s_t0[kl.i[s] & kSlotMask];
+                            qd = s_t1[kl.i[s] & kSlotMask];
                         }
                         dqb_accumulate(qr, qd, f, div_rcp(w[s], sum, rs), &br, &bd);
                     }
@@ -621,9 +621,9 @@ __global__ __launch_bounds__(S::kThreads) void k_pass_a(
                             r1 = rec.t1;
                             r2 = rec.t2;
                         } else {
-                            r0 = s_t0[kl.i[s]];
-                            r1 = s_t1[kl.i[s]];
-                            r2 = s_t2[kl.i[s]];
+                            r0 = s_t0[kl.i[s] & kSlotMask];
+                            r1 = s_t1[kl.i[s] & kSlotMask];
+                            r2 = s_t2[kl.i[s] & kSlotMask];
                         }
                         const float ws = div_rcp(w[s], sum, rs);
                         float t[3];
