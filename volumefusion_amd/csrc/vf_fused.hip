@@ -33,6 +33,8 @@
 #include <mutex>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "vf_host.hpp"
 
 namespace vf {
@@ -868,6 +870,8 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
 }  // namespace
 
 // Device buffers of the brick machinery, grown on demand.
+constexpr int kDescSlots = 4;  // resident descriptor sets per workspace
+
 struct Buffers {
     Ctrl* ctrl = nullptr;
     NodeRec* nrec = nullptr;
@@ -895,8 +899,19 @@ struct vf_workspace {
     uint64_t dirty_tiles = 0;  // collision tiles [0, dirty_tiles) may hold flags from the last frame
     std::vector<vf::VolDesc> vols_h;
     std::vector<vf::VolPtr> vptr_h;
-    // optional per-kernel timing: 4 events per frame (before k_prep_bin, after
-    // the prep-and-list kernels, pass A and pass B)
+    // Device copies of recent descriptor sets (buf.vols / buf.vptr hold
+    // kDescSlots sets of vol_cap entries): a frame whose descriptors equal a
+    // resident set launches on it without an upload.  desc_n[s] = volumes in
+    // set s (0: empty), desc_use[s] = last frame that used it.
+    std::vector<vf::VolDesc> desc_vols[vf::kDescSlots];
+    std::vector<vf::VolPtr> desc_vptr[vf::kDescSlots];
+    int desc_n[vf::kDescSlots] = {};
+    uint64_t desc_use[vf::kDescSlots] = {};
+    uint64_t desc_clock = 0;
+    // optional per-kernel timing: 6 events per frame, carried by the kernel
+    // launches themselves (start of k_prep_bin, end of k_compact, start and
+    // end of pass A, start and end of pass B), so no marker is queued
+    // between kernels
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
@@ -909,6 +924,7 @@ namespace vf {
 namespace {
 
 constexpr int kMaxTimed = 4096;
+constexpr int kTimedEvents = 6;  // per timed frame
 
 void free_buffers(Buffers& b) {
     void* ptrs[] = {b.ctrl,    b.nrec,         b.brick_count, b.brick_nodes, b.active, b.bitmap, b.tile_flag,
@@ -979,8 +995,8 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
         st = dev_alloc(&nbuf.totals, (uint64_t)kStCount * kStatSlots, &bytes);
         if (!st) VF_HIP(hipMemsetAsync(nbuf.totals, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
     }
-    if (!st) st = dev_alloc(&nbuf.vols, nbuf.vol_cap, &bytes);
-    if (!st) st = dev_alloc(&nbuf.vptr, nbuf.vol_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.vols, nbuf.vol_cap * kDescSlots, &bytes);
+    if (!st) st = dev_alloc(&nbuf.vptr, nbuf.vol_cap * kDescSlots, &bytes);
     if (st) {
         free_buffers(nbuf);
         return st;
@@ -998,6 +1014,7 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     b = nbuf;
     ws->frame = 0;
     ws->dirty_tiles = 0;
+    for (int s = 0; s < kDescSlots; ++s) ws->desc_n[s] = 0;
     return VF_OK;
 }
 
@@ -1069,7 +1086,19 @@ struct FrameSetup {
     uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
     uint64_t tiles = 0;        // collision tiles of this batch
     uint64_t clear_tiles = 0;  // tiles whose flags prep clears: every tile a previous frame or a replay may flag
+    const VolDesc* vols = nullptr;  // the frame's device descriptor set
+    const VolPtr* vptr = nullptr;
 };
+
+// A launch that carries timing events (recorded at the kernel's start and
+// end, no separate marker in the queue) when either is given.
+template <typename F, typename... Args>
+void launch(F kernel, dim3 grid, dim3 block, hipStream_t stream, hipEvent_t start, hipEvent_t stop, Args... args) {
+    if (start || stop)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, start, stop, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+}
 
 template <class S>
 const void* pass_a_kernel(int k, bool knn_only) {
@@ -1132,6 +1161,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     for (int v = 0; v < nvol; ++v) {
         const vf_frame empty_frame = {nullptr, 0, 0, {{0}}};
         VolDesc& vd = ws->vols_h[v];
+        memset(&vd, 0, sizeof(vd));  // padding too: descriptor sets are compared bytewise
         uint64_t n;
         st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, shape, &vd, &n);
         if (st) return st;
@@ -1188,8 +1218,30 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     st = ensure(ws, need, stream);
     if (st) return st;
     Buffers& B = ws->buf;
-    VF_HIP(hipMemcpyAsync(B.vols, ws->vols_h.data(), sizeof(VolDesc) * nvol, hipMemcpyHostToDevice, stream));
-    VF_HIP(hipMemcpyAsync(B.vptr, ws->vptr_h.data(), sizeof(VolPtr) * nvol, hipMemcpyHostToDevice, stream));
+    // the descriptor set: a resident copy when one matches, else upload into
+    // the least recently used slot (stream-ordered after the frames that
+    // read it)
+    int slot = -1;
+    for (int s = 0; s < kDescSlots && slot < 0; ++s)
+        if (ws->desc_n[s] == nvol && !memcmp(ws->desc_vols[s].data(), ws->vols_h.data(), sizeof(VolDesc) * nvol) &&
+            !memcmp(ws->desc_vptr[s].data(), ws->vptr_h.data(), sizeof(VolPtr) * nvol))
+            slot = s;
+    if (slot < 0) {
+        slot = 0;
+        for (int s = 1; s < kDescSlots; ++s)
+            if (ws->desc_use[s] < ws->desc_use[slot]) slot = s;
+        ws->desc_n[slot] = 0;
+        VF_HIP(hipMemcpyAsync(B.vols + (uint64_t)slot * B.vol_cap, ws->vols_h.data(), sizeof(VolDesc) * nvol,
+                              hipMemcpyHostToDevice, stream));
+        VF_HIP(hipMemcpyAsync(B.vptr + (uint64_t)slot * B.vol_cap, ws->vptr_h.data(), sizeof(VolPtr) * nvol,
+                              hipMemcpyHostToDevice, stream));
+        ws->desc_vols[slot].assign(ws->vols_h.begin(), ws->vols_h.begin() + nvol);
+        ws->desc_vptr[slot].assign(ws->vptr_h.begin(), ws->vptr_h.begin() + nvol);
+        ws->desc_n[slot] = nvol;
+    }
+    ws->desc_use[slot] = ++ws->desc_clock;
+    fs->vols = B.vols + (uint64_t)slot * B.vol_cap;
+    fs->vptr = B.vptr + (uint64_t)slot * B.vol_cap;
 
     fs->nvol = nvol;
     fs->k = p->k;
@@ -1210,15 +1262,18 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
 template <class S>
 int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOut kout, hipEvent_t* ev,
                          hipStream_t stream) {
-    hipLaunchKernelGGL(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), 0, stream, B.vols,
-                       fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,
-                       (uint32_t)fs.clear_tiles, B.brick_cap, parity_arg, B.stat, B.totals);
+    hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (ev)
+        for (int i = 0; i < 6; ++i) e[i] = ev[i];
+    const VolDesc* const vols = fs.vols;
+    launch(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), stream, e[0], nullptr,
+           vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap,
+           B.tile_flag, (uint32_t)fs.clear_tiles, B.brick_cap, parity_arg, B.stat, B.totals);
     VF_LAUNCH_CHECK("k_prep_bin");
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads - 1) / kPrepThreads)), dim3(kPrepThreads),
-                       0, stream, B.vols, fs.nvol, (uint32_t)fs.bricks, B.brick_count, B.active, B.ctrl, B.brick_cap,
-                       parity_arg);
+    launch(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads - 1) / kPrepThreads)), dim3(kPrepThreads), stream,
+           nullptr, e[1], vols, fs.nvol, (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.active,
+           B.ctrl, B.brick_cap, parity_arg);
     VF_LAUNCH_CHECK("k_compact");
-    if (ev) VF_HIP(hipEventRecord(ev[1], stream));
     Arena arena;
     arena.records = B.records;
     arena.counts = B.arena_counts;
@@ -1226,15 +1281,17 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     arena.n_arenas = (uint32_t)fs.blocks_a;
     arena.overflow = B.records + fs.blocks_a * fs.arena_cap;
     arena.overflow_cap = (uint32_t)fs.overflow_cap;
-#define VF_PASS_A(KK)                                                                                                \
-    if (fs.knn_only)                                                                                                 \
-        hipLaunchKernelGGL((k_pass_a<KK, true, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,       \
-                           B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
-                           B.brick_cap, parity_arg, arena, B.stat, B.totals, kout);                                  \
-    else                                                                                                             \
-        hipLaunchKernelGGL((k_pass_a<KK, false, S>), dim3((unsigned)fs.blocks_a), dim3(S::kThreads), 0, stream,      \
-                           B.vols, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap, B.tile_flag,    \
-                           B.brick_cap, parity_arg, arena, B.stat, B.totals, kout)
+#define VF_PASS_A(KK)                                                                                                 \
+    if (fs.knn_only)                                                                                                  \
+        launch(k_pass_a<KK, true, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],             \
+               vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes,          \
+               (const uint32_t*)B.active, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat,      \
+               B.totals, kout);                                                                                       \
+    else                                                                                                              \
+        launch(k_pass_a<KK, false, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],            \
+               vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes,          \
+               (const uint32_t*)B.active, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat,      \
+               B.totals, kout)
     switch (fs.k) {
         case 1: VF_PASS_A(1); break;
         case 2: VF_PASS_A(2); break;
@@ -1247,19 +1304,18 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     }
 #undef VF_PASS_A
     VF_LAUNCH_CHECK("k_pass_a");
-    if (ev) VF_HIP(hipEventRecord(ev[2], stream));
     if (!fs.knn_only) {
-        hipLaunchKernelGGL(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), 0, stream, B.vptr, fs.nvol, arena, B.ctrl,
-                           B.bitmap, B.vols, B.nrec, B.stat, B.totals);
+        launch(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), stream, e[4], e[5], fs.vptr, fs.nvol, arena, B.ctrl,
+               (const uint32_t*)B.bitmap, vols, (const NodeRec*)B.nrec, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_pass_b");
-        if (ev) VF_HIP(hipEventRecord(ev[3], stream));
     }
     return VF_OK;
 }
 
-// The frame's kernels on `stream`: k_prep_bin, k_pass_a, k_pass_b.  With
-// parity_arg < 0 the kernels take the frame parity from the device counter
-// (what a recorded graph needs); ev (4 events) brackets the kernels.
+// The frame's kernels on `stream`: k_prep_bin, k_compact, k_pass_a,
+// k_pass_b.  With parity_arg < 0 the kernels take the frame parity from the
+// device counter (what a recorded graph needs); ev (6 events, or NULL) times
+// the prep-and-list kernels, pass A and pass B.
 int launch_kernels(vf_workspace* ws, const FrameSetup& fs, int parity_arg, KnnOut kout, hipEvent_t* ev,
                    hipStream_t stream) {
     Buffers& B = ws->buf;
@@ -1280,13 +1336,12 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
     hipEvent_t* ev = nullptr;
     if (ws->timing && !knn_only) {
         if (ws->events.empty()) {
-            ws->events.resize(4 * kMaxTimed);
+            ws->events.resize(kTimedEvents * kMaxTimed);
             for (auto& e : ws->events) VF_HIP(hipEventCreate(&e));
         }
         if (ws->timed_frames >= kMaxTimed) return fail(VF_ERR_CAPACITY, "more than %d timed frames", kMaxTimed);
-        ev = &ws->events[4 * ws->timed_frames];
+        ev = &ws->events[kTimedEvents * ws->timed_frames];
         ws->timed_frames++;
-        VF_HIP(hipEventRecord(ev[0], stream));
     }
     st = launch_kernels(ws, fs, (int)(ws->frame & 1u), kout, ev, stream);
     if (st) return st;
@@ -1608,11 +1663,12 @@ int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
     if (!ws || !ms3 || !frames) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_kernel_ms: NULL argument");
     ms3[0] = ms3[1] = ms3[2] = 0.0f;
     *frames = ws->timed_frames;
-    if (ws->timed_frames) VF_HIP(hipEventSynchronize(ws->events[4 * ws->timed_frames - 1]));
+    if (ws->timed_frames) VF_HIP(hipEventSynchronize(ws->events[kTimedEvents * ws->timed_frames - 1]));
     for (int f = 0; f < ws->timed_frames; ++f) {
         for (int k = 0; k < 3; ++k) {
             float t = 0.0f;
-            VF_HIP(hipEventElapsedTime(&t, ws->events[4 * f + k], ws->events[4 * f + k + 1]));
+            const hipEvent_t* e = &ws->events[kTimedEvents * f + 2 * k];
+            VF_HIP(hipEventElapsedTime(&t, e[0], e[1]));
             ms3[k] += t;
         }
     }
