@@ -345,11 +345,13 @@ int vf_workspace_last_stats(vf_workspace* ws, vf_frame_stats* stats, vf_stream s
  * `stream`); reset != 0 zeroes them afterwards. */
 int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_stream stream);
 
-/* Per-kernel timing: when enabled, each call records HIP events on its stream
- * around its kernels.  vf_workspace_kernel_ms waits for the last one,
- * returns the summed milliseconds of k_prep_bin with k_compact, k_pass_a and
- * k_pass_b over the timed calls since enabling (or the previous query) and
- * starts over. */
+/* Per-kernel timing: when enabled, each call's kernel launches carry HIP
+ * events (recorded at the kernels' start and end on the call's stream).
+ * enable 1 times k_prep_bin with k_compact, k_pass_a and k_pass_b; enable 2
+ * times k_pass_a alone (fewer events between the kernels); 0 turns timing
+ * off.  vf_workspace_kernel_ms waits for the last event, returns the summed
+ * milliseconds of the three (0 for those not timed) over the timed calls
+ * since enabling (or the previous query) and starts over. */
 int vf_workspace_set_timing(vf_workspace* ws, int enable);
 int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames);
 

@@ -950,8 +950,8 @@ struct vf_workspace {
     // optional per-kernel timing: 6 events per frame, carried by the kernel
     // launches themselves (start of k_prep_bin, end of k_compact, start and
     // end of pass A, start and end of pass B), so no marker is queued
-    // between kernels
-    bool timing = false;
+    // between kernels; timing 1: all three, 2: pass A only
+    int timing = 0;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
     vf_tuning tuning = {0, 0, 0.0f};
@@ -1139,6 +1139,7 @@ struct FrameSetup {
     uint64_t clear_tiles = 0;  // tiles whose flags prep clears: every tile a previous frame or a replay may flag
     const VolDesc* vols = nullptr;  // the frame's device descriptor set
     const VolPtr* vptr = nullptr;
+    int timing = 0;                 // events on all three kernel groups (1) or on pass A only (2)
 };
 
 // A launch that carries timing events (recorded at the kernel's start and
@@ -1315,7 +1316,7 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
                          hipStream_t stream) {
     hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (ev)
-        for (int i = 0; i < 6; ++i) e[i] = ev[i];
+        for (int i = 0; i < 6; ++i) e[i] = (fs.timing == 2 && (i < 2 || i > 3)) ? nullptr : ev[i];
     const VolDesc* const vols = fs.vols;
     launch(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), stream, e[0], nullptr,
            vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap,
@@ -1379,6 +1380,17 @@ int launch_kernels(vf_workspace* ws, const FrameSetup& fs, int parity_arg, KnnOu
     }
 }
 
+// At least `frames` timed frames' events exist (created in doubling steps).
+int grow_events(vf_workspace* ws, int frames) {
+    const size_t want = (size_t)kTimedEvents * (size_t)frames;
+    if (ws->events.size() >= want) return VF_OK;
+    const size_t have = ws->events.size();
+    const size_t n = std::min<size_t>(std::max(want, 2 * have), (size_t)kTimedEvents * kMaxTimed);
+    ws->events.resize(n, nullptr);
+    for (size_t i = have; i < n; ++i) VF_HIP(hipEventCreate(&ws->events[i]));
+    return VF_OK;
+}
+
 int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_graph* graphs, const vf_frame* frames,
                  const vf_warp_params* p, bool knn_only, KnnOut kout, hipStream_t stream) {
     FrameSetup fs;
@@ -1386,14 +1398,13 @@ int launch_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_gra
     if (st) return st;
     hipEvent_t* ev = nullptr;
     if (ws->timing && !knn_only) {
-        if (ws->events.empty()) {
-            ws->events.resize(kTimedEvents * kMaxTimed);
-            for (auto& e : ws->events) VF_HIP(hipEventCreate(&e));
-        }
         if (ws->timed_frames >= kMaxTimed) return fail(VF_ERR_CAPACITY, "more than %d timed frames", kMaxTimed);
+        st = grow_events(ws, ws->timed_frames + 1);
+        if (st) return st;
         ev = &ws->events[kTimedEvents * ws->timed_frames];
         ws->timed_frames++;
     }
+    fs.timing = ws->timing;
     st = launch_kernels(ws, fs, (int)(ws->frame & 1u), kout, ev, stream);
     if (st) return st;
     ws->frame++;
@@ -1705,9 +1716,12 @@ int vf_workspace_totals(vf_workspace* ws, vf_frame_stats* totals, int reset, vf_
 
 int vf_workspace_set_timing(vf_workspace* ws, int enable) {
     if (!ws) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_set_timing: NULL workspace");
-    ws->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(VF_ERR_INVALID_ARGUMENT, "timing mode %d: supported 0, 1, 2", enable);
+    ws->timing = enable;
     ws->timed_frames = 0;
-    return VF_OK;
+    // events for the first frames now, so that creating them does not fall
+    // between the timed frames' launches
+    return ws->timing ? grow_events(ws, 256) : VF_OK;
 }
 
 int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
@@ -1717,6 +1731,7 @@ int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames) {
     if (ws->timed_frames) VF_HIP(hipEventSynchronize(ws->events[kTimedEvents * ws->timed_frames - 1]));
     for (int f = 0; f < ws->timed_frames; ++f) {
         for (int k = 0; k < 3; ++k) {
+            if (ws->timing == 2 && k != 1) continue;
             float t = 0.0f;
             const hipEvent_t* e = &ws->events[kTimedEvents * f + 2 * k];
             VF_HIP(hipEventElapsedTime(&t, e[0], e[1]));

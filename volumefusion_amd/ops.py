@@ -366,8 +366,9 @@ class Workspace:
         L.call("vf_workspace_totals", self.handle, C.byref(st), 1 if reset else 0, _s(stream))
         return st.as_dict()
 
-    def set_timing(self, enable: bool) -> None:
-        L.call("vf_workspace_set_timing", self.handle, 1 if enable else 0)
+    def set_timing(self, enable, pass_a_only: bool = False) -> None:
+        """Kernel timing on (all three kernel groups, or pass A alone) or off."""
+        L.call("vf_workspace_set_timing", self.handle, (2 if pass_a_only else 1) if enable else 0)
 
     def kernel_ms(self) -> tuple:
         """(k_prep_bin, k_pass_a, k_pass_b) summed milliseconds, frames timed."""
