@@ -391,9 +391,15 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     }
     const uint32_t nactive = ctrl->active_count[parity];
 
-    const int lx = tid % kBX;
-    const int ly = (tid / kBX) % kBY;
-    const int lz = tid / (kBX * kBY);
+    // Each wave takes a 4x4x4 block of the brick: neighbour counts vary
+    // less inside a compact block than along a 64-voxel slab, and a wave's
+    // per-neighbour work is set by its largest count.
+    static_assert(kBX % 4 == 0 && kBY % 4 == 0 && kBZ % 4 == 0, "bricks are tiled by 4x4x4 wave blocks");
+    constexpr int kWavesX = kBX / 4, kWavesY = kBY / 4;
+    const int lane_in = tid & 63, wave_in = tid >> 6;
+    const int lx = (wave_in % kWavesX) * 4 + (lane_in & 3);
+    const int ly = ((wave_in / kWavesX) % kWavesY) * 4 + ((lane_in >> 2) & 3);
+    const int lz = (wave_in / (kWavesX * kWavesY)) * 4 + (lane_in >> 4);
     static_assert(kBX <= 32 && kBY <= 32 && kBZ <= 32, "brick-local coordinates fit the byte-parallel cube test");
     const uint32_t lpk = (uint32_t)lx | ((uint32_t)ly << 8) | ((uint32_t)lz << 16);
     uint32_t fill = 0;  // records in this workgroup's arena (uniform)
