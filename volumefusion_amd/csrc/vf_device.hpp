@@ -155,7 +155,23 @@ __device__ __forceinline__ float warp_weight_rcp(float den, float rden, float d,
     return expw_dom(-div_rcp(q, den, rden), tab);
 }
 
-// Batcher odd-even merge sort networks on 64-bit keys for 1, 2, 4, 8 keys.
+// Inclusive prefix sum over the wave in DPP moves (VALU only): shifts by 1,
+// 2, 4, 8 within each row of 16 lanes, then row 15's and row 31's lane added
+// to the rows above them.
+#define VF_DPP(x, ctrl, rows) (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(x), ctrl, rows, 0xf, false)
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += VF_DPP(x, 0x111, 0xf);  // row_shr:1
+    x += VF_DPP(x, 0x112, 0xf);  // row_shr:2
+    x += VF_DPP(x, 0x114, 0xf);  // row_shr:4
+    x += VF_DPP(x, 0x118, 0xf);  // row_shr:8
+    x += VF_DPP(x, 0x142, 0xa);  // row_bcast:15 into rows 1 and 3
+    x += VF_DPP(x, 0x143, 0xc);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+#undef VF_DPP
+
+// Sorting networks on 64-bit keys for 2, 4, 6, 8 keys (Batcher's odd-even
+// merge sort for 2, 4, 8).
 __device__ __forceinline__ void cmpx(unsigned long long& a, unsigned long long& b) {
     const bool sw = b < a;
     const unsigned long long lo = sw ? b : a;
@@ -172,6 +188,13 @@ __device__ __forceinline__ void sort_keys(unsigned long long* k) {
         cmpx(k[0], k[1]); cmpx(k[2], k[3]);
         cmpx(k[0], k[2]); cmpx(k[1], k[3]);
         cmpx(k[1], k[2]);
+    } else if constexpr (N == 6) {
+        // 12 comparators (optimal for six inputs; checked on all 0-1 inputs)
+        cmpx(k[0], k[5]); cmpx(k[1], k[3]); cmpx(k[2], k[4]);
+        cmpx(k[1], k[2]); cmpx(k[3], k[4]);
+        cmpx(k[0], k[3]); cmpx(k[2], k[5]);
+        cmpx(k[0], k[1]); cmpx(k[2], k[3]); cmpx(k[4], k[5]);
+        cmpx(k[1], k[2]); cmpx(k[3], k[4]);
     } else if constexpr (N == 8) {
         cmpx(k[0], k[1]); cmpx(k[2], k[3]); cmpx(k[4], k[5]); cmpx(k[6], k[7]);
         cmpx(k[0], k[2]); cmpx(k[1], k[3]); cmpx(k[4], k[6]); cmpx(k[5], k[7]);
@@ -231,15 +254,17 @@ struct KnnSlots {
     }
 
     // Sorted selection into registers: d[s], i[s] for s < n.  The network is
-    // the smallest power of two that holds the largest n of the wave; slots
-    // at or beyond a lane's n are never read by the caller.  Called by whole
-    // waves (the size test is a ballot).
+    // the smallest of 2, 4, 6 and 8 keys (2 and 4 for K <= 4) that holds the
+    // largest n of the wave; slots at or beyond a lane's n are never read by
+    // the caller.  Called by whole waves (the size test is a ballot).
     __device__ __forceinline__ void finish(float* d, uint32_t* i) const {
         constexpr int P = pow2_at_least(K);
         unsigned long long k[P];
         if constexpr (P == 8) {
-            if (__ballot(n > 4))
+            if (__ballot(n > 6))
                 load_sort<8, P>(k);
+            else if (__ballot(n > 4))
+                load_sort<6, P>(k);
             else if (__ballot(n > 2))
                 load_sort<4, P>(k);
             else

@@ -277,46 +277,65 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
 }
 
 // The frame's active bricks (those with a non-empty node list), in brick
-// order, one thread per brick of the batch; one list append per workgroup.
+// order within each workgroup's run of kCompactPer * kPrepThreads bricks; one
+// list append per workgroup (appends to one counter serialise, so a
+// workgroup takes many bricks).
+constexpr int kCompactPer = 2;
 __global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restrict__ vols, int nvol,
                                                           uint32_t nbricks, const uint32_t* __restrict__ brick_count,
                                                           uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
                                                           uint64_t brick_cap, int parity_arg) {
-    __shared__ uint32_t s_wcnt[kPrepThreads / 64];
+    constexpr int kWaves = kPrepThreads / 64;
+    __shared__ uint32_t s_cnt[kCompactPer * kWaves];  // per (round, wave): count, then first position
     __shared__ uint32_t s_base;
-    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t parity = frame_parity(ctrl, parity_arg);
-    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
-    bool act = false;
-    uint32_t entry = 0;
-    if (gb < nbricks && brick_count[gb] != 0u) {
-        int lo = 0, hi = nvol - 1;  // last volume whose first brick is <= gb
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (vols[mid].brick_base <= gb)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        act = true;
-        entry = ((uint32_t)lo << 24) | (gb - vols[lo].brick_base);
-    }
-    const unsigned long long mact = __ballot(act);
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
-    if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(mact);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t sum = 0;
-        for (int w = 0; w < kPrepThreads / 64; ++w) {
-            const uint32_t c = s_wcnt[w];
-            s_wcnt[w] = sum;
-            sum += c;
-        }
-        s_base = sum ? atomicAdd(&ctrl->active_count[parity], sum) : 0u;
+    const uint32_t parity = frame_parity(ctrl, parity_arg);
+    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
+    const uint64_t gb0 = (uint64_t)blockIdx.x * (kPrepThreads * kCompactPer) + threadIdx.x;
+    uint32_t flags = 0;  // bit r: brick gb0 + r * kPrepThreads is active
+#pragma unroll
+    for (int r = 0; r < kCompactPer; ++r) {
+        const uint64_t gb = gb0 + (uint64_t)(r * kPrepThreads);
+        const bool act = gb < nbricks && brick_count[gb] != 0u;
+        flags |= (uint32_t)act << r;
+        const unsigned long long m = __ballot(act);
+        if (lane == 0) s_cnt[r * kWaves + wave] = (uint32_t)__popcll(m);
     }
     __syncthreads();
-    if (act) active[s_base + s_wcnt[wave] + (uint32_t)__popcll(mact & ((1ull << lane) - 1ull))] = entry;
+    if (wave == 0) {
+        // exclusive prefix over (round, wave), round-major: two entries a lane
+        static_assert(kCompactPer * kWaves <= 128, "two table entries per lane of one wave");
+        const int e0 = 2 * lane;
+        const uint32_t c0 = e0 < kCompactPer * kWaves ? s_cnt[e0] : 0u;
+        const uint32_t c1 = e0 + 1 < kCompactPer * kWaves ? s_cnt[e0 + 1] : 0u;
+        const uint32_t incl = wave_inclusive_sum(c0 + c1);
+        const uint32_t excl = incl - (c0 + c1);
+        if (e0 < kCompactPer * kWaves) s_cnt[e0] = excl;
+        if (e0 + 1 < kCompactPer * kWaves) s_cnt[e0 + 1] = excl + c0;
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (lane == 0) s_base = total ? atomicAdd(&ctrl->active_count[parity], total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = s_base;
+#pragma unroll
+    for (int r = 0; r < kCompactPer; ++r) {
+        const bool act = (flags >> r) & 1u;
+        const unsigned long long m = __ballot(act);
+        if (act) {
+            const uint32_t gb = (uint32_t)(gb0 + (uint64_t)(r * kPrepThreads));
+            int lo = 0, hi = nvol - 1;  // last volume whose first brick is <= gb
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (vols[mid].brick_base <= gb)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            active[base + s_cnt[r * kWaves + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
+                ((uint32_t)lo << 24) | (gb - vols[lo].brick_base);
+        }
+    }
 }
 
 // A node cube as the ranges [A, B] of brick-local coordinates it covers per
@@ -1328,7 +1347,8 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
            vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap,
            B.tile_flag, (uint32_t)fs.clear_tiles, B.brick_cap, parity_arg, B.stat, B.totals);
     VF_LAUNCH_CHECK("k_prep_bin");
-    launch(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads - 1) / kPrepThreads)), dim3(kPrepThreads), stream,
+    launch(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads * kCompactPer - 1) / (kPrepThreads * kCompactPer))),
+           dim3(kPrepThreads), stream,
            nullptr, e[1], vols, fs.nvol, (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.active,
            B.ctrl, B.brick_cap, parity_arg);
     VF_LAUNCH_CHECK("k_compact");

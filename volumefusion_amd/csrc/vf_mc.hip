@@ -261,21 +261,6 @@ __device__ __forceinline__ void make_cube(const McGeom& g, int32_t x, int32_t j,
     cb->cs = zero ? 0 : cs;
 }
 
-// Inclusive prefix sum over the wave in DPP moves (VALU only): shifts by 1,
-// 2, 4, 8 within each row of 16 lanes, then row 15's and row 31's lane added
-// to the rows above them.
-#define VF_DPP(x, ctrl, rows) (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(x), ctrl, rows, 0xf, false)
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
-    x += VF_DPP(x, 0x111, 0xf);  // row_shr:1
-    x += VF_DPP(x, 0x112, 0xf);  // row_shr:2
-    x += VF_DPP(x, 0x114, 0xf);  // row_shr:4
-    x += VF_DPP(x, 0x118, 0xf);  // row_shr:8
-    x += VF_DPP(x, 0x142, 0xa);  // row_bcast:15 into rows 1 and 3
-    x += VF_DPP(x, 0x143, 0xc);  // row_bcast:31 into rows 2 and 3
-    return x;
-}
-#undef VF_DPP
-
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_sum(v), 63);
 }
