@@ -870,7 +870,7 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
     unsigned long long collided = 0, updated = 0;
     // this workgroup's share of the per-workgroup arenas, kUnrollB records per
     // thread in flight (their loads issued before any is applied)
-    constexpr uint32_t kUnrollB = 4;
+    constexpr uint32_t kUnrollB = 8;
     for (uint32_t a = blockIdx.x; a < arena.n_arenas; a += gridDim.x) {
         const uint32_t n = arena.counts[a];
         const uint4* recs = arena.records + (uint64_t)a * arena.cap;
