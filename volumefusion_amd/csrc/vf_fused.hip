@@ -162,6 +162,7 @@ __device__ __forceinline__ void stat_add(unsigned long long* stat, unsigned long
 }
 
 constexpr int kPrepThreads = 1024;
+constexpr int kMaxBatch = 256;  // volumes per call
 
 template <class S>
 __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
@@ -170,10 +171,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag,
     uint32_t clear_tiles, uint64_t brick_cap, int parity_arg,
     unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals) {
+    __shared__ uint32_t s_k1[kMaxBatch];  // first (node, brick) item of each volume
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
+    for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_k1[v] = vols[v].k1_base;
 
     // Clear the collision tiles the previous frame on this workspace flagged
     // (tiles [0, clear_tiles) hold every flag it can have set).
@@ -195,8 +198,14 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     uint32_t entry = 0;
     do {
         if (gid >= total) break;
-        int v = 0;
-        while (v + 1 < nvol && vols[v + 1].k1_base <= gid) ++v;
+        int v = 0, hi = nvol - 1;  // last volume whose first item is <= gid
+        while (v < hi) {
+            const int mid = (v + hi + 1) >> 1;
+            if (s_k1[mid] <= gid)
+                v = mid;
+            else
+                hi = mid - 1;
+        }
         const VolDesc& vd = vols[v];
         const uint32_t local = gid - vd.k1_base;
         if (local >= vd.k1_count) break;
@@ -288,6 +297,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restr
     constexpr int kWaves = kPrepThreads / 64;
     __shared__ uint32_t s_cnt[kCompactPer * kWaves];  // per (round, wave): count, then first position
     __shared__ uint32_t s_base;
+    __shared__ uint32_t s_bb[kMaxBatch];  // first brick of each volume
+    for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_bb[v] = vols[v].brick_base;
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
@@ -327,13 +338,13 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restr
             int lo = 0, hi = nvol - 1;  // last volume whose first brick is <= gb
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (vols[mid].brick_base <= gb)
+                if (s_bb[mid] <= gb)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
             active[base + s_cnt[r * kWaves + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
-                ((uint32_t)lo << 24) | (gb - vols[lo].brick_base);
+                ((uint32_t)lo << 24) | (gb - s_bb[lo]);
         }
     }
 }
@@ -861,7 +872,7 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
                                                 const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec,
                                                 unsigned long long* __restrict__ stat,
                                                 unsigned long long* __restrict__ totals) {
-    __shared__ VolPtr s_vp[256];
+    __shared__ VolPtr s_vp[kMaxBatch];
     __shared__ unsigned long long s_cnt[2];
     const int tid = threadIdx.x;
     for (int v = tid; v < nvol; v += blockDim.x) s_vp[v] = vptr[v];
@@ -1227,7 +1238,8 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     if (st) return st;
     st = check_tuning(tuning);
     if (st) return st;
-    if (nvol < 1 || nvol > 256) return fail(VF_ERR_INVALID_ARGUMENT, "batch of %d volumes: supported 1..256", nvol);
+    if (nvol < 1 || nvol > kMaxBatch)
+        return fail(VF_ERR_INVALID_ARGUMENT, "batch of %d volumes: supported 1..%d", nvol, kMaxBatch);
     for (int v = 1; v < nvol; ++v)
         if (p[v].k != p[0].k) return fail(VF_ERR_INVALID_ARGUMENT, "batched volumes must share k");
     const int shape = knn_only ? 0 : tuning.brick_shape;
