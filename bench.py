@@ -3,13 +3,13 @@
 
 A step is one frame of warped integration (the reference's per-frame block,
 src/VolumeFusion.cu:702-742) of the workload's volume, on inputs already
-resident in HBM: k_prep_bin, k_pass_a, k_pass_b on one stream.  Prints one
-JSON line:
+resident in HBM: k_prep_bin, k_compact, k_pass_a, k_pass_b on one stream.
+Prints one JSON line:
 
   value      warped voxel updates per second (voxels whose tsdf/weight changed)
   roofline   dominant kernel (k_pass_a): algorithmic bytes per launch / its
-             average duration from HIP events recorded on its stream, against
-             the 8 TB/s HBM peak.  Algorithmic bytes per frame (whole path):
+             average duration from HIP events carried by its launches in the
+             timed frames (on its stream), against the 8 TB/s HBM peak.  Algorithmic bytes per frame (whole path):
              16 U + 4 W H + 80 G (tsdf+weight read and write of updated voxels,
              one depth read, node position+radius+transform); k_pass_a's share
              is 8 U + 4 W H + 80 G, k_pass_b's is 8 U.
