@@ -44,6 +44,19 @@ __device__ __forceinline__ float div_by_recip64(float a, float b, double rb) {
     return q;
 }
 
+// 1 / x with the bits of the IEEE division 1.0f / x: v_rcp_f32 (within one
+// ulp) and one Newton step in fused multiply-adds, y0 + y0 (1 - x y0).  The
+// step gives the correctly rounded reciprocal for every x in [2^-126, 2^126)
+// (tests/cpp/numerics_check.hip runs all of them on the device); other x take
+// the division itself.
+__device__ __forceinline__ float rcp_rn(float x) {
+    if (__builtin_expect(x >= 0x1p-126f && x < 0x1p126f, 1)) {
+        const float y0 = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, y0, 1.0f), y0, y0);
+    }
+    return 1.0f / x;
+}
+
 // Correctly rounded square root, the same bits as __builtin_sqrtf.  For
 // s >= 2^-96 the compiler's expansion of __builtin_sqrtf on gfx950 is
 // v_sqrt_f32 (within 1 ulp) followed by a one-ulp correction: with
@@ -476,7 +489,7 @@ struct Camera {
 __device__ __forceinline__ int64_t project_sample(const Camera& cam, const float* __restrict__ depth, const float p[3],
                                                   float* tn) {
     if (p[2] < 1.e-4f) return -1;
-    const float zInv = 1.0f / p[2];
+    const float zInv = rcp_rn(p[2]);
     const float ux = cam.fx * p[0];
     const float uf = ux * zInv + cam.cx;
     const float vy = cam.fy * p[1];

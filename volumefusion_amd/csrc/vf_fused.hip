@@ -666,7 +666,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 acc[1] = pc[1];
                 acc[2] = pc[2];
             } else {
-            const float rs = 1.0f / sum;
+            const float rs = rcp_rn(sum);
             if (vd.blend == VF_BLEND_DQB) {
                 // dual-quaternion blend of the selected nodes' rigid parts
                 float4 br = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bd = br;
