@@ -322,7 +322,10 @@ typedef struct vf_frame_stats {
 
 /* Workspace for up to `max_volumes` volumes per call, each with at most
  * max_voxels voxels and max_nodes nodes (total over the batch).  Allocates
- * device memory on the current device. */
+ * device memory on the current device.  Calls on one workspace must be
+ * ordered (one stream, or streams synchronised between calls): its buffers,
+ * including the last few frames' descriptor sets kept on the device so that
+ * a repeated call uploads nothing, are reused from call to call. */
 int vf_workspace_create(vf_workspace** ws, int max_volumes, uint64_t max_total_voxels, uint64_t max_total_nodes,
                         vf_stream stream);
 int vf_workspace_destroy(vf_workspace* ws);
@@ -333,7 +336,7 @@ uint64_t vf_workspace_bytes(const vf_workspace* ws);
 int vf_warped_integrate(vf_workspace* ws, const vf_volume* volume, const vf_graph* graph, const vf_frame* frame,
                         const vf_warp_params* params, vf_stream stream);
 
-/* Many independent volumes, one frame each, in the same three launches.
+/* Many independent volumes, one frame each, in the same four launches.
  * params[i] belongs to volume i; all volumes must share params[i].k. */
 int vf_warped_integrate_batch(vf_workspace* ws, int n, const vf_volume* volumes, const vf_graph* graphs,
                               const vf_frame* frames, const vf_warp_params* params, vf_stream stream);

@@ -1163,7 +1163,7 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     return VF_OK;
 }
 
-// Everything a frame's three launches need besides the device buffers.
+// Everything a frame's four launches need besides the device buffers.
 struct FrameSetup {
     int nvol = 0;
     int k = 0;
