@@ -20,6 +20,19 @@ def test_every_brick_shape_matches_oracle(vf, oracle, shape):
     assert st["updated"] > 500
 
 
+@pytest.mark.parametrize("shape", range(len(L.BRICK_SHAPES)))
+def test_node_cubes_wider_than_bricks(vf, oracle, shape):
+    """Node cubes 41-43 voxels wide (radius 0.1 m at about 5 mm voxels):
+    wider than every brick shape and than the 40x36x44 volume along x and y,
+    so each brick's range of a cube is clipped on both sides, and voxels have
+    more than k = 8 candidates among the 12 nodes."""
+    print(f"brick shape {L.BRICK_SHAPES[shape]}", flush=True)
+    s = scenes.small_scene(seed=33, k=8, nodes=12, radius=0.1)
+    assert min(s.cube) > 33 and s.cube[0] > s.dim[0] and s.cube[1] > s.dim[1]
+    st = run_frames(vf, oracle, s, [0, 1], tuning=(shape, 0))
+    assert st["updated"] > 500
+
+
 def test_one_workgroup_per_cu_and_spill(vf, oracle):
     """One resident workgroup per CU (long static schedules) and spilled brick
     lists, on the smallest brick."""
