@@ -292,10 +292,12 @@ __device__ __forceinline__ uint32_t slice_bits(const Slice& s, float iso) {
     return (s.t0 > iso ? 1u : 0u) | (s.t1 > iso ? 2u : 0u) | ((s.w0 == 0.0f || s.w1 == 0.0f) ? 4u : 0u);
 }
 
-// FULL needs many registers only for surface cubes (edge points of the
-// degenerate-triangle test); capping it keeps the z-march at full occupancy.
+// FULL needs about 120 VGPRs for surface cubes (edge points of the
+// degenerate-triangle test).  Four waves per SIMD keep them in registers;
+// capping the count at eight waves (64 VGPRs) spilled 152 B per lane and
+// measured 0.76 ms for the full 512^3 extraction against 0.70 ms.
 template <bool FULL>
-__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_mc_count(
+__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_mc_count(
     McGeom g, const float* __restrict__ tsdf, const float* __restrict__ wt, uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_lut[256];
     static_assert(kMcBlock == 256, "one lookup entry per thread");
