@@ -106,6 +106,38 @@ def test_batched_equals_sequential(vf, oracle):
         np.testing.assert_array_equal(g.points.download(), nodes_or)
 
 
+def test_batch_of_many_volumes_some_without_nodes(vf, oracle):
+    """Twenty small volumes in one call, every fourth with an empty motion
+    graph: an empty graph owns no (node, brick) items, so its first item is
+    the next volume's, and the kernels must still find each item's and each
+    brick's volume.  Every volume gets the oracle's bits; the empty ones stay
+    untouched."""
+    vols = []
+    for i in range(20):
+        s = scenes.small_scene(seed=60 + i, k=8, nodes=20 + 3 * i, dim=(16 + 2 * (i % 5), 20, 18 + 4 * (i % 3)))
+        if i % 4 == 1:
+            s.nodes = s.nodes[:0]
+            s.radii = s.radii[:0]
+        vols.append(s)
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0).reshape(-1, 16), s.radii) for s in vols]
+    depths = [dev(s.depth(0)) for s in vols]
+    ws = ops.Workspace(max_volumes=len(vols))
+    ops.warped_integrate_batch(ws, tsdfs, graphs, depths, [s.intrinsics() for s in vols], 8, 0.012, 64.0,
+                               [s.cube for s in vols])
+    updated = 0
+    for i, (s, t, g) in enumerate(zip(vols, tsdfs, graphs)):
+        n = s.n_voxels
+        t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+        if s.nodes.shape[0]:
+            oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, s.transforms(0), s.dim, s.voxel_size, s.size,
+                                s.cube, s.center, s.depth(0), s.delta, s.intrinsics(), s.max_weight)
+        np.testing.assert_array_equal(t.tsdf.download(), t_or, err_msg=f"tsdf of volume {i}")
+        np.testing.assert_array_equal(t.weights.download(), w_or, err_msg=f"weights of volume {i}")
+        updated += int((w_or > 0).sum())
+    assert updated > 1000
+
+
 def test_fused_deterministic_repeat(vf, oracle):
     """Same inputs twice (fresh volumes): identical bits (no race in the knn)."""
     s = scenes.small_scene(seed=14, k=8, nodes=90)
