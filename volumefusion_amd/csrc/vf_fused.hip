@@ -9,12 +9,15 @@
 //               its reciprocal, transform rows); append the node to the
 //               brick's list.  Also clears the collision tiles the previous
 //               frame flagged.
-//   k_compact   one thread per brick: the active list (bricks with a
-//               non-empty node list) in brick order.
+//   k_compact   two bricks per thread: the active list (bricks with a
+//               non-empty node list) in brick order within each workgroup's
+//               run.
 //   k_pass_a    one workgroup per active brick at a time, bricks drawn from
-//               per-XCD queues (x fastest; one voxel per thread): stage the
-//               brick's node list in LDS sorted by node index, per voxel
-//               select the canonical top-K, weights, blend, target voxel.
+//               per-XCD queues (one voxel per thread, each wave a 4x4x4
+//               block): stage the brick's node list in LDS (each cube as
+//               brick-local byte ranges), per voxel select the canonical
+//               top-K (ties to the smaller node index), weights, blend,
+//               target voxel.
 //               Collisions are counted in an LDS window of 8^3 tiles around
 //               the brick (2 bits per voxel: seen once / seen twice) and
 //               merged into the global bitmap once per brick, a word at a
@@ -167,15 +170,12 @@ constexpr int kMaxBatch = 256;  // volumes per call
 template <class S>
 __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     const VolDesc* __restrict__ vols, int nvol, uint32_t total, NodeRec* __restrict__ nrec,
-    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ active2,
-    Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag,
-    uint32_t clear_tiles, uint64_t brick_cap, int parity_arg,
-    unsigned long long* __restrict__ stat, unsigned long long* __restrict__ totals) {
+    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, Ctrl* __restrict__ ctrl,
+    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t clear_tiles,
+    unsigned long long* __restrict__ stat) {
     __shared__ uint32_t s_k1[kMaxBatch];  // first (node, brick) item of each volume
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    const uint32_t parity = frame_parity(ctrl, parity_arg);
-    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
     for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_k1[v] = vols[v].k1_base;
 
     // Clear the collision tiles the previous frame on this workspace flagged
@@ -192,10 +192,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     if (gid == 0) ctrl->overflow_records = 0u;
     __syncthreads();
 
-    // This thread's (node, brick) pair; `act` when it is the brick's first
-    // node, which makes the brick active.
-    bool act = false;
-    uint32_t entry = 0;
+    // This thread's (node, brick) pair: the node joins the brick's list.
     do {
         if (gid >= total) break;
         int v = 0, hi = nvol - 1;  // last volume whose first item is <= gid
@@ -277,12 +274,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
         const uint32_t gb = vd.brick_base + b;
         const uint32_t slot = atomicAdd(brick_count + gb, 1u);
         if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
-        act = slot == 0;
-        entry = ((uint32_t)v << 24) | b;
     } while (false);
-
-    (void)act;
-    (void)entry;
 }
 
 // The frame's active bricks (those with a non-empty node list), in brick
@@ -1356,8 +1348,8 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
         for (int i = 0; i < 6; ++i) e[i] = (fs.timing == 2 && (i < 2 || i > 3)) ? nullptr : ev[i];
     const VolDesc* const vols = fs.vols;
     launch(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), stream, e[0], nullptr,
-           vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.active, B.ctrl, B.bitmap,
-           B.tile_flag, (uint32_t)fs.clear_tiles, B.brick_cap, parity_arg, B.stat, B.totals);
+           vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.ctrl, B.bitmap, B.tile_flag,
+           (uint32_t)fs.clear_tiles, B.stat);
     VF_LAUNCH_CHECK("k_prep_bin");
     launch(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads * kCompactPer - 1) / (kPrepThreads * kCompactPer))),
            dim3(kPrepThreads), stream,
