@@ -138,6 +138,51 @@ def test_batch_of_many_volumes_some_without_nodes(vf, oracle):
     assert updated > 1000
 
 
+def test_batch_at_the_volume_limit(vf, oracle):
+    """256 volumes, the most one call takes, every seventh with an empty motion
+    graph: the per-volume tables in LDS are full and the searches over them
+    take every step.  Each volume gets the oracle's bits; a 257th volume is
+    rejected on the host with VF_ERR_INVALID_ARGUMENT before anything runs.
+    The scenes share two node sets (dart throwing is slow) and differ in
+    extent, node count, warps and noise."""
+    import dataclasses
+
+    from volumefusion_amd import _lib as L
+
+    bases = [scenes.small_scene(seed=300 + b, k=8, nodes=14, dim=(12, 12, 12)) for b in range(2)]
+    vols = []
+    for i in range(256):
+        b = bases[i % 2]
+        dim = (9 + i % 5, 10 + i % 3, 8 + i % 4)
+        g = 0 if i % 7 == 3 else 4 + i % 11
+        vs = b.voxel_size[0]
+        vols.append(dataclasses.replace(b, dim=dim, size=tuple(float(np.float32(d * vs)) for d in dim),
+                                        nodes=b.nodes[:g].copy(), radii=b.radii[:g].copy(), seed=400 + i))
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0).reshape(-1, 16), s.radii) for s in vols]
+    depths = [dev(s.depth(0)) for s in vols]
+    ws = ops.Workspace(max_volumes=257)
+    with pytest.raises(L.VFError) as e:
+        ops.warped_integrate_batch(ws, tsdfs + tsdfs[:1], graphs + graphs[:1], depths + depths[:1],
+                                   [s.intrinsics() for s in vols + vols[:1]], 8, 0.012, 64.0,
+                                   [s.cube for s in vols + vols[:1]])
+    assert e.value.status == 1  # VF_ERR_INVALID_ARGUMENT
+    ops.warped_integrate_batch(ws, tsdfs, graphs, depths, [s.intrinsics() for s in vols], 8, 0.012, 64.0,
+                               [s.cube for s in vols])
+    updated = 0
+    for i, (s, t, g) in enumerate(zip(vols, tsdfs, graphs)):
+        n = s.n_voxels
+        t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+        if s.nodes.shape[0]:
+            oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, s.transforms(0), s.dim, s.voxel_size, s.size,
+                                s.cube, s.center, s.depth(0), s.delta, s.intrinsics(), s.max_weight)
+        np.testing.assert_array_equal(t.tsdf.download(), t_or, err_msg=f"tsdf of volume {i}")
+        np.testing.assert_array_equal(t.weights.download(), w_or, err_msg=f"weights of volume {i}")
+        np.testing.assert_array_equal(g.points.download(), nodes_or, err_msg=f"nodes of volume {i}")
+        updated += int((w_or > 0).sum())
+    assert updated > 10000
+
+
 def test_fused_deterministic_repeat(vf, oracle):
     """Same inputs twice (fresh volumes): identical bits (no race in the knn)."""
     s = scenes.small_scene(seed=14, k=8, nodes=90)
