@@ -381,7 +381,7 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
                 "submit_lateness_p99_ms": _pct([x * 1e3 for x in late], 99)}
 
     poisson = poisson_run(fps, seconds, 0)
-    ramp = [poisson_run(rate, 1.0, r + 1) for r, rate in enumerate((120.0, 240.0, 360.0, 480.0))]
+    ramp = [poisson_run(rate, 1.0, r + 1) for r, rate in enumerate((120.0, 240.0, 300.0, 330.0, 360.0, 480.0))]
     within = [x for x in [poisson] + ramp if x["p99_ms"] is not None and x["p99_ms"] < 33.0]
     capacity = max(within, key=lambda x: x["achieved_fps"]) if within else None
 
