@@ -829,7 +829,7 @@ struct VolPtr {
     float* tsdf;
     float* wt;
     uint32_t voxel_base;
-    uint32_t pad;
+    uint32_t node_base;
 };
 
 // The collision word of a pending record's target: bit (target & 31) is set
@@ -859,8 +859,9 @@ __device__ __forceinline__ bool apply_record(const uint4 rec, uint32_t word, con
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr, int nvol, Arena arena,
-                                                Ctrl* __restrict__ ctrl, const uint32_t* __restrict__ bitmap,
+__global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr, int nvol, uint32_t total_nodes,
+                                                Arena arena, Ctrl* __restrict__ ctrl,
+                                                const uint32_t* __restrict__ bitmap,
                                                 const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec,
                                                 unsigned long long* __restrict__ stat,
                                                 unsigned long long* __restrict__ totals) {
@@ -910,18 +911,27 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
             ++collided;
     }
     // node positions as the reference's two translations leave them,
-    // fl(fl(p - c) + c) (src/VolumeFusion.cu:708, 721)
+    // fl(fl(p - c) + c) (src/VolumeFusion.cu:708, 721): one pass over the
+    // batch's nodes, each thread finding its node's volume in LDS (a loop
+    // over the volumes would chain one descriptor load per volume)
     const uint32_t gid = blockIdx.x * blockDim.x + tid;
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    for (int v = 0; v < nvol; ++v) {
-        const VolDesc& vd = vols[v];
-        if (!vd.nodes_out) continue;
-        for (uint32_t j = gid; j < vd.G; j += nthreads) {
-            const float4 m = nrec[vd.node_base + j].m;
-            vd.nodes_out[3ull * j + 0] = m.x + vd.c[0];
-            vd.nodes_out[3ull * j + 1] = m.y + vd.c[1];
-            vd.nodes_out[3ull * j + 2] = m.z + vd.c[2];
+    for (uint32_t g = gid; g < total_nodes; g += nthreads) {
+        int lo = 0, hi = nvol - 1;  // last volume whose first node is <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_vp[mid].node_base <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
         }
+        const VolDesc& vd = vols[lo];
+        const uint32_t j = g - vd.node_base;
+        if (!vd.nodes_out || j >= vd.G) continue;
+        const float4 m = nrec[g].m;
+        vd.nodes_out[3ull * j + 0] = m.x + vd.c[0];
+        vd.nodes_out[3ull * j + 1] = m.y + vd.c[1];
+        vd.nodes_out[3ull * j + 2] = m.z + vd.c[2];
     }
     if (collided) atomicAdd(&s_cnt[0], collided);
     if (updated) atomicAdd(&s_cnt[1], updated);
@@ -1164,6 +1174,7 @@ struct FrameSetup {
     uint32_t k1 = 0;
     uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
     uint64_t tiles = 0;        // collision tiles of this batch
+    uint64_t nodes = 0;        // motion-graph nodes of this batch
     uint64_t clear_tiles = 0;  // tiles whose flags prep clears: every tile a previous frame or a replay may flag
     const VolDesc* vols = nullptr;  // the frame's device descriptor set
     const VolPtr* vptr = nullptr;
@@ -1263,7 +1274,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
         ws->vptr_h[v].tsdf = vd.tsdf;
         ws->vptr_h[v].wt = vd.wt;
         ws->vptr_h[v].voxel_base = vd.voxel_base;
-        ws->vptr_h[v].pad = 0;
+        ws->vptr_h[v].node_base = vd.node_base;
         if (nb > 0xffffffull) return fail(VF_ERR_CAPACITY, "volume has more than 2^24 bricks");
         if (need.bricks > 0xffffffffull || need.tiles >= (1ull << 23) || k1 > 0xffffffffull ||
             voxels > 0xffffffffull)
@@ -1333,6 +1344,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->blocks_a = blocks_a;
     fs->bricks = need.bricks;
     fs->tiles = need.tiles;
+    fs->nodes = need.nodes;
     fs->clear_tiles = std::max<uint64_t>(ws->dirty_tiles, need.tiles);
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
     fs->arena_cap = arena_cap;
@@ -1387,7 +1399,8 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
 #undef VF_PASS_A
     VF_LAUNCH_CHECK("k_pass_a");
     if (!fs.knn_only) {
-        launch(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), stream, e[4], e[5], fs.vptr, fs.nvol, arena, B.ctrl,
+        launch(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), stream, e[4], e[5], fs.vptr, fs.nvol,
+               (uint32_t)fs.nodes, arena, B.ctrl,
                (const uint32_t*)B.bitmap, vols, (const NodeRec*)B.nrec, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_pass_b");
     }
