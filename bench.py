@@ -16,17 +16,17 @@ Prints one JSON line:
   cpu_baseline  the C oracle's voxel-major frame on the host cores, on a bounded
              z-slab sample of the same workload.
 
-  batching   config 4: 16 volumes of uneven size, one frame each: 16 plain
+  batching   batch16: 16 volumes of uneven size, one frame each: 16 plain
              per-volume calls per step against one autotuned frame plan
              replayed as a HIP graph (the same work, results identical).
-  extras     on the config-3 state: marching cubes of the integrated volume,
+  extras     on the ellipsoids512 state: marching cubes of the integrated volume,
              the fused depth ingest of a frame, the fused point warp.
-  serving    config 5: 32 resident streams, Poisson frame arrivals at 30 fps
+  serving    streams32: 32 resident streams, Poisson frame arrivals at 30 fps
              each, continuous batching; p50/p99 frame latency (submission to
              completion) and aggregate updates/s; then the same streams
              saturated against plain per-volume calls.
 
-Usage: python bench.py [--steps K] [--warmup W] [--workload config3]
+Usage: python bench.py [--steps K] [--warmup W] [--workload ellipsoids512]
                        [--no-cpu] [--no-batching] [--no-serving]
 """
 from __future__ import annotations
@@ -57,13 +57,13 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="config3", choices=["config1", "config2", "config3"])
+    ap.add_argument("--workload", default="ellipsoids512", choices=["sphere128", "sphere256_warped", "ellipsoids512"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device copy of known size (HBM counter calibration under rocprofv3)")
-    ap.add_argument("--no-batching", action="store_true", help="skip the config-4 batching measurement")
-    ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving measurement")
+    ap.add_argument("--no-batching", action="store_true", help="skip the batch16 batching measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the streams32 serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson arrival run")
     ap.add_argument("--no-tune", action="store_true", help="default brick shape instead of the autotuned one")
     ap.add_argument("--tuning", default=None, metavar="SHAPE,WG",
@@ -75,7 +75,8 @@ def parse_args():
 def workload(name: str):
     from volumefusion_amd import scenes
 
-    return {"config1": scenes.config1, "config2": scenes.config2, "config3": scenes.config3}[name]()
+    return {"sphere128": scenes.sphere128, "sphere256_warped": scenes.sphere256_warped,
+            "ellipsoids512": scenes.ellipsoids512}[name]()
 
 
 def algorithmic_bytes(U: float, W: int, H: int, G: int) -> dict:
@@ -304,7 +305,7 @@ def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
     for p_ in plans:
         p_.close()
     return {
-        "workload": f"config4: {n} volumes, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
+        "workload": f"batch16: {n} volumes, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
                     f"{min(v.nodes.shape[0] for v in vols)}..{max(v.nodes.shape[0] for v in vols)} nodes, k=8, "
                     "640x576 frames",
         "voxels": int(sum(v.n_voxels for v in vols)), "nodes": int(sum(v.nodes.shape[0] for v in vols)),
@@ -445,7 +446,7 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
             a.free()
             b.free()
     return {
-        "workload": f"config5: {n} streams, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
+        "workload": f"streams32: {n} streams, {min(v.dim[0] for v in vols)}^3..{max(v.dim[0] for v in vols)}^3, "
                     f"{min(v.nodes.shape[0] for v in vols)}..{max(v.nodes.shape[0] for v in vols)} nodes, k=8, "
                     f"640x576 frames from page-locked host memory, Poisson arrivals",
         "poisson": poisson, "ramp": ramp,
@@ -471,10 +472,10 @@ def main() -> int:
     c4 = c5 = None
     jobs = [(s, 0), (s, 1)]
     if not args.no_batching:
-        c4 = scenes.config4()
+        c4 = scenes.batch16()
         jobs += [(v, f) for v in c4 for f in (0, 1)]
     if not args.no_serving:
-        c5 = scenes.config5()
+        c5 = scenes.streams32()
         jobs += [(v, f) for v in c5 for f in (0, 1)]
     rendered = scenes.render_frames(jobs)
     print(f"inputs ready ({len(jobs)} frames)", file=sys.stderr, flush=True)

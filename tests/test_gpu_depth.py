@@ -12,7 +12,7 @@ F = np.float32
 
 
 def frame(seed=0):
-    s = scenes.config1()
+    s = scenes.sphere128()
     d = s.depth(seed).copy()  # 640x576, sphere with noise, zeros for no hit
     d[100:110, 200:215] = np.nan
     d[300, 300:310] = 2.5  # beyond the threshold

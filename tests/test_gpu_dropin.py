@@ -263,7 +263,7 @@ def test_reference_sequence_chained(vf, oracle):
 
 
 def test_integrate_plain_and_camera_frame(vf, oracle):
-    s = scenes.config1()
+    s = scenes.sphere128()
     depth = s.depth(0)
     H, W = depth.shape
     n = s.n_voxels

@@ -70,18 +70,18 @@ def test_fused_empty_graph_leaves_volume(vf, oracle):
     assert (tsdf.tsdf.download() == -1).all() and (tsdf.weights.download() == 0).all()
 
 
-def test_fused_config1(vf, oracle):
+def test_fused_sphere128(vf, oracle):
     """128^3, 256 nodes, identity transforms, k=4 (the CPU-runnable config)."""
-    s = scenes.config1()
+    s = scenes.sphere128()
     st = run_frames(vf, oracle, s, [0, 1])
     assert st["updated"] > 10000
 
 
 @pytest.mark.slow
-def test_fused_config2_frames(vf, oracle):
+def test_fused_sphere256_warped_frames(vf, oracle):
     """256^3, 2048 nodes with random warps, k=8: first three frames against the
     gather oracle (itself equal to the node-major oracle)."""
-    s = scenes.config2()
+    s = scenes.sphere256_warped()
     run_frames(vf, oracle, s, [0, 1, 2], gather=True)
 
 

@@ -320,36 +320,36 @@ CAM_640 = Camera(640, 576, 504.0, 504.0, 320.0, 288.0)
 CAM_1024 = Camera(1024, 1024, 900.0, 900.0, 512.0, 512.0)
 
 
-def config1(seed: int = 0) -> VolumeScene:
+def sphere128(seed: int = 0) -> VolumeScene:
     """128^3 at 4 mm centred at z=1 m; one 640x576 frame of a sphere (r=0.3 m
     at 1 m, sigma 1 mm); 256 Poisson-disk nodes, identity transforms, k=4,
     r=25 mm, delta 12 mm, max weight 64."""
     rng = np.random.default_rng([seed, 0])
     sph = make_surface(rng, (0.0, 0.0, 1.0), (0.3, 0.3, 0.3), eps=0.0, rot=np.eye(3))
     nodes = poisson_disk([sph], 256, rng)
-    return VolumeScene("config1", (128, 128, 128), (0.512, 0.512, 0.512), (0.0, 0.0, 1.0), CAM_640, nodes,
+    return VolumeScene("sphere128", (128, 128, 128), (0.512, 0.512, 0.512), (0.0, 0.0, 1.0), CAM_640, nodes,
                        np.full(256, 0.025, np.float32), 0.025, 4, 0.012, 64.0, [sph], seed, noise_sigma=0.001)
 
 
-def config2(seed: int = 0) -> VolumeScene:
+def sphere256_warped(seed: int = 0) -> VolumeScene:
     """256^3 at 3 mm; 640x576 frames of a sphere; 2048 nodes with smooth random
     warps (<= 5 deg, <= 10 mm); k=8, r=20 mm."""
     rng = np.random.default_rng([seed, 0])
     sph = make_surface(rng, (0.0, 0.0, 1.0), (0.3, 0.3, 0.3), eps=0.0, rot=np.eye(3))
     nodes = poisson_disk([sph], 2048, rng)
-    return VolumeScene("config2", (256, 256, 256), (0.768, 0.768, 0.768), (0.0, 0.0, 1.0), CAM_640, nodes,
+    return VolumeScene("sphere256_warped", (256, 256, 256), (0.768, 0.768, 0.768), (0.0, 0.0, 1.0), CAM_640, nodes,
                        np.full(2048, 0.020, np.float32), 0.020, 8, 0.012, 64.0, [sph], seed, noise_sigma=0.001,
                        warp_angle_deg=5.0, warp_translation=0.010)
 
 
-def config3(seed: int = 0) -> VolumeScene:
+def ellipsoids512(seed: int = 0) -> VolumeScene:
     """512^3 at 2 mm; 1024x1024 frame (fx=fy=900) of two intersecting deformed
     ellipsoids, sigma 1.5 mm, 5% dropout; 4096 nodes, k=8, r=20 mm."""
     rng = np.random.default_rng([seed, 0])
     e1 = make_surface(rng, (-0.09, 0.0, 1.0), (0.30, 0.22, 0.18), eps=0.06)
     e2 = make_surface(rng, (0.12, 0.05, 1.05), (0.22, 0.28, 0.20), eps=0.06)
     nodes = poisson_disk([e1, e2], 4096, rng)
-    return VolumeScene("config3", (512, 512, 512), (1.024, 1.024, 1.024), (0.0, 0.0, 1.0), CAM_1024, nodes,
+    return VolumeScene("ellipsoids512", (512, 512, 512), (1.024, 1.024, 1.024), (0.0, 0.0, 1.0), CAM_1024, nodes,
                        np.full(4096, 0.020, np.float32), 0.020, 8, 0.012, 64.0, [e1, e2], seed, noise_sigma=0.0015,
                        dropout=0.05, warp_angle_deg=5.0, warp_translation=0.010)
 
@@ -368,7 +368,7 @@ def random_volume(name: str, rng_seed, dim: int, nodes: int, side: float = 0.8) 
                        noise_sigma=0.001, warp_angle_deg=5.0, warp_translation=0.010)
 
 
-def config4(seed: int = 0, count: int = 16) -> list[VolumeScene]:
+def batch16(seed: int = 0, count: int = 16) -> list[VolumeScene]:
     """16 volumes, resolution log-uniform in [64, 384]^3, nodes log-uniform in
     [128, 4096], k=8, each with its own 640x576 frame."""
     rng = np.random.default_rng([seed, 4])
@@ -376,18 +376,18 @@ def config4(seed: int = 0, count: int = 16) -> list[VolumeScene]:
     for i in range(count):
         dim = int(round(math.exp(rng.uniform(math.log(64), math.log(384)))))
         g = int(round(math.exp(rng.uniform(math.log(128), math.log(4096)))))
-        out.append(random_volume(f"config4_{i}", [seed, 40, i], dim, g))
+        out.append(random_volume(f"batch16_{i}", [seed, 40, i], dim, g))
     return out
 
 
-def config5(seed: int = 0, count: int = 32) -> list[VolumeScene]:
+def streams32(seed: int = 0, count: int = 32) -> list[VolumeScene]:
     """32 streams, volumes 128^3..512^3, 512..4096 nodes, k=8."""
     rng = np.random.default_rng([seed, 5])
     out = []
     for i in range(count):
         dim = int(round(math.exp(rng.uniform(math.log(128), math.log(512)))))
         g = int(round(math.exp(rng.uniform(math.log(512), math.log(4096)))))
-        out.append(random_volume(f"config5_{i}", [seed, 50, i], dim, g))
+        out.append(random_volume(f"streams32_{i}", [seed, 50, i], dim, g))
     return out
 
 
