@@ -85,6 +85,16 @@ def test_fused_sphere256_warped_frames(vf, oracle):
     run_frames(vf, oracle, s, [0, 1, 2], gather=True)
 
 
+def test_fused_ellipsoids512_full_size(vf, oracle):
+    """The headline workload at its full size (512^3 at 2 mm, 4096 nodes, k=8,
+    1024x1024 depth with noise and dropout), two frames, every voxel and node
+    position bit for bit against the gather oracle on 16 host threads, and
+    the frame counters equal."""
+    s = scenes.ellipsoids512()
+    st = run_frames(vf, oracle, s, [0, 1], gather=True, threads=16)
+    assert st["updated"] > 1_000_000 and st["supported"] > 8_000_000
+
+
 def test_batched_equals_sequential(vf, oracle):
     """Volumes of uneven size and node count in one batched call give the same
     bits as one call per volume, and as the oracle."""
