@@ -125,3 +125,35 @@ def test_plan_autotune_leaves_volumes_and_keeps_results(vf, oracle):
     assert plan2.tuning()["wg_per_cu"] == chosen["wg_per_cu"]
     plan.close()
     plan2.close()
+
+
+def test_batch16_full_size_graph_replays(vf, oracle):
+    """The 16-volume batching workload at its full size (64^3..346^3, 274..3044
+    nodes, k=8, one 640x576 frame each) as one plan replayed as a HIP graph,
+    two frames: every voxel and node position of every volume bit for bit
+    against the gather oracle on 16 host threads."""
+    vols = scenes.batch16()
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii) for s in vols]
+    depths = [dev(s.depth(0)) for s in vols]
+    ref = [(np.full(s.n_voxels, -1, F), np.zeros(s.n_voxels, F), s.nodes.copy()) for s in vols]
+    stream = Stream()
+    plan = ops.Plan(tsdfs, graphs, depths, [s.intrinsics() for s in vols], 8, [s.delta for s in vols],
+                    [s.max_weight for s in vols], [s.cube for s in vols], graph=True, stream=stream)
+    for f in range(2):
+        print(f"batch16 frame {f}", flush=True)
+        frames = [(s.depth(f), s.transforms(f)) for s in vols]
+        for g, d, (depth, T) in zip(graphs, depths, frames):
+            g.transforms.upload(T)
+            d.upload(depth)
+        plan.launch(stream, sync=True)
+        for s, (t, w, nodes), (depth, T) in zip(vols, ref, frames):
+            oracle.warped_frame_gather(s.k, t, w, nodes, s.radii, T, s.dim, s.voxel_size, s.size, s.cube, s.center,
+                                       depth, s.delta, s.intrinsics(), s.max_weight, nthreads=16)
+        for i, (tsdf, g, (t, w, nodes)) in enumerate(zip(tsdfs, graphs, ref)):
+            np.testing.assert_array_equal(tsdf.weights.download(), w, err_msg=f"frame {f}: weights of volume {i}")
+            np.testing.assert_array_equal(tsdf.tsdf.download(), t, err_msg=f"frame {f}: tsdf of volume {i}")
+            np.testing.assert_array_equal(g.points.download(), nodes, err_msg=f"frame {f}: nodes of volume {i}")
+        last, _ = plan.stats()
+        assert last["updated"] > 1_000_000
+    plan.close()
