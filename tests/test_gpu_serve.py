@@ -126,3 +126,32 @@ def test_borrowed_submit_equals_copied(vf):
             b.free()
     for x, y in zip(*out):
         np.testing.assert_array_equal(x, y)
+
+
+def test_streams32_full_size_one_frame_each(vf, oracle):
+    """The 32-stream serving workload at its full size (129^3..491^3,
+    624..3721 nodes, k=8, 640x576 frames) through the serving loop with its
+    benchmark settings (eight lanes of at most eight frames): every stream's
+    volume and node positions bit for bit against the gather oracle on 16
+    host threads."""
+    vols = scenes.streams32()
+    frames = [(s.depth(0), s.transforms(0)) for s in vols]
+    srv = serve.Server(200 * GiB, k=8, max_batch=8, lanes=8)
+    ids, keeps, tickets = [], [], []
+    for s in vols:
+        d, kp = serve.scene_desc(s)
+        ids.append(srv.open(d))
+        keeps.append(kp)
+    for sid, (depth, T) in zip(ids, frames):
+        tickets.append(srv.submit(sid, depth, T))
+    results = [srv.wait(t) for t in tickets]
+    assert all(r["status"] == 0 for r in results)
+    for i, (sid, s, (depth, T)) in enumerate(zip(ids, vols, frames)):
+        t, w, p = srv.read(sid)
+        t_or, w_or, n_or = np.full(s.n_voxels, -1, F), np.zeros(s.n_voxels, F), s.nodes.copy()
+        oracle.warped_frame_gather(s.k, t_or, w_or, n_or, s.radii, T, s.dim, s.voxel_size, s.size, s.cube,
+                                   s.center, depth, s.delta, s.intrinsics(), s.max_weight, nthreads=16)
+        np.testing.assert_array_equal(w, w_or, err_msg=f"weights of stream {i}")
+        np.testing.assert_array_equal(t, t_or, err_msg=f"tsdf of stream {i}")
+        np.testing.assert_array_equal(p, n_or, err_msg=f"nodes of stream {i}")
+    srv.close_server()
