@@ -79,10 +79,11 @@ def test_fused_sphere128(vf, oracle):
 
 @pytest.mark.slow
 def test_fused_sphere256_warped_frames(vf, oracle):
-    """256^3, 2048 nodes with random warps, k=8: first three frames against the
-    gather oracle (itself equal to the node-major oracle)."""
+    """256^3, 2048 nodes with random warps, k=8, all 30 frames of the workload:
+    every voxel and node position checked bit for bit after every frame
+    against the gather oracle (itself equal to the node-major oracle)."""
     s = scenes.sphere256_warped()
-    run_frames(vf, oracle, s, [0, 1, 2], gather=True)
+    run_frames(vf, oracle, s, list(range(30)), gather=True)
 
 
 def test_fused_ellipsoids512_full_size(vf, oracle):
