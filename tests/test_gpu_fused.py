@@ -1,4 +1,4 @@
-"""GPU parity of the fused three-kernel frame (and its batched form) against
+"""GPU parity of the fused four-kernel frame (and its batched form) against
 the oracle's frame, bit for bit, over several frames."""
 import numpy as np
 import pytest
