@@ -69,6 +69,8 @@ def parse_args():
     ap.add_argument("--tuning", default=None, metavar="SHAPE,WG",
                     help="fixed brick shape index and workgroups per CU instead of autotuning (e.g. 2,0)")
     ap.add_argument("--no-extras", action="store_true", help="skip marching cubes, ingest and point warp")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the binary64 error report of sphere128, sphere256_warped and ellipsoids512")
     return ap.parse_args()
 
 
@@ -77,6 +79,22 @@ def workload(name: str):
 
     return {"sphere128": scenes.sphere128, "sphere256_warped": scenes.sphere256_warped,
             "ellipsoids512": scenes.ellipsoids512}[name]()
+
+
+# The sources the fused frame's kernels are built from: counter profiles are
+# valid only for the sources they were taken on.
+KERNEL_SOURCES = ("volumefusion_amd/csrc/vf_fused.hip", "volumefusion_amd/csrc/vf_device.hpp",
+                  "volumefusion_amd/csrc/vf_host.hpp")
+
+
+def kernel_sources_sha256() -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update(rel.encode())
+        h.update((REPO / rel).read_bytes())
+    return h.hexdigest()
 
 
 def algorithmic_bytes(U: float, W: int, H: int, G: int) -> dict:
@@ -120,6 +138,92 @@ def cpu_baseline(s, depth, T, seconds: float) -> dict:
     dt1, st1 = run(mid - half, mid + half, 1)
     out["single_core"] = {"value": st1["updated"] / dt1, "cores": 1,
                           "sample": f"one frame, source voxels z in [{mid - half}, {mid + half}) of {Z}, {dt1:.2f} s"}
+    out["host_cpus"] = os.cpu_count()
+    out["threads_note"] = ("threads = min(16, os.cpu_count()): one GPU's share of the host is 16 cores on the "
+                           "MI355X boxes this runs on")
+    out["plain_sphere128"] = cpu_plain_sphere128(seconds / 6)
+    return out
+
+
+def cpu_plain_sphere128(seconds: float) -> dict:
+    """The plain method on the CPU path (TSDFVolume::integrate,
+    src/TSDFVolume.cpp:55-171, restated in the oracle, single-threaded like the
+    reference's): one static-camera frame of sphere128 into a fresh 128^3
+    volume, repeated for about `seconds`."""
+    from oracle import oracle as O
+    from volumefusion_amd import scenes
+
+    s = scenes.sphere128()
+    depth = s.depth(0)
+    n = s.n_voxels
+    pose = np.eye(4, dtype=np.float32)
+    pose[:3, 3] = np.asarray(s.center, np.float32)
+    pose16 = np.ascontiguousarray(pose.T.reshape(16))
+    vol = O.make_vol(s.dim, s.voxel_size, s.size)
+    H, W = depth.shape
+    K = np.ascontiguousarray(s.intrinsics(), np.float32)
+    fn = O.lib().vfo_tsdfvolume_integrate
+    total, frames, upd = 0.0, 0, 0
+    while total < seconds or frames < 2:
+        t, w = np.full(n, -1, np.float32), np.zeros(n, np.float32)
+        t0 = time.perf_counter()
+        u = fn(t, w, depth, W, H, float(s.delta), O.C.byref(vol), pose16, K)
+        total += time.perf_counter() - t0
+        frames += 1
+        upd += int(u)
+    return {"value": upd / total, "unit": "voxel updates/s (plain method)", "cores": 1, "kind": "port",
+            "voxels_per_s": n * frames / total, "ms_per_frame": total / frames * 1e3,
+            "sample": f"sphere128: {frames} frames of TSDFVolume::integrate on one core, {upd // frames} updates "
+                      "per frame"}
+
+
+def fp64_error(tuning, threads: int) -> dict:
+    """Max and RMS |tsdf - tsdf64| of the device frames against the oracle's
+    binary64 companion (same discrete decisions, continuous values in
+    binary64), in units of the truncation distance (the stored tsdf is
+    sdf / delta), over the voxels with weight > 0: sphere128 (its one frame),
+    sphere256_warped (all 30 frames, checked after every frame),
+    ellipsoids512 (two frames).  The device frames are also checked bit for
+    bit against the float oracle.  Outside the timed region."""
+    from oracle import oracle as O
+    from volumefusion_amd import ops, scenes
+    from volumefusion_amd.device import DeviceArray
+
+    F = np.float32
+    out = {}
+    for name, fn, frames in (("sphere128", scenes.sphere128, 1), ("sphere256_warped", scenes.sphere256_warped, 30),
+                             ("ellipsoids512", scenes.ellipsoids512, 2)):
+        s = fn()
+        n = s.n_voxels
+        t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+        t64, w64 = np.full(n, -1.0), np.zeros(n)
+        tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
+        graph = ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii)
+        ws = ops.Workspace()
+        ws.set_tuning(*tuning)
+        per_frame = []
+        exact = True
+        t0 = time.perf_counter()
+        for f in range(frames):
+            depth, T = s.depth(f), s.transforms(f)
+            O.warped_frame_gather(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size, s.cube,
+                                  s.center, depth, s.delta, s.intrinsics(), s.max_weight, nthreads=threads,
+                                  tsdf64=t64, wt64=w64)
+            graph.transforms.upload(T)
+            ops.warped_integrate(ws, tsdf, graph, DeviceArray.from_numpy(depth), s.intrinsics(), s.k, s.delta,
+                                 s.max_weight, s.cube)
+            td, wd = tsdf.tsdf.download(), tsdf.weights.download()
+            exact = exact and np.array_equal(td, t_or) and np.array_equal(wd, w_or)
+            m = w_or > 0
+            e = np.abs(td[m].astype(np.float64) - t64[m])
+            per_frame.append((float(e.max()) if e.size else 0.0, float(np.sqrt(np.mean(e * e))) if e.size else 0.0))
+        out[name] = {"frames": frames, "voxels_weighted": int((w_or > 0).sum()),
+                     "max": max(x[0] for x in per_frame), "rms_last_frame": per_frame[-1][1],
+                     "rms_max_over_frames": max(x[1] for x in per_frame),
+                     "bit_exact_vs_float_oracle": bool(exact), "seconds": time.perf_counter() - t0}
+        del tsdf, graph, ws
+    out["unit"] = "truncation distance (tsdf = sdf / delta)"
+    out["tolerance_rms"] = 1e-5
     return out
 
 
@@ -216,6 +320,35 @@ def bench_extras(s, tsdf, graph, depth_np, stream) -> dict:
                          "points_per_s": npts / (ms * 1e-3),
                          "node_tests_per_s": npts * graph.count / (ms * 1e-3)}
     return out
+
+
+def bench_plain_sphere128(stream) -> dict:
+    """The plain method on the device (the reference's
+    integrateTSDFFromFrameKernel, src/TSDFIntegration.cu:69-96): one
+    static-camera frame of sphere128 into a 128^3 volume, timed with HIP
+    events over repeated frames (the volume keeps integrating the same
+    frame)."""
+    from volumefusion_amd import ops, scenes
+    from volumefusion_amd.device import DeviceArray, Event
+
+    s = scenes.sphere128()
+    tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta, stream=stream)
+    d = DeviceArray.from_numpy(s.depth(0), stream)
+    K = s.intrinsics()
+    for _ in range(3):
+        tsdf.integrate(d, K, 15.0, stream=stream)
+    reps = 50
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        tsdf.integrate(d, K, 15.0, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    n = s.n_voxels
+    return {"workload": "sphere128: 128^3 at 4 mm, one 640x576 frame, plain method", "ms_per_frame": ms,
+            "voxels_per_s": n / (ms * 1e-3), "hbm_bytes_per_frame": 16.0 * n,
+            "achieved_GBps": 16.0 * n / (ms * 1e-3) / 1e9}
 
 
 def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
@@ -565,16 +698,22 @@ def main() -> int:
     t_frame = sum(kms) / nk * 1e-3
     achieved = ab["k_pass_a"] / t_a
     # HBM bytes and VALU instructions per k_pass_a launch, from the committed
-    # rocprofv3 counter runs of this workload (valid for the brick shape they
-    # were taken with)
+    # rocprofv3 counter runs of this workload: valid only for the brick shape
+    # and the kernel sources they were taken with (else reported as stale)
     traffic = None
     valu = None
+    counters_note = None
     tfile = REPO / "profiles" / f"traffic_{args.workload}.json"
     if tfile.exists():
         try:
             prof = json.loads(tfile.read_text())
             ka = prof.get("k_pass_a", {})
-            if prof.get("brick") == tuning["brick"]:
+            if prof.get("kernel_sources_sha256") != kernel_sources_sha256():
+                counters_note = (f"stale: {tfile.name} was measured on other kernel sources; traffic and "
+                                 f"valu_roofline omitted")
+            elif prof.get("brick") != tuning["brick"]:
+                counters_note = f"stale: {tfile.name} was measured with brick {prof.get('brick')}"
+            else:
                 traffic = ka.get("hbm_bytes_per_launch")
                 if ka.get("valu_insts_per_launch"):
                     issue = ka["valu_insts_per_launch"] * VALU_CYCLES / (SIMDS * CLOCK_HZ)
@@ -620,6 +759,7 @@ def main() -> int:
             "bound": "hbm", "kernel": "k_pass_a", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
             "algorithmic_bytes_per_launch": ab["k_pass_a"], "avg_launch_ms": t_a * 1e3,
+            **({"counters": counters_note} if counters_note else {}),
         },
         "valu_roofline": valu,
         "frame_roofline": {
@@ -636,6 +776,7 @@ def main() -> int:
         print(f"extras: marching cubes {ex['marching_cubes']['ms']:.3f} ms, ingest "
               f"{ex['depth_ingest']['ms_wall_incl_centroid_readback']:.3f} ms, point warp "
               f"{ex['point_warp']['ms']:.3f} ms", file=sys.stderr, flush=True)
+    result["plain_sphere128_device"] = bench_plain_sphere128(stream)
     del tsdf, graph, ws
     if c4 is not None:
         off = 2
@@ -650,6 +791,12 @@ def main() -> int:
         print(f"serving: p50 {sv['poisson']['p50_ms']:.2f} ms, p99 {sv['poisson']['p99_ms']:.2f} ms, "
               f"saturated {sv['saturated']['fps']:.0f} fps vs plain {sv['plain']['with_upload']['fps']:.0f} fps",
               file=sys.stderr, flush=True)
+    if not args.no_fp64:
+        result["fp64_error"] = fp64_error((L.BRICK_SHAPES.index(tuple(tuning["brick"])), tuning["wg_per_cu"]),
+                                          min(16, os.cpu_count() or 1))
+        fe = result["fp64_error"]
+        print("fp64 error: " + ", ".join(f"{k} max {v['max']:.3g} rms {v['rms_last_frame']:.3g}"
+                                         for k, v in fe.items() if isinstance(v, dict)), file=sys.stderr, flush=True)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(s, depths[0], Ts[0], args.cpu_seconds)
     else:

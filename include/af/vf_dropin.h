@@ -282,7 +282,7 @@ inline void runTranslateKernel(Eigen::Vector3f* mesh, unsigned int meshSize, Eig
 }
 
 // The whole TSDF-integration block of the frame loop, src/VolumeFusion.cu:702-742
-// (volume rotation I, translation = frameCentroid), in four kernels and
+// (volume rotation I, translation = frameCentroid), in five kernels and
 // without the per-voxel knn/vertex/collision buffers.  `motionGraph` node
 // positions receive the reference's round trip fl(fl(p - c) + c).
 template <int K>

@@ -48,7 +48,8 @@ enum {
     VF_ERR_HIP = 3,
     VF_ERR_OUT_OF_MEMORY = 4,
     VF_ERR_WORKSPACE = 5,
-    VF_ERR_CAPACITY = 6
+    VF_ERR_CAPACITY = 6,
+    VF_ERR_SINGULAR = 7
 };
 
 /* Text of a status code, and the message of the last failure on this thread. */
@@ -267,7 +268,7 @@ int vf_eval_warp_weight(const float* radius, const float* dist, float* out, size
 /* One call performs the whole TSDF-integration block of the reference's frame
  * loop, src/VolumeFusion.cu:702-742 (translate nodes by -c, knn, weights,
  * vertices, translate back, warp, count collisions, integrate, with volume
- * rotation I and translation c), in four kernels and with no per-voxel
+ * rotation I and translation c), in five kernels and with no per-voxel
  * scratch arrays.  The tsdf/weight arrays are updated in place; when
  * graph.points_out is not NULL it receives the node positions as the
  * reference's two translations leave them, fl(fl(p - c) + c). */
@@ -315,8 +316,10 @@ typedef struct vf_frame_stats {
     uint64_t pending;         /* voxels that passed the projective tests   */
     uint64_t collided;        /* pending voxels dropped, target count > 1  */
     uint64_t updated;         /* voxels whose (tsdf, weight) changed       */
-    uint64_t active_bricks;   /* bricks touched by some node cube          */
+    uint64_t active_bricks;   /* bricks pass A swept                       */
     uint64_t overflow_bricks; /* bricks whose node list spilled            */
+    uint64_t screened_bricks; /* bricks with a node list the screen proved
+                                 unable to change the frame (not swept)    */
 } vf_frame_stats;
 
 
@@ -336,7 +339,7 @@ uint64_t vf_workspace_bytes(const vf_workspace* ws);
 int vf_warped_integrate(vf_workspace* ws, const vf_volume* volume, const vf_graph* graph, const vf_frame* frame,
                         const vf_warp_params* params, vf_stream stream);
 
-/* Many independent volumes, one frame each, in the same four launches.
+/* Many independent volumes, one frame each, in the same launches.
  * params[i] belongs to volume i; all volumes must share params[i].k. */
 int vf_warped_integrate_batch(vf_workspace* ws, int n, const vf_volume* volumes, const vf_graph* graphs,
                               const vf_frame* frames, const vf_warp_params* params, vf_stream stream);
@@ -361,8 +364,16 @@ int vf_workspace_kernel_ms(vf_workspace* ws, float* ms3, int* frames);
 /* Launch tuning.  brick_shape selects the voxel brick one pass-A workgroup
  * sweeps per step (one voxel per thread); wg_per_cu caps the resident pass-A
  * workgroups per CU (0 = as many as fit).  Every setting gives identical
- * results; only the speed differs.  ms_per_frame is what the autotuner
- * measured (0 when set by hand). */
+ * tsdf, weights and node positions; only the speed differs.  ms_per_frame is
+ * what the autotuner measured (0 when set by hand).
+ *
+ * The brick screen (on unless flags has VF_TUNE_NO_SCREEN) drops, before the
+ * voxel sweep, every brick that provably holds no voxel passing the
+ * projective tests and no voxel whose warped target can be the target of
+ * one: per-node displacement bounds of the blend, depth-tile maxima of the
+ * frame.  It is used for LBS volumes.  With it, vf_frame_stats.supported and
+ * in_bounds count the swept bricks' voxels only (pending, collided and
+ * updated are unchanged). */
 enum {
     VF_BRICK_16x8x4 = 0, /* 512 threads (default) */
     VF_BRICK_16x4x4 = 1, /* 256 threads */
@@ -371,10 +382,12 @@ enum {
     VF_BRICK_32x4x4 = 4, /* 512 threads */
     VF_BRICK_SHAPES = 5
 };
+enum { VF_TUNE_NO_SCREEN = 1 };
 typedef struct vf_tuning {
     int32_t brick_shape;
     int32_t wg_per_cu;
     float ms_per_frame;
+    int32_t flags;            /* VF_TUNE_NO_SCREEN: sweep every brick with a node list */
 } vf_tuning;
 
 /* Tuning used by later vf_warped_integrate* calls on this workspace. */
@@ -386,7 +399,7 @@ int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning);
 /* A plan fixes a batch of volumes and the device buffers of their graphs and
  * frames; their contents (depth, transforms, node positions, tsdf) may change
  * between launches, the pointers may not.  The plan owns its workspace, and
- * with VF_PLAN_GRAPH it records its four kernels into a HIP graph on the
+ * with VF_PLAN_GRAPH it records its kernels into a HIP graph on the
  * first launch and replays that graph afterwards (the kernels read the frame
  * parity from device memory, so the recorded launches stay valid).  Creating a
  * plan uploads its descriptors on `stream` and waits for them.  A plan's
@@ -412,6 +425,106 @@ int vf_plan_get_tuning(const vf_plan* plan, vf_tuning* tuning);
  * (either pointer may be NULL); blocks on `stream`. */
 int vf_plan_stats(vf_plan* plan, vf_frame_stats* last, vf_frame_stats* totals, int reset, vf_stream stream);
 uint64_t vf_plan_bytes(const vf_plan* plan);
+
+/* ------------------------------------------------------------------------ */
+/* Gauss-Newton motion optimisation                                         */
+/* ------------------------------------------------------------------------ */
+/* One ICP iteration of the reference (src/VolumeFusion.cu:434-692): residuals
+ * of point-to-plane correspondences and of the regularisation graph, the
+ * normal equations JTJ x = JTr, their LU solution and the first-order twist
+ * update T <- T - [w]x|t of every node.  Vectors are vf_float3, transforms
+ * column-major vf_mat4, updates float[6] per node (w0 w1 w2 t0 t1 t2).
+ * The reference's kernels one by one:                                      */
+
+/* af::runCorrespondencePairsKernel (include/af/ComputeCorrespondence.cuh:18-26,
+ * src/ComputeCorrespondence.cu:46-82): mesh point i pairs with pixel
+ * (int(x/z fx + cx), int(y/z fy + cy)) of a depth_dim.x-row, depth_dim.y-column
+ * frame when the frame vertex there is within thresh and its depth above
+ * 1e-4.  Pairs (i, pixel) are appended after *corr_size (device counter),
+ * which advances by their count; they come in ascending i (the reference
+ * appends them in the order of an atomic race).  Uses the workspace's
+ * scratch (NULL: the device's default workspace). */
+int vf_correspondence_pairs(vf_workspace* ws, vf_int2* corr, uint32_t* corr_size, const vf_float3* warped,
+                            size_t mesh_size, vf_mat3 intrinsics, const float* depth, vf_int2 depth_dim,
+                            const vf_float3* frame_mesh, float thresh, vf_stream stream);
+/* af::runFill_r_DepthKernel (include/af/MotionOptimization.cuh:34-40):
+ * r[c] = weight * n.dot(warped[corr.x] - depth_mesh[corr.y]). */
+int vf_fill_r_depth(float* r, float weight, const vf_int2* corr, uint32_t corr_size, const vf_float3* warped,
+                    const vf_float3* depth_mesh, const vf_float3* normals, vf_stream stream);
+/* af::runFill_r_MRegKernel (:42-47): r[3 (g reg_k + i) + a] =
+ * ((T_g p_g) - (T_nb p_g))[a] * weight for nb = graph_knn[g reg_k + i] < G
+ * (rows of absent neighbours untouched). */
+int vf_fill_r_mreg(float* r, float weight, const vf_float3* graph, const vf_mat4* transforms, uint32_t graph_size,
+                   const uint32_t* graph_knn, int reg_k, vf_stream stream);
+/* af::runFill_J_DepthKernel and af::runFill_J_MRegKernel (:11-32): the
+ * reference's dense Jacobian, j_height rows of j_width float[6] (zeroed by
+ * the caller); k <= 8 mesh knn slots, reg_k <= 8 neighbours. */
+int vf_fill_j_depth(float* J, float weight, uint32_t j_height, uint32_t j_width, const vf_int2* corr,
+                    uint32_t corr_size, const vf_float3* canon, int k, const uint32_t* knn_idxs,
+                    const float* knn_weights, const vf_float3* depth_mesh, const vf_float3* normals,
+                    vf_stream stream);
+int vf_fill_j_mreg(float* J, float weight, uint32_t j_height, uint32_t j_width, const vf_float3* graph,
+                   const vf_mat4* transforms, uint32_t graph_size, const uint32_t* graph_knn, int reg_k,
+                   vf_stream stream);
+/* af::runUpdateTransformsKernel (:88): T <- T - U, U = [[0 -w2 w1 t0];
+ * [w2 0 -w0 t1]; [-w1 w0 0 t2]; [0 0 0 0]]. */
+int vf_update_transforms(vf_mat4* transforms, const float* updates, uint32_t graph_size, vf_stream stream);
+/* af::linearSolverLUStable (include/af/LinearSolver.cuh:20, src/LinearSolver.cu:179-225):
+ * x = A^-1 b by LU with partial pivoting (rocSOLVER getrf, getrs) on a copy
+ * of the column-major n x n A; VF_ERR_SINGULAR on a zero pivot.  Blocks. */
+int vf_lu_solve(int n, const float* A, int lda, const float* b, float* x, vf_stream stream);
+
+/* The sparse description of one ICP iteration's least-squares problem. */
+typedef struct vf_gn_problem {
+    const vf_int2* corr;        /* (mesh index, frame pixel) pairs         */
+    uint32_t corr_size;
+    const vf_float3* canon;     /* canonical mesh                          */
+    int k;                      /* mesh knn slots, 1..8 (reference: 6)     */
+    const uint32_t* knn_idxs;   /* [mesh][k] nearest nodes                  */
+    const float* knn_weights;   /* [mesh][k] normalised weights             */
+    const vf_float3* normals;   /* frame normals, by pixel                  */
+    const vf_float3* graph;     /* node positions                           */
+    uint32_t graph_size;
+    const uint32_t* graph_knn;  /* [G][reg_k] regularisation neighbours, >= G: none */
+    int reg_k;                  /* 1..8 (reference: 4)                      */
+    float weight_depth;         /* settings.energyWeightDepth               */
+    float weight_reg;           /* settings.energyWeightMReg                */
+} vf_gn_problem;
+
+/* The normal equations the reference assembles from its contributing pairs
+ * (src/MotionOptimization.cu:227-479 with the sort and reduce_by_key of
+ * src/VolumeFusion.cu:510-613), built here from the sparse rows without a
+ * dense J: JTJ (6G x 6G) is the identity except for the 6x6 blocks of node
+ * pairs that share a row, each summed over their common rows in ascending
+ * row order (depth rows first, then the 3 reg_k rows of each node in order);
+ * JTr (G x 6) likewise from r (depth rows, then regularisation rows).  Either
+ * output may be NULL.  Blocks. */
+int vf_gn_normal_equations(vf_workspace* ws, float* JTJ, float* JTr, const float* r, const vf_gn_problem* problem,
+                           vf_stream stream);
+
+/* A solver for one ICP iteration: builds the rows and JTJ and factorises
+ * JTJ once (its value does not change during the Gauss-Newton iterations);
+ * the problem's buffers must stay alive and unchanged while it exists. */
+typedef struct vf_gn_solver vf_gn_solver;
+typedef struct vf_gn_report {
+    int iterations;       /* updates applied                               */
+    int stopped;          /* 1: the energy decrease fell below min_step    */
+    double final_energy;  /* energy of the last accepted transforms        */
+    double energy[32];    /* energy at the start of each iteration         */
+} vf_gn_report;
+int vf_gn_solver_create(vf_gn_solver** solver, const vf_gn_problem* problem, vf_stream stream);
+int vf_gn_solver_destroy(vf_gn_solver* solver);
+/* The Gauss-Newton loop of src/VolumeFusion.cu:541-691: pending starts as a
+ * copy of transforms; each iteration computes the residuals (depth rows from
+ * warped, the mesh warped by pending, and depth_mesh; regularisation rows
+ * from pending) and their energy (sum of squares in binary64, fixed order;
+ * the reference sums in float in thrust's order); stops when the energy drops
+ * by less than min_step; else commits pending to transforms (after the first
+ * iteration), solves JTJ x = JTr, updates pending and re-warps the mesh.
+ * Blocks; reads back one double per iteration. */
+int vf_gn_solver_iterate(vf_gn_solver* solver, vf_mat4* transforms, vf_mat4* pending, vf_float3* warped,
+                         const vf_float3* canon, size_t mesh_size, const vf_float3* depth_mesh, int max_iters,
+                         float min_step, vf_gn_report* report, vf_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: resident reconstruction streams, continuous batching            */

@@ -114,7 +114,8 @@ def lib() -> C.CDLL:
         L.vfo_warped_frame.restype = C.c_int
         L.vfo_warped_frame_gather.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_size_t,
                                               C.POINTER(Vol), _i32p, _f32p, _f32p, C.c_int, C.c_int, C.c_float,
-                                              _f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(Stats), C.c_int]
+                                              _f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(Stats), C.c_int,
+                                              C.c_void_p, C.c_void_p]
         L.vfo_warped_frame_gather.restype = C.c_int
         L.vfo_warped_frame_f64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_size_t, C.POINTER(Vol),
                                            _f32p, _f32p, C.c_int, C.c_float, C.c_float, _i32p, _u32p, _i32p,
@@ -138,6 +139,24 @@ def lib() -> C.CDLL:
         L.vfo_mat4_to_dq_f64.argtypes = [_f32p, _f64p]
         L.vfo_dqb_apply.argtypes = [C.POINTER(C.c_void_p), _f32p, C.c_int, _f32p, _f32p]
         L.vfo_dqb_apply_f64.argtypes = [C.POINTER(C.c_void_p), _f64p, C.c_int, _f64p, _f64p]
+        L.vfo_corr_pairs.argtypes = [_i32p, _f32p, C.c_size_t, _f32p, _f32p, C.c_int, C.c_int, _f32p, C.c_float]
+        L.vfo_corr_pairs.restype = C.c_int64
+        L.vfo_gn_r_depth.argtypes = [_f32p, C.c_float, _i32p, C.c_size_t, _f32p, _f32p, _f32p]
+        L.vfo_gn_r_depth.restype = None
+        L.vfo_gn_r_mreg.argtypes = [_f32p, C.c_float, _f32p, _f32p, C.c_size_t, _u32p, C.c_int]
+        L.vfo_gn_r_mreg.restype = None
+        L.vfo_gn_j_depth.argtypes = [_f32p, C.c_float, C.c_size_t, C.c_size_t, _i32p, _f32p, _u32p, _f32p, C.c_int,
+                                     _f32p]
+        L.vfo_gn_j_depth.restype = None
+        L.vfo_gn_j_mreg.argtypes = [_f32p, C.c_float, _f32p, C.c_size_t, _u32p, C.c_int]
+        L.vfo_gn_j_mreg.restype = None
+        L.vfo_gn_normal_eq.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_size_t, C.c_size_t, _i32p, _u32p, _f32p, C.c_int,
+                                       _u32p, C.c_int]
+        L.vfo_gn_normal_eq.restype = C.c_int
+        L.vfo_gn_update.argtypes = [_f32p, _f32p, C.c_size_t]
+        L.vfo_gn_update.restype = None
+        L.vfo_lu_solve.argtypes = [C.c_int, _f32p, _f32p, _f64p]
+        L.vfo_lu_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -439,9 +458,18 @@ def warped_frame(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, de
 
 
 def warped_frame_gather(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, depth, delta, Kin, maxw,
-                        nthreads=None, z0=0, z1=0, blend=0) -> dict:
+                        nthreads=None, z0=0, z1=0, blend=0, tsdf64=None, wt64=None) -> dict:
     """Voxel-major multi-threaded restatement of the same frame (sources in
-    z in [z0, z1); z1 <= 0 = whole volume).  Updates tsdf, wt, nodes in place."""
+    z in [z0, z1); z1 <= 0 = whole volume).  Updates tsdf, wt, nodes in place.
+    With tsdf64/wt64 (float64 arrays) it also advances the binary64 companion
+    state: the same discrete decisions, continuous values in binary64 (the
+    equal of warped_frame_f64)."""
+    if (tsdf64 is None) != (wt64 is None):
+        raise ValueError("tsdf64 and wt64 go together")
+    if tsdf64 is not None:
+        n = int(dim[0]) * int(dim[1]) * int(dim[2])
+        assert tsdf64.dtype == np.float64 and wt64.dtype == np.float64 and tsdf64.size == n and wt64.size == n
+        assert tsdf64.flags.c_contiguous and wt64.flags.c_contiguous
     if nthreads is None:
         nthreads = os.cpu_count() or 1
     st = Stats()
@@ -451,7 +479,9 @@ def warped_frame_gather(k, tsdf, wt, nodes, radii, T, dim, vs, size, cube, c, de
     rc = lib().vfo_warped_frame_gather(k, tsdf, wt, nodes, _f32(radii), _f32(T), nodes.shape[0], C.byref(v),
                                        np.ascontiguousarray(cube, np.int32), _f32(c), _f32(depth), W, H,
                                        float(delta), _f32(Kin), float(maxw), int(nthreads), int(z0), int(z1),
-                                       C.byref(st), int(blend))
+                                       C.byref(st), int(blend),
+                                       tsdf64.ctypes.data if tsdf64 is not None else None,
+                                       wt64.ctypes.data if wt64 is not None else None)
     if rc != 0:
         raise RuntimeError(f"oracle gather frame failed ({rc})")
     return st.as_dict()
@@ -465,3 +495,80 @@ def warped_frame_f64(k, tsdf64, wt64, nodes_before, radii, T, dim, vs, size, c, 
                                C.byref(v), _f32(c), _f32(depth), W, float(delta), float(maxw),
                                np.ascontiguousarray(cnt, np.int32), np.ascontiguousarray(idx, np.uint32),
                                np.ascontiguousarray(pix, np.int32), int(blend))
+
+
+# ---- Gauss-Newton motion optimisation (vf_gn.c) ------------------------------
+
+def corr_pairs(warped, Kin, depth, frame_mesh, thresh) -> np.ndarray:
+    """correspondencePairsKernel (src/ComputeCorrespondence.cu:46-82), pairs in
+    ascending mesh index: int32 [C, 2] of (mesh index, pixel index)."""
+    warped = _f32(warped)
+    M = warped.shape[0]
+    H, W = depth.shape
+    out = np.zeros((max(M, 1), 2), np.int32)
+    n = lib().vfo_corr_pairs(out, warped, M, _f32(Kin), _f32(depth), H, W, _f32(frame_mesh), float(thresh))
+    return out[:n].copy()
+
+
+def gn_r_depth(wD, corr, warped, dmesh, dnormals) -> np.ndarray:
+    corr = np.ascontiguousarray(corr, np.int32)
+    r = np.zeros(max(corr.shape[0], 1), np.float32)
+    lib().vfo_gn_r_depth(r, float(wD), corr, corr.shape[0], _f32(warped), _f32(dmesh), _f32(dnormals))
+    return r[:corr.shape[0]]
+
+
+def gn_r_mreg(wR, graph, T, gknn) -> np.ndarray:
+    gknn = np.ascontiguousarray(gknn, np.uint32)
+    G, R = gknn.shape
+    r = np.zeros(max(G * R * 3, 1), np.float32)
+    lib().vfo_gn_r_mreg(r, float(wR), _f32(graph), _f32(T), G, gknn, R)
+    return r[:G * R * 3]
+
+
+def gn_j_depth(wD, corr, canon, knn, kw, dnormals, G) -> np.ndarray:
+    corr = np.ascontiguousarray(corr, np.int32)
+    knn = np.ascontiguousarray(knn, np.uint32)
+    C = corr.shape[0]
+    J = np.zeros((max(C, 1), G, 6), np.float32)
+    lib().vfo_gn_j_depth(J, float(wD), C, G, corr, _f32(canon), knn, _f32(kw), knn.shape[1], _f32(dnormals))
+    return J[:C]
+
+
+def gn_j_mreg(wR, graph, gknn) -> np.ndarray:
+    gknn = np.ascontiguousarray(gknn, np.uint32)
+    G, R = gknn.shape
+    J = np.zeros((max(G * R * 3, 1), G, 6), np.float32)
+    lib().vfo_gn_j_mreg(J, float(wR), _f32(graph), G, gknn, R)
+    return J[:G * R * 3]
+
+
+def gn_normal_eq(J, r, corr, knn, kw, gknn):
+    """JTJ (6G x 6G) and JTr (G x 6) from the dense J (depth rows, then the
+    3 R rows of each node) and r."""
+    gknn = np.ascontiguousarray(gknn, np.uint32)
+    knn = np.ascontiguousarray(knn, np.uint32)
+    corr = np.ascontiguousarray(corr, np.int32)
+    G, R = gknn.shape
+    C = corr.shape[0]
+    JTJ = np.zeros((6 * G, 6 * G), np.float32)
+    JTr = np.zeros((G, 6), np.float32)
+    rc = lib().vfo_gn_normal_eq(JTJ, JTr, _f32(J), _f32(r), C, G, corr if C else np.zeros((1, 2), np.int32), knn,
+                                _f32(kw), knn.shape[1], gknn, R)
+    if rc != 0:
+        raise MemoryError("oracle normal equations allocation failed")
+    return JTJ, JTr
+
+
+def gn_update(T, u) -> np.ndarray:
+    T = _f32(T).copy()
+    lib().vfo_gn_update(T, _f32(u), T.shape[0])
+    return T
+
+
+def lu_solve(A, b):
+    """LU with partial pivoting in binary64: (x, info)."""
+    A = _f32(A)
+    n = A.shape[0]
+    x = np.zeros(max(n, 1), np.float64)
+    info = lib().vfo_lu_solve(n, A, _f32(b), x)
+    return x[:n], int(info)

@@ -1116,6 +1116,11 @@ int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* r
  * Sources are restricted to z in [z0, z1) (z1 <= 0 means the whole volume);
  * collisions are counted among those sources only. */
 
+static void f64_warped_point(const geom_t* g, const vfo_vol* vol, int64_t x, int64_t y, int64_t z, int n,
+                             const uint32_t* il, const float* nodes, const float* radii, const float* T,
+                             const float* c, int blend, double acc[3]);
+static void f64_update(double* tsdf64, double* wt64, size_t v, double z, float d, float delta, float maxw);
+
 typedef struct {
     float m[3];
     float r2;       /* 2r, the candidate bound          */
@@ -1147,6 +1152,13 @@ typedef struct {
     const float* Kin;
     int nthreads, tid;
     int64_t supported, in_bounds, collided, updated, candidates, overflow;
+    /* binary64 companion (optional): the frame's positions before the
+     * translations, the warped z of each in-bounds voxel, the state */
+    const vfo_vol* vol;
+    const float* nodes0;
+    double* z64;
+    double* tsdf64;
+    double* wt64;
 } gwork_t;
 
 static void* gather_phase1(void* arg) {
@@ -1221,6 +1233,12 @@ static void* gather_phase1(void* arg) {
                 wk->in_bounds++;
                 wk->sup[v] = 1;
                 wk->tgt[v] = (uint32_t)u;
+                if (wk->z64) {
+                    double acc64[3];
+                    f64_warped_point(g, wk->vol, x, y, z, n, il, wk->nodes0, wk->radii, wk->T, wk->c, wk->DQ != NULL,
+                                     acc64);
+                    wk->z64[v] = acc64[2];
+                }
                 wk->pw[3 * v + 0] = acc[0];
                 wk->pw[3 * v + 1] = acc[1];
                 wk->pw[3 * v + 2] = acc[2];
@@ -1244,8 +1262,13 @@ static void* gather_phase2(void* arg) {
                     wk->collided++;
                     continue;
                 }
-                wk->updated += update_tsdf(wk->tsdf, wk->wt, wk->pw + 3 * v, v, wk->depth, wk->W, wk->H, wk->delta,
-                                           wk->Kin, wk->maxw) >= 0;
+                const int64_t pi = update_tsdf(wk->tsdf, wk->wt, wk->pw + 3 * v, v, wk->depth, wk->W, wk->H,
+                                               wk->delta, wk->Kin, wk->maxw);
+                if (pi >= 0) {
+                    wk->updated++;
+                    if (wk->tsdf64)
+                        f64_update(wk->tsdf64, wk->wt64, v, wk->z64[v], wk->depth[pi], wk->delta, wk->maxw);
+                }
             }
     return NULL;
 }
@@ -1253,7 +1276,7 @@ static void* gather_phase2(void* arg) {
 int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T,
                             size_t G, const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth,
                             int W, int H, float delta, const float* Kin, float maxw, int nthreads, int z0, int z1,
-                            vfo_stats* st, int blend) {
+                            vfo_stats* st, int blend, double* tsdf64, double* wt64) {
     if (K < 1 || K > 64)
         return -2;
     geom_t g;
@@ -1274,8 +1297,9 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
     uint32_t* tgt = (uint32_t*)malloc(sizeof(uint32_t) * N);
     int32_t* col = (int32_t*)calloc(N, sizeof(int32_t));
     float* DQ = blend ? (float*)malloc(sizeof(float) * 8 * (G ? G : 1)) : NULL;
-    if (!gn || !boff || !pw || !sup || !tgt || !col || (blend && !DQ)) {
-        free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ);
+    double* z64 = tsdf64 ? (double*)malloc(sizeof(double) * (N ? N : 1)) : NULL;
+    if (!gn || !boff || !pw || !sup || !tgt || !col || (blend && !DQ) || (tsdf64 && !z64)) {
+        free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ); free(z64);
         return -1;
     }
     for (size_t j = 0; blend && j < G; ++j)
@@ -1339,6 +1363,7 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
                 w->z0 = z0; w->z1 = z1; w->pw = pw; w->sup = sup; w->tgt = tgt; w->col = col;
                 w->tsdf = tsdf; w->wt = wt; w->depth = depth; w->W = W; w->H = H; w->delta = delta; w->maxw = maxw;
                 w->Kin = Kin; w->nthreads = nthreads; w->tid = t;
+                w->vol = vol; w->nodes0 = nodes; w->z64 = z64; w->tsdf64 = tsdf64; w->wt64 = wt64;
             }
             for (int t = 0; t < nthreads; ++t)
                 pthread_create(th + t, NULL, gather_phase1, wk + t);
@@ -1369,7 +1394,7 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
     for (size_t j = 0; j < G; ++j)
         for (int a = 0; a < 3; ++a)
             nodes[3 * j + a] = gn[j].m[a] + c[a];
-    free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ);
+    free(gn); free(boff); free(pw); free(sup); free(tgt); free(col); free(DQ); free(z64);
     return 0;
 }
 
@@ -1381,11 +1406,71 @@ int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const f
  * distances, Gaussian weights, the blended position and the running average.
  * tsdf64/wt64 carry the binary64 state across frames.  nodes are the
  * positions before the frame's translations. */
+/* The binary64 warped position of voxel (x, y, z) with its n selected nodes
+ * il[0..n): node offsets, squared distances, Gaussian weights, normalised
+ * blend (LBS over Mat4f or DQB) in binary64. */
+static void f64_warped_point(const geom_t* g, const vfo_vol* vol, int64_t x, int64_t y, int64_t z, int n,
+                             const uint32_t* il, const float* nodes, const float* radii, const float* T,
+                             const float* c, int blend, double acc[3]) {
+    double p[3];
+    p[0] = (double)x * (double)g->vs[0] - 0.5 * (double)vol->size[0];
+    p[1] = (double)y * (double)g->vs[1] - 0.5 * (double)vol->size[1];
+    p[2] = (double)z * (double)g->vs[2] - 0.5 * (double)vol->size[2];
+    double w[64], sum = 0.0;
+    for (int s = 0; s < n; ++s) {
+        const uint32_t j = il[s];
+        double d2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double dd = p[a] - ((double)nodes[3 * j + a] - (double)c[a]);
+            d2 += dd * dd;
+        }
+        const double r = (double)radii[j];
+        w[s] = exp(-d2 / (2.0 * r * r));
+        sum += w[s];
+    }
+    double pc[3];
+    for (int a = 0; a < 3; ++a) {
+        pc[a] = p[a] + (double)c[a];
+        acc[a] = 0.0;
+    }
+    if (blend) {
+        double dqs[64][8], wn[64];
+        const double* q[64];
+        for (int s = 0; s < n; ++s) {
+            vfo_mat4_to_dq_f64(T + 16 * (size_t)il[s], dqs[s]);
+            q[s] = dqs[s];
+            wn[s] = w[s] / sum;
+        }
+        vfo_dqb_apply_f64(q, wn, n, pc, acc);
+        return;
+    }
+    for (int s = 0; s < n; ++s) {
+        const float* M = T + 16 * (size_t)il[s];
+        for (int a = 0; a < 3; ++a) {
+            const double t = (double)M[0 * 4 + a] * pc[0] + (double)M[1 * 4 + a] * pc[1] +
+                             (double)M[2 * 4 + a] * pc[2] + (double)M[3 * 4 + a];
+            acc[a] += (w[s] / sum) * t;
+        }
+    }
+}
+
+/* The binary64 running average of src/TSDFIntegration.cu:46-64 for a voxel
+ * the float frame updated from depth pixel value d. */
+static void f64_update(double* tsdf64, double* wt64, size_t v, double z, float d, float delta, float maxw) {
+    const double sdf = z - (double)d;
+    const double dl = (double)delta;
+    const double tn = sdf < -dl ? -1.0 : sdf / dl;
+    const double wo = wt64[v];
+    tsdf64[v] = (tsdf64[v] * wo + tn) / (wo + 1.0);
+    wt64[v] = (wo + 1.0) < (double)maxw ? (wo + 1.0) : (double)maxw;
+}
+
 int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes, const float* radii,
                          const float* T, size_t G, const vfo_vol* vol, const float* c, const float* depth, int W,
                          float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix,
                          int blend) {
     (void)G;
+    (void)W;
     geom_t g;
     geom_init(&g, vol);
     for (int64_t z = 0; z < g.dim[2]; ++z)
@@ -1395,51 +1480,10 @@ int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes
                 if (pix[v] < 0 || !cnt[v])
                     continue;
                 const int n = cnt[v] > K ? K : cnt[v];
-                double p[3];
-                p[0] = (double)x * (double)g.vs[0] - 0.5 * (double)vol->size[0];
-                p[1] = (double)y * (double)g.vs[1] - 0.5 * (double)vol->size[1];
-                p[2] = (double)z * (double)g.vs[2] - 0.5 * (double)vol->size[2];
-                double w[64], sum = 0.0;
-                for (int s = 0; s < n; ++s) {
-                    const uint32_t j = idx[(size_t)K * v + s];
-                    double d2 = 0.0;
-                    for (int a = 0; a < 3; ++a) {
-                        const double dd = p[a] - ((double)nodes[3 * j + a] - (double)c[a]);
-                        d2 += dd * dd;
-                    }
-                    const double r = (double)radii[j];
-                    w[s] = exp(-d2 / (2.0 * r * r));
-                    sum += w[s];
-                }
-                double pc[3], acc[3] = {0.0, 0.0, 0.0};
-                for (int a = 0; a < 3; ++a)
-                    pc[a] = p[a] + (double)c[a];
-                if (blend) {
-                    double dqs[64][8], wn[64];
-                    const double* q[64];
-                    for (int s = 0; s < n; ++s) {
-                        vfo_mat4_to_dq_f64(T + 16 * (size_t)idx[(size_t)K * v + s], dqs[s]);
-                        q[s] = dqs[s];
-                        wn[s] = w[s] / sum;
-                    }
-                    vfo_dqb_apply_f64(q, wn, n, pc, acc);
-                }
-                for (int s = 0; s < n && !blend; ++s) {
-                    const float* M = T + 16 * (size_t)idx[(size_t)K * v + s];
-                    for (int a = 0; a < 3; ++a) {
-                        const double t = (double)M[0 * 4 + a] * pc[0] + (double)M[1 * 4 + a] * pc[1] +
-                                         (double)M[2 * 4 + a] * pc[2] + (double)M[3 * 4 + a];
-                        acc[a] += (w[s] / sum) * t;
-                    }
-                }
-                const double sdf = acc[2] - (double)depth[pix[v]];
-                const double dl = (double)delta;
-                const double tn = sdf < -dl ? -1.0 : sdf / dl;
-                const double wo = wt64[v];
-                tsdf64[v] = (tsdf64[v] * wo + tn) / (wo + 1.0);
-                wt64[v] = (wo + 1.0) < (double)maxw ? (wo + 1.0) : (double)maxw;
+                double acc[3];
+                f64_warped_point(&g, vol, x, y, z, n, idx + (size_t)K * v, nodes, radii, T, c, blend, acc);
+                f64_update(tsdf64, wt64, v, acc[2], depth[pix[v]], delta, maxw);
             }
-    (void)W;
     return 0;
 }
 

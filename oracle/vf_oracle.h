@@ -48,7 +48,9 @@
  *   vfo_warped_frame      equals the independently structured voxel-major
  *                         vfo_warped_frame_gather over several frames; every
  *                         stage above
- *   vfo_warped_frame_f64  reporting only (max/RMS error of the float path)
+ *   vfo_warped_frame_f64  reporting only (max/RMS error of the float path);
+ *                         vfo_warped_frame_gather's binary64 companion
+ *                         (tsdf64/wt64 not NULL) equals it bit for bit
  *   vfo_mat4_to_dq, vfo_dqb_apply (DQB, no reference counterpart)
  *                         closed forms: identity -> p bit for bit; one rigid
  *                         node -> R p + t; identity and a rotation by theta,
@@ -176,12 +178,26 @@ int vfo_warped_frame(int K, float* tsdf, float* wt, float* nodes, const float* r
 int vfo_warped_frame_gather(int K, float* tsdf, float* wt, float* nodes, const float* radii, const float* T,
                             size_t G, const vfo_vol* vol, const int32_t* cube, const float* c, const float* depth,
                             int W, int H, float delta, const float* Kin, float maxw, int nthreads, int z0, int z1,
-                            vfo_stats* st, int blend);
+                            vfo_stats* st, int blend, double* tsdf64, double* wt64);
 
 int vfo_warped_frame_f64(int K, double* tsdf64, double* wt64, const float* nodes, const float* radii,
                          const float* T, size_t G, const vfo_vol* vol, const float* c, const float* depth, int W,
                          float delta, float maxw, const int32_t* cnt, const uint32_t* idx, const int32_t* pix,
                          int blend);
+
+/* ---- Gauss-Newton motion optimisation (vf_gn.c) ---------------------------- */
+int64_t vfo_corr_pairs(int32_t* corr, const float* warped, size_t M, const float* Kin, const float* depth, int rows,
+                       int cols, const float* frame_mesh, float thresh);
+void vfo_gn_r_depth(float* r, float wD, const int32_t* corr, size_t C, const float* warped, const float* dmesh,
+                    const float* dnormals);
+void vfo_gn_r_mreg(float* r, float wR, const float* graph, const float* T, size_t G, const uint32_t* gknn, int R);
+void vfo_gn_j_depth(float* J, float wD, size_t C, size_t G, const int32_t* corr, const float* canon,
+                    const uint32_t* knn, const float* kw, int K, const float* dnormals);
+void vfo_gn_j_mreg(float* J, float wR, const float* graph, size_t G, const uint32_t* gknn, int R);
+int vfo_gn_normal_eq(float* JTJ, float* JTr, const float* J, const float* r, size_t C, size_t G, const int32_t* corr,
+                     const uint32_t* knn, const float* kw, int K, const uint32_t* gknn, int R);
+void vfo_gn_update(float* T, const float* u, size_t G);
+int vfo_lu_solve(int n, const float* A, const float* b, double* x);
 
 #ifdef __cplusplus
 }

@@ -31,3 +31,16 @@ def vf():
     n = device.device_count()
     assert n >= 1, "no HIP device visible: GPU tests need an MI355X"
     return L
+
+
+def pytest_terminal_summary(terminalreporter):
+    """The binary64 error report of the GPU parity tests (device tsdf against
+    the oracle's binary64 companion, same discrete decisions)."""
+    try:
+        import fp64_report
+    except ImportError:
+        return
+    if fp64_report.RESULTS:
+        terminalreporter.section("binary64 error of the device tsdf")
+        for line in fp64_report.lines():
+            terminalreporter.write_line(line)

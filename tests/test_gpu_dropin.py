@@ -262,6 +262,30 @@ def test_reference_sequence_chained(vf, oracle):
     np.testing.assert_array_equal(g_nodes.download(), nodes_or)
 
 
+def test_plain_device_vs_cpu_tsdfvolume_path(vf, oracle):
+    """The plain-method workload (sphere128: one static-camera frame into a
+    128^3 volume) on the device kernel against the reference's CPU path,
+    TSDFVolume::integrate (src/TSDFVolume.cpp:55-171, restated in the oracle),
+    with the reference's own assertion for this pair
+    (test/src/TSDFIntegrationTest.cu:76-79): every tsdf within 4 ulps
+    (ASSERT_FLOAT_EQ), every weight equal."""
+    s = scenes.sphere128()
+    depth = s.depth(0)
+    n = s.n_voxels
+    c = np.array(s.center, F)
+    pose = np.eye(4, dtype=F)
+    pose[:3, 3] = c
+    tc, wc, uc = oracle.tsdfvolume_integrate(np.full(n, -1, F), np.zeros(n, F), depth, s.delta, s.dim, s.voxel_size,
+                                             s.size, np.ascontiguousarray(pose.T.reshape(16)), s.intrinsics())
+    tsdf = ops.DeviceTsdf(s.size, s.dim, center=c, delta=s.delta)
+    tsdf.integrate(dev(depth), s.intrinsics(), 15.0)
+    td, wd = tsdf.tsdf.download(), tsdf.weights.download()
+    np.testing.assert_array_equal(wd, wc)
+    ulps = np.abs(td.view(np.int32).astype(np.int64) - tc.view(np.int32))
+    assert ulps.max() <= 4
+    assert uc > 10000 and int((wd > 0).sum()) == uc
+
+
 def test_integrate_plain_and_camera_frame(vf, oracle):
     s = scenes.sphere128()
     depth = s.depth(0)

@@ -1,4 +1,4 @@
-"""GPU parity of the fused four-kernel frame (and its batched form) against
+"""GPU parity of the fused five-kernel frame (and its batched form) against
 the oracle's frame, bit for bit, over several frames."""
 import numpy as np
 import pytest
@@ -14,20 +14,37 @@ def dev(a):
     return DeviceArray.from_numpy(np.ascontiguousarray(a))
 
 
-def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None, blend=0):
+def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None, blend=0,
+               screen=True, depth_fn=None, transforms_fn=None, f64=False):
+    """Frames through the fused call against the oracle, bit for bit after
+    every frame.  With the brick screen off the supported and in-bounds
+    counts equal the oracle's too; with it they cover the swept bricks only.
+    With f64 the gather oracle also carries the binary64 companion state, and
+    the device tsdf's max and RMS error against it (in units of the
+    truncation distance: the stored tsdf is sdf / delta) is checked after
+    every frame and reported.  Returns the last frame's (oracle stats, device
+    stats)."""
     n = s.n_voxels
     t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+    t64 = w64 = None
+    if f64:
+        gather = True
+        t64, w64 = np.full(n, -1.0), np.zeros(n)
+    errs = []
     tsdf = ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta)
-    graph = ops.DeviceMotionGraph(s.nodes, s.transforms(frames[0]), s.radii)
+    graph = ops.DeviceMotionGraph(s.nodes, transforms_fn(frames[0]) if transforms_fn else s.transforms(frames[0]),
+                                  s.radii)
     ws = ops.Workspace()
-    if tuning is not None:
-        ws.set_tuning(*tuning)
+    tuning = tuple(tuning) if tuning is not None else (0, 0)
+    ws.set_tuning(tuning[0], tuning[1], screen)
+    gs = None
     for f in frames:
-        depth, T = s.depth(f), s.transforms(f)
+        depth = depth_fn(f) if depth_fn else s.depth(f)
+        T = transforms_fn(f) if transforms_fn else s.transforms(f)
         if gather:
             st = oracle.warped_frame_gather(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size,
                                             s.cube, s.center, depth, s.delta, s.intrinsics(), s.max_weight,
-                                            nthreads=threads, blend=blend)
+                                            nthreads=threads, blend=blend, tsdf64=t64, wt64=w64)
         else:
             st = oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, T, s.dim, s.voxel_size, s.size, s.cube,
                                      s.center, depth, s.delta, s.intrinsics(), s.max_weight, blend=blend).stats
@@ -36,20 +53,38 @@ def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True
                              blend=blend)
         gs = ws.last_stats()
         if check_stats:
-            assert gs["supported"] == st["supported"]
-            assert gs["in_bounds"] == st["in_bounds"]
+            if not screen:
+                assert gs["supported"] == st["supported"]
+                assert gs["in_bounds"] == st["in_bounds"]
+                assert gs["screened_bricks"] == 0
             assert gs["updated"] == st["updated"]
         np.testing.assert_array_equal(tsdf.weights.download(), w_or, err_msg=f"weights, frame {f}")
         np.testing.assert_array_equal(tsdf.tsdf.download(), t_or, err_msg=f"tsdf, frame {f}")
         np.testing.assert_array_equal(graph.points.download(), nodes_or, err_msg=f"nodes, frame {f}")
-    return st
+        if f64:
+            td = tsdf.tsdf.download()
+            np.testing.assert_array_equal(w64.astype(F), w_or, err_msg=f"binary64 weights, frame {f}")
+            m = w_or > 0
+            e = np.abs(td[m].astype(np.float64) - t64[m])
+            emax, erms = float(e.max()), float(np.sqrt(np.mean(e * e)))
+            errs.append((emax, erms))
+            assert erms <= 1e-5, f"frame {f}: RMS |tsdf - tsdf64| = {erms:.3g} above 1e-5 of delta"
+    if f64:
+        from fp64_report import record
+
+        record(s.name, len(frames), max(x[0] for x in errs), errs[-1][1], max(x[1] for x in errs),
+               int((w_or > 0).sum()))
+    return st, gs
 
 
+@pytest.mark.parametrize("screen", [True, False])
 @pytest.mark.parametrize("k", [1, 4, 6, 8])
-def test_fused_small_multi_frame(vf, oracle, k):
+def test_fused_small_multi_frame(vf, oracle, k, screen):
     s = scenes.small_scene(seed=11, k=k, nodes=70)
-    st = run_frames(vf, oracle, s, [0, 1, 2, 3])
+    st, gs = run_frames(vf, oracle, s, [0, 1, 2, 3], screen=screen)
     assert st["updated"] > 500 and st["collided"] > 0
+    if screen:
+        assert gs["screened_bricks"] > 0, "the far side of the sphere is screened"
 
 
 def test_fused_spill_path(vf, oracle):
@@ -57,6 +92,110 @@ def test_fused_spill_path(vf, oracle):
     path inside the kernel."""
     s = scenes.small_scene(seed=12, k=8, nodes=400, radius=0.03)
     run_frames(vf, oracle, s, [0, 1])
+    run_frames(vf, oracle, s, [0, 1], screen=False)
+
+
+def _special_depth(s, f, nan):
+    """The frame with +-inf, zero, negative and too small pixels sprinkled
+    over it (all fail the projective test), and NaN pixels (which pass it:
+    their sdf compares false both ways): none, a block in the middle, or
+    sprinkled everywhere as well."""
+    d = s.depth(f).copy()
+    rng = np.random.default_rng([s.seed, 77, f])
+    H, W = d.shape
+    specials = ((np.inf, 0.01), (-np.inf, 0.01), (0.0, 0.01), (-0.5, 0.01), (5e-5, 0.01))
+    if nan == "everywhere":
+        specials = ((np.nan, 0.02),) + specials
+    for value, frac in specials:
+        m = rng.random(d.shape) < frac
+        d[m] = value
+    if nan != "none":
+        d[H // 2 - 6:H // 2 + 6, W // 2 - 6:W // 2 + 6] = np.nan
+    return d
+
+
+@pytest.mark.parametrize("nan", ["none", "block", "everywhere"])
+def test_fused_screen_special_depth_pixels(vf, oracle, nan):
+    """Brick screen with NaN, infinite, zero and negative depth pixels:
+    bit-exact against the oracle over four frames with the screen on.  The
+    depth-tile maxima must keep every brick a NaN pixel can make pending (a
+    NaN block in the middle keeps the bricks behind the sphere's centre and
+    whatever can share their targets); with NaN pixels all over the frame
+    every brick's footprint holds one, so the screen keeps everything; with
+    no NaN the screen drops bricks."""
+    s = scenes.small_scene(seed=15, k=8, nodes=80)
+    st, gs = run_frames(vf, oracle, s, [0, 1, 2, 3], depth_fn=lambda f: _special_depth(s, f, nan))
+    assert st["updated"] > 500
+    if nan == "everywhere":
+        assert gs["screened_bricks"] == 0
+    elif nan == "none":
+        assert gs["screened_bricks"] > 0
+
+
+@pytest.mark.parametrize("bad", ["nan_transform", "inf_transform", "zero_radius", "tiny_radius", "nan_node",
+                                 "inf_node"])
+def test_fused_screen_unbounded_node(vf, oracle, bad):
+    """A node whose displacement the screen cannot bound (non-finite
+    transform or position, a radius whose weights can be NaN): the screen
+    keeps every brick of the volume, and the frame stays bit-exact.  A node
+    at infinity has its cube outside the volume and binds nothing."""
+    s = scenes.small_scene(seed=16, k=8, nodes=60)
+    clean = scenes.small_scene(seed=16, k=8, nodes=60)  # warps are made from finite positions
+    j = 7
+    if bad == "zero_radius":
+        s.radii = s.radii.copy()
+        s.radii[j] = 0.0
+    elif bad == "tiny_radius":
+        s.radii = s.radii.copy()
+        s.radii[j] = 1e-30
+    elif bad in ("nan_node", "inf_node"):
+        s.nodes = s.nodes.copy()
+        s.nodes[j, 0] = np.nan if bad == "nan_node" else np.inf
+
+    def transforms(f):
+        T = clean.transforms(f).copy()
+        if bad == "nan_transform":
+            T[j, 12] = np.nan
+        elif bad == "inf_transform":
+            T[j, 0] = np.inf
+        return T
+
+    st, gs = run_frames(vf, oracle, s, [0, 1], transforms_fn=transforms)
+    if bad == "inf_node":
+        assert gs["screened_bricks"] > 0
+    else:
+        assert gs["screened_bricks"] == 0
+
+
+@pytest.mark.parametrize("radius", [1e20, 1e19, 1e-11])
+def test_fused_extreme_radius(vf, oracle, radius):
+    """A node radius whose 2r^2 is infinite, huge or tiny: the brick's weights
+    take IEEE divisions (exp(-d^2 / inf) = 1 where a precomputed reciprocal
+    would give NaN), bit-exact against the oracle; the screen keeps the whole
+    volume."""
+    s = scenes.small_scene(seed=19, k=8, nodes=60)
+    s.radii = s.radii.copy()
+    s.radii[5] = radius
+    st, gs = run_frames(vf, oracle, s, [0, 1])
+    assert st["updated"] > 500 and gs["screened_bricks"] == 0
+
+
+def test_fused_screen_volume_across_camera_plane(vf, oracle):
+    """A volume reaching behind the camera (z from -0.03 to 0.17 m) around a
+    small sphere in front of it: bricks near z = 0 have unbounded pixel
+    footprints and are kept, bricks behind the camera are dropped."""
+    import dataclasses
+
+    rng = np.random.default_rng([17, 9])
+    sph = scenes.make_surface(rng, (0.0, 0.0, 0.11), (0.045, 0.045, 0.045), eps=0.0, rot=np.eye(3))
+    nodes = scenes.poisson_disk([sph], 60, rng)
+    base = scenes.small_scene(seed=17, k=8, nodes=8)
+    dim = (40, 40, 40)
+    s = dataclasses.replace(base, dim=dim, size=(0.2, 0.2, 0.2), center=(0.0, 0.0, 0.07), nodes=nodes,
+                            radii=np.full(60, 0.015, F), graph_radius=0.015, surfaces=[sph],
+                            cam=scenes.Camera(160, 144, 126.0, 126.0, 80.0, 72.0))
+    st, gs = run_frames(vf, oracle, s, [0, 1, 2])
+    assert st["updated"] > 100
 
 
 def test_fused_empty_graph_leaves_volume(vf, oracle):
@@ -71,29 +210,38 @@ def test_fused_empty_graph_leaves_volume(vf, oracle):
 
 
 def test_fused_sphere128(vf, oracle):
-    """128^3, 256 nodes, identity transforms, k=4 (the CPU-runnable config)."""
+    """128^3, 256 nodes, identity transforms, k=4 (the CPU-runnable config):
+    its one frame with the screen and the binary64 error report, then two
+    frames without the screen (frame counters equal the oracle's)."""
     s = scenes.sphere128()
-    st = run_frames(vf, oracle, s, [0, 1])
-    assert st["updated"] > 10000
+    st, gs = run_frames(vf, oracle, s, [0], f64=True)
+    assert st["updated"] > 10000 and gs["screened_bricks"] > 0
+    run_frames(vf, oracle, s, [0, 1], screen=False)
 
 
 @pytest.mark.slow
 def test_fused_sphere256_warped_frames(vf, oracle):
-    """256^3, 2048 nodes with random warps, k=8, all 30 frames of the workload:
-    every voxel and node position checked bit for bit after every frame
-    against the gather oracle (itself equal to the node-major oracle)."""
+    """256^3, 2048 nodes with random warps, k=8, all 30 frames of the workload
+    in the benchmark's configuration: every voxel and node position checked
+    bit for bit after every frame against the gather oracle (itself equal to
+    the node-major oracle), and the binary64 error after every frame."""
     s = scenes.sphere256_warped()
-    run_frames(vf, oracle, s, list(range(30)), gather=True)
+    run_frames(vf, oracle, s, list(range(30)), tuning=(2, 5), f64=True)
 
 
-def test_fused_ellipsoids512_full_size(vf, oracle):
+@pytest.mark.parametrize("tuning,screen", [((0, 0), False), ((2, 5), True)])
+def test_fused_ellipsoids512_full_size(vf, oracle, tuning, screen):
     """The headline workload at its full size (512^3 at 2 mm, 4096 nodes, k=8,
     1024x1024 depth with noise and dropout), two frames, every voxel and node
-    position bit for bit against the gather oracle on 16 host threads, and
-    the frame counters equal."""
+    position bit for bit against the gather oracle on 16 host threads: the
+    default 16x8x4 bricks without the brick screen (frame counters equal
+    too), and the configuration the benchmark times (8x8x4 bricks, five
+    workgroups per CU, screen on)."""
     s = scenes.ellipsoids512()
-    st = run_frames(vf, oracle, s, [0, 1], gather=True, threads=16)
+    st, gs = run_frames(vf, oracle, s, [0, 1], gather=True, threads=16, tuning=tuning, screen=screen, f64=screen)
     assert st["updated"] > 1_000_000 and st["supported"] > 8_000_000
+    if screen:
+        assert gs["screened_bricks"] > 10_000
 
 
 def test_batched_equals_sequential(vf, oracle):
@@ -212,7 +360,7 @@ def test_fused_dqb_multi_frame(vf, oracle, k):
     """Dual-quaternion blending: the device frame equals the oracle's DQB
     frame bit for bit."""
     s = scenes.small_scene(seed=21, k=k, nodes=70)
-    st = run_frames(vf, oracle, s, [0, 1, 2], blend=1)
+    st, _ = run_frames(vf, oracle, s, [0, 1, 2], blend=1)
     assert st["updated"] > 500
 
 

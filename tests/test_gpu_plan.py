@@ -16,7 +16,7 @@ F = np.float32
 def test_every_brick_shape_matches_oracle(vf, oracle, shape):
     print(f"brick shape {L.BRICK_SHAPES[shape]}", flush=True)
     s = scenes.small_scene(seed=31, k=8, nodes=80)
-    st = run_frames(vf, oracle, s, [0, 1, 2], tuning=(shape, 0))
+    st, _ = run_frames(vf, oracle, s, [0, 1, 2], tuning=(shape, 0))
     assert st["updated"] > 500
 
 
@@ -29,7 +29,7 @@ def test_node_cubes_wider_than_bricks(vf, oracle, shape):
     print(f"brick shape {L.BRICK_SHAPES[shape]}", flush=True)
     s = scenes.small_scene(seed=33, k=8, nodes=12, radius=0.1)
     assert min(s.cube) > 33 and s.cube[0] > s.dim[0] and s.cube[1] > s.dim[1]
-    st = run_frames(vf, oracle, s, [0, 1], tuning=(shape, 0))
+    st, _ = run_frames(vf, oracle, s, [0, 1], tuning=(shape, 0))
     assert st["updated"] > 500
 
 

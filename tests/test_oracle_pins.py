@@ -94,6 +94,19 @@ def test_weight_tsdf_closed_forms(oracle):
     assert L.vfo_weight_mesh_host(r, 0.5 * r) == float(F(math.exp(-0.125)))
 
 
+def test_weight_tsdf_extreme_radii(oracle):
+    """IEEE semantics at extreme radii, where a precomputed reciprocal of
+    2r^2 would fail (the kernels take IEEE divisions for such nodes):
+    2r^2 = inf (r = 1e20) or finite but huge (r = 1e19) gives exp(-0) = 1;
+    r = 0 with d = 0 is 0/0 = NaN; a tiny r with d > 0 underflows to 0."""
+    L = oracle.lib()
+    assert L.vfo_weight_tsdf(1e20, 0.5) == 1.0
+    assert L.vfo_weight_tsdf(1e19, 0.03) == 1.0
+    assert math.isnan(L.vfo_weight_tsdf(0.0, 0.0))
+    assert L.vfo_weight_tsdf(1e-11, 1e-3) == 0.0
+    assert L.vfo_weight_tsdf(1e-11, 1e-11) == float(F(math.exp(-0.5)))
+
+
 def test_markstein_division_exhaustive_mantissas(oracle):
     """The kernels divide by a per-node 1/(2 r^2) and a per-voxel 1/sum with a
     single fma correction.  Every dividend mantissa in [2^-12, 4) against
@@ -437,6 +450,37 @@ def test_frame_f64_error_small(oracle):
     assert math.sqrt(float(np.mean(err ** 2))) <= unit
 
 
+@pytest.mark.parametrize("blend", [0, 1])
+def test_gather_f64_companion_equals_node_major_f64(oracle, blend):
+    """The gather frame's binary64 companion (computed beside the float frame
+    on host threads) equals the node-major binary64 frame, which takes the
+    float frame's knn lists and pixels, bit for bit over three frames; the
+    float states agree too."""
+    from volumefusion_amd import scenes
+
+    s = scenes.small_scene(seed=18, k=6, nodes=70)
+    n = s.n_voxels
+    ta, wa, na = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+    tb, wb, nb = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+    a64, aw64 = np.full(n, -1.0), np.zeros(n)
+    b64, bw64 = np.full(n, -1.0), np.zeros(n)
+    for f in range(3):
+        depth, T = s.depth(f), s.transforms(f)
+        before = na.copy()
+        r = oracle.warped_frame(s.k, ta, wa, na, s.radii, T, s.dim, s.voxel_size, s.size, s.cube, s.center, depth,
+                                s.delta, s.intrinsics(), s.max_weight, want=("cnt", "idx", "pix"), blend=blend)
+        oracle.warped_frame_f64(s.k, a64, aw64, before, s.radii, T, s.dim, s.voxel_size, s.size, s.center, depth,
+                                s.delta, s.max_weight, r.cnt, r.idx, r.pix, blend=blend)
+        oracle.warped_frame_gather(s.k, tb, wb, nb, s.radii, T, s.dim, s.voxel_size, s.size, s.cube, s.center,
+                                   depth, s.delta, s.intrinsics(), s.max_weight, nthreads=3, blend=blend,
+                                   tsdf64=b64, wt64=bw64)
+        np.testing.assert_array_equal(tb, ta)
+        np.testing.assert_array_equal(wb, wa)
+        np.testing.assert_array_equal(b64, a64)
+        np.testing.assert_array_equal(bw64, aw64)
+    assert (aw64 > 0).sum() > 500
+
+
 def test_identity_warp_k1_equals_plain_integration_except_collisions(oracle):
     """With one knn slot and identity transforms the warp is exact (w = 1,
     T p = p), so the warped frame integrates each voxel at the point the plain
@@ -476,6 +520,60 @@ def test_identity_warp_k1_equals_plain_integration_except_collisions(oracle):
     np.testing.assert_array_equal(tw[ok], tp[ok])
     np.testing.assert_array_equal(ww[ok], wp[ok])
     assert (tw[~ok] == -1).all() and (ww[~ok] == 0).all()
+
+
+def _rotated_plane_expected(R3, t, dim, vs, size, depth, delta, fx, cx):
+    """Closed form of one plain integration (src/TSDFIntegration.cu:69-96 and
+    updateTsdf) for a volume rotated by R3 (3x3, rows) with dyadic entries and
+    positions, so that q = R p + t is exact: pixel floor(fx qx / qz + cx + 0.5)
+    (columns and rows alike), sdf = qz - d exact (Sterbenz), sample
+    fl(sdf / delta) or -1.  Voxels whose pixel coordinate lies within 1e-4 of
+    a rounding boundary are marked ambiguous."""
+    H, W = depth.shape
+    z, y, x = np.meshgrid(*(np.arange(d) for d in dim[::-1]), indexing="ij")
+    p = np.stack([x * vs[0] - size[0] / 2, y * vs[1] - size[1] / 2, z * vs[2] - size[2] / 2], -1).reshape(-1, 3)
+    q = p @ np.asarray(R3, np.float64).T + np.asarray(t, np.float64)
+    uf = fx * q[:, 0] / q[:, 2] + cx + 0.5
+    vf = fx * q[:, 1] / q[:, 2] + cx + 0.5
+    amb = (np.abs(uf - np.round(uf)) < 1e-4) | (np.abs(vf - np.round(vf)) < 1e-4)
+    px, py = np.floor(uf).astype(np.int64), np.floor(vf).astype(np.int64)
+    inside = (q[:, 2] >= 1e-4) & (px >= 0) & (px < W) & (py >= 0) & (py < H)
+    d = np.where(inside, depth[np.clip(py, 0, H - 1), np.clip(px, 0, W - 1)].astype(np.float64), np.inf)
+    sdf = (q[:, 2] - d).astype(F)
+    upd = inside & (sdf <= F(delta))
+    tn = np.where(sdf < -F(delta), F(-1), sdf / F(delta)).astype(F)
+    return np.where(upd, tn, F(-1)), np.where(upd, F(1), F(0)), amb
+
+
+def test_plain_and_camera_frame_rotated_closed_form(oracle):
+    """A volume rotated by 90 degrees about z in front of a plane tilted along
+    the image columns: plain integration (rotation R, src/TSDFIntegration.cu:93)
+    and camera-frame integration (world->camera rotation, :145) both equal the
+    closed form, bit for bit away from pixel-rounding boundaries.  The
+    transposed rotation predicts different samples on many voxels, so a
+    transposed R in the shared 3x3 product would fail."""
+    dim, vs, size, K = _plane_setup()
+    Rz = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    t = np.array([2.0 ** -7, -(2.0 ** -6), 1.0])
+    u = np.arange(64)
+    depth = np.tile((1.0 + 0.002 * (u - 32)).astype(F), (64, 1))
+    delta = 0.03
+    n = 16 ** 3
+    col = lambda M: np.ascontiguousarray(np.asarray(M, F).T.reshape(9))  # column-major Mat3f
+    I = np.eye(3)
+    a = oracle.integrate_plain(np.full(n, -1, F), np.zeros(n, F), depth, delta, dim, vs, size, col(Rz),
+                               t.astype(F), K, 15.0)
+    b = oracle.integrate_camera_frame(np.full(n, -1, F), np.zeros(n, F), depth, delta, dim, vs, size, col(Rz),
+                                      t.astype(F), col(I), np.zeros(3, F), K, 15.0)
+    te, we, amb = _rotated_plane_expected(Rz, t, dim, vs, size, depth, delta, 100.0, 32.0)
+    tT, wT, ambT = _rotated_plane_expected(Rz.T, t, dim, vs, size, depth, delta, 100.0, 32.0)
+    ok = ~amb
+    for ts, ws, _ in (a, b):
+        np.testing.assert_array_equal(ts[ok], te[ok])
+        np.testing.assert_array_equal(ws[ok], we[ok])
+    assert (we[ok] > 0).sum() > 500 and ((te[ok] > -1) & (te[ok] < 1)).sum() > 100
+    both = ok & ~ambT
+    assert (tT[both] != te[both]).sum() > 200
 
 
 def test_camera_frame_identity_pose_equals_plain(oracle):

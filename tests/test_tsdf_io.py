@@ -42,8 +42,9 @@ def test_native_round_trip_and_header(tmp_path):
 
 
 def test_legacy_layout(tmp_path):
-    """Byte offsets of TSDFVolume::save: dims 0, size 12 (24 bytes), delta 36,
-    tsdf 40, weights 40 + 4N, then 3N colour bytes and 4N colour weights."""
+    """Byte offsets of TSDFVolume::save: dims 0, size 12 (24 bytes: the size
+    and the voxel size that follows it in the object), delta 36, tsdf 40,
+    weights 40 + 4N, then 3N colour bytes and 4N colour weights."""
     dim = (4, 3, 2)
     t, w = volume(dim, 1)
     n = t.size
@@ -53,7 +54,9 @@ def test_legacy_layout(tmp_path):
     assert len(raw) == 12 + 24 + 4 + 4 * n + 4 * n + 3 * n + 4 * n
     assert struct.unpack_from("<3i", raw, 0) == dim
     assert struct.unpack_from("<3f", raw, 12) == tuple(F(x) for x in (0.4, 0.3, 0.2))
-    assert raw[24:36] == bytes(12)
+    # m_voxelSize, which TSDFVolume::load takes from the second half of the
+    # field: size / float(dim) in float
+    assert struct.unpack_from("<3f", raw, 24) == tuple(F(x) / F(d) for x, d in zip((0.4, 0.3, 0.2), dim))
     assert struct.unpack_from("<f", raw, 36)[0] == F(0.01)
     np.testing.assert_array_equal(np.frombuffer(raw, F, n, 40), t)
     np.testing.assert_array_equal(np.frombuffer(raw, F, n, 40 + 4 * n), w)

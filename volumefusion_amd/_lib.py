@@ -84,6 +84,7 @@ class FrameStats(C.Structure):
         ("updated", C.c_uint64),
         ("active_bricks", C.c_uint64),
         ("overflow_bricks", C.c_uint64),
+        ("screened_bricks", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -91,7 +92,11 @@ class FrameStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("brick_shape", C.c_int32), ("wg_per_cu", C.c_int32), ("ms_per_frame", C.c_float)]
+    _fields_ = [("brick_shape", C.c_int32), ("wg_per_cu", C.c_int32), ("ms_per_frame", C.c_float),
+                ("flags", C.c_int32)]
+
+
+TUNE_NO_SCREEN = 1  # VF_TUNE_NO_SCREEN
 
 
 BRICK_SHAPES = [(16, 8, 4), (16, 4, 4), (8, 8, 4), (8, 8, 8), (32, 4, 4)]  # VF_BRICK_* order
@@ -155,6 +160,29 @@ class ServerStats(C.Structure):
         ("totals", FrameStats),
     ]
 
+class GNProblem(C.Structure):
+    _fields_ = [
+        ("corr", C.c_void_p),
+        ("corr_size", C.c_uint32),
+        ("canon", C.c_void_p),
+        ("k", C.c_int),
+        ("knn_idxs", C.c_void_p),
+        ("knn_weights", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("graph", C.c_void_p),
+        ("graph_size", C.c_uint32),
+        ("graph_knn", C.c_void_p),
+        ("reg_k", C.c_int),
+        ("weight_depth", C.c_float),
+        ("weight_reg", C.c_float),
+    ]
+
+
+class GNReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("stopped", C.c_int), ("final_energy", C.c_double),
+                ("energy", C.c_double * 32)]
+
+
 vp = C.c_void_p
 i32 = C.c_int32
 u32 = C.c_uint32
@@ -183,6 +211,17 @@ _SIGS = {
     "vf_warp_mesh": [C.c_int, vp, vp, vp, vp, vp, sz, sz, vp],
     "vf_mesh_knn": [vp, C.c_int, vp, vp, vp, sz, vp, sz, C.c_int, vp],
     "vf_warp_points": [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
+    "vf_correspondence_pairs": [vp, vp, vp, vp, sz, Mat3, vp, Int2, vp, f32, vp],
+    "vf_fill_r_depth": [vp, f32, vp, u32, vp, vp, vp, vp],
+    "vf_fill_r_mreg": [vp, f32, vp, vp, u32, vp, C.c_int, vp],
+    "vf_fill_j_depth": [vp, f32, u32, u32, vp, u32, vp, C.c_int, vp, vp, vp, vp, vp],
+    "vf_fill_j_mreg": [vp, f32, u32, u32, vp, vp, u32, vp, C.c_int, vp],
+    "vf_update_transforms": [vp, vp, u32, vp],
+    "vf_lu_solve": [C.c_int, vp, C.c_int, vp, vp, vp],
+    "vf_gn_normal_equations": [vp, vp, vp, vp, C.POINTER(GNProblem), vp],
+    "vf_gn_solver_create": [C.POINTER(vp), C.POINTER(GNProblem), vp],
+    "vf_gn_solver_destroy": [vp],
+    "vf_gn_solver_iterate": [vp, vp, vp, vp, vp, sz, vp, C.c_int, f32, C.POINTER(GNReport), vp],
     "vf_filter_depth": [vp, u32, f32, f32, vp],
     "vf_mask_depth": [vp, vp, u32, vp],
     "vf_bilateral_filter": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, f32, vp],

@@ -154,6 +154,13 @@ __device__ __forceinline__ float div_rcp(float a, float b, float y) {
     return __builtin_fmaf(rem, y, q0);
 }
 
+// The weight denominators 2 r^2 for which the Markstein path below (weight
+// exponent and normalisation) is exact: every intermediate of div_rcp stays a
+// normal number.  Nodes outside the range take IEEE divisions.
+constexpr float kDenLo = 0x1p-64f;
+constexpr float kDenHi = 0x1p64f;
+__device__ __forceinline__ bool den_safe(float den) { return den >= kDenLo && den <= kDenHi; }
+
 // Gaussian warp weight of src/TSDFWarp.cu:17, exp(-(d*d) / (2 * r*r)), no
 // cutoff.  `den` is 2*(r*r), evaluated once per node.
 __device__ __forceinline__ float warp_weight(float den, float d) {
@@ -161,8 +168,9 @@ __device__ __forceinline__ float warp_weight(float den, float d) {
     return expw(num / den, kExp2Table);
 }
 
-// The same weight with the node's 1/den precomputed; identical result.  The
-// argument -(d*d)/den lies in [-2.000001, 0] for candidates (d <= 2r).
+// The same weight with the node's 1/den precomputed; identical result for
+// den_safe(den).  The argument -(d*d)/den lies in [-2.000001, 0] for
+// candidates (d <= 2r).
 __device__ __forceinline__ float warp_weight_rcp(float den, float rden, float d, const double* tab) {
     const float q = d * d;
     return expw_dom(-div_rcp(q, den, rden), tab);

@@ -1,5 +1,6 @@
 // vf_fused.hip -- the warped TSDF integration of one frame (or a batch of
-// independent volumes, one frame each) in four kernels.
+// independent volumes, one frame each) in five kernels (four with the brick
+// screen off).
 //
 // The reference runs nine dense launches with ~100 B of per-voxel scratch
 // (src/VolumeFusion.cu:699-742, buffers at :252-260).  Here:
@@ -7,11 +8,15 @@
 //   k_prep_bin  one thread per (node, brick overlapped by the node's cube):
 //               node record (translated position, cube corner, 2r, 2r^2 and
 //               its reciprocal, transform rows); append the node to the
-//               brick's list.  Also clears the collision tiles the previous
-//               frame flagged.
-//   k_compact   two bricks per thread: the active list (bricks with a
-//               non-empty node list) in brick order within each workgroup's
-//               run.
+//               brick's list; the brick's displacement bound.  Also clears
+//               the collision tiles the previous frame flagged, and (extra
+//               workgroups) takes the depth-tile maxima of the frame.
+//   k_compact   two bricks per thread: the bricks with a non-empty node list
+//               in brick order within each workgroup's run, each with the
+//               brick screen's verdict; bricks that may hold a pending voxel
+//               mark their target tiles.
+//   k_select    the active list: the candidates the screen keeps (see the
+//               brick screen below).
 //   k_pass_a    one workgroup per active brick at a time, bricks drawn from
 //               per-XCD queues (one voxel per thread, each wave a 4x4x4
 //               block): stage the brick's node list in LDS (each cube as
@@ -28,7 +33,8 @@
 //               reached twice, else write tsdf and weight.
 //
 // Unsupported voxels (no node cube covers their brick) are never touched, so
-// HBM traffic scales with the supported shell, not with the dense volume.
+// HBM traffic scales with the supported shell, not with the dense volume;
+// the screen also skips supported bricks that cannot change the frame.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -82,7 +88,21 @@ using Shape4 = Shape<32, 4, 4>;  // 512 threads, 128 B rows
 constexpr int kNumShapes = 5;
 constexpr int kShapeDims[kNumShapes][3] = {{16, 8, 4}, {16, 4, 4}, {8, 8, 4}, {8, 8, 8}, {32, 4, 4}};
 
-enum { kStSupported, kStInBounds, kStPending, kStCollided, kStUpdated, kStActive, kStSpilled, kStCount = 8 };
+enum {
+    kStSupported,
+    kStInBounds,
+    kStPending,
+    kStCollided,
+    kStUpdated,
+    kStActive,
+    kStSpilled,
+    kStScreened,
+    kStCount = 8
+};
+
+// Depth-tile maxima for the brick screen: tiles of 16 and of 128 pixels.
+constexpr int kTile0 = 16;
+constexpr int kTile1 = 128;
 
 struct VolDesc {
     Geom g;
@@ -113,6 +133,13 @@ struct VolDesc {
     uint32_t k1_base;
     uint32_t k1_count;
     int32_t blend;  // VF_BLEND_LBS or VF_BLEND_DQB
+    // Brick screen: pass A sweeps only the bricks that can hold a pending
+    // voxel or whose voxels can reach the target of one (screen != 0).
+    int32_t screen;
+    uint32_t pyr_blk;    // first depth-tile workgroup of this volume in k_prep_bin's grid
+    uint32_t pyr_base0;  // first 16-pixel tile maximum (pw0 x ph0, row-major)
+    uint32_t pyr_base1;  // first 128-pixel tile maximum (pw1 x ph1)
+    int32_t pw0, ph0, pw1, ph1;
 };
 
 struct NodeRec {
@@ -130,6 +157,7 @@ constexpr uint32_t kQueueStride = 64; // counters 256 bytes apart
 
 struct Ctrl {
     uint32_t active_count[2];  // by frame parity
+    uint32_t cand_count[2];    // bricks with a node list before the screen, by frame parity
     uint32_t queue[2][kQueues * kQueueStride];  // pass-A queue counters by frame parity
     uint32_t overflow_records;
     uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
@@ -167,15 +195,157 @@ __device__ __forceinline__ void stat_add(unsigned long long* stat, unsigned long
 constexpr int kPrepThreads = 1024;
 constexpr int kMaxBatch = 256;  // volumes per call
 
+// ---- brick screen ----------------------------------------------------------
+//
+// A supported voxel matters to the frame only if it is pending (passes the
+// projective tests of src/TSDFIntegration.cu:32-58) or if its warped target is
+// the target of a pending voxel (the count > 1 rule, :252-299).  The screen
+// proves both false for whole bricks and drops them before pass A:
+//
+//  * Every candidate node's warp moves a voxel of its cube by at most D_j
+//    (per axis): T_j p - p is affine in p, so |T_j p - p| is largest at a
+//    corner of the cube (clipped to the brick).  The LBS blend is a convex
+//    combination of the candidates' T_j p (weights exp(.) in [e^-2, 1],
+//    normalised), so the warped point lies within D_b = max_j D_j of the
+//    voxel, up to rounding (covered by a relative slack of 2^-16).
+//  * The brick's warped points lie in a box; no point of the box passes the
+//    projective tests when the box is behind the largest valid depth over its
+//    pixel footprint by more than delta (depth tile maxima), or has no pixel
+//    in the frame, or lies in front of z = 1e-4.
+//  * The targets lie in the box's voxel range (+-1).  Pending-capable bricks
+//    mark the 8^3 tiles of their target range; any other brick is dropped
+//    when its target range meets no marked tile: none of its voxels can share
+//    a target with a pending voxel.
+//
+// Dropped bricks change no tsdf, weight or collision decision.  Statistics of
+// supported and in-bounds voxels then count the swept bricks only.
+// The screen is off for DQB volumes (the blend is not a convex combination)
+// and for nodes whose weights could be NaN.
+
+constexpr uint32_t kInfBits = 0x7f800000u;  // +inf as float bits: unbounded displacement
+
+// D_j over the voxels [vlo, vhi] of node j's cube, in binary64: per axis a,
+// f_a(p) = (T p)_a - p_a is affine in p, so its largest magnitude over the
+// box is |f_a(centre)| + sum_k |T_ak - [a == k]| halfwidth_k; plus a
+// rounding slack of 2^-16 times a bound on the terms of the float product
+// (sum_k |T_ak| P_k + |T_a3| + P_a, P_k the largest |p_k| on the box).
+// +inf when the node's weights could leave [e^-2, 1] (radius not positive or
+// 2r^2 outside the Markstein-safe range), its position or transform is not
+// finite, or the bound does not fit a float.
+__device__ uint32_t disp_bound_bits(const VolDesc& vd, const float* M, const float m[3], float r,
+                                    const int32_t vlo[3], const int32_t vhi[3]) {
+    if (!(r > 0.0f) || !den_safe(2.0f * (r * r))) return kInfBits;
+    if (!__builtin_isfinite(m[0]) || !__builtin_isfinite(m[1]) || !__builtin_isfinite(m[2])) return kInfBits;
+    float e[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) e[i] = M[i];  // column-major: M(a, j) = e[4 j + a]
+    bool fin = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fin = fin && __builtin_isfinite(e[4 * j + a]);
+    if (!fin) return kInfBits;
+    double c[3], h[3], P[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double off = (double)vd.c[a] - (double)vd.g.halfs[a], vs = (double)vd.g.vs[a];
+        const double lo = (double)vlo[a] * vs + off, hi = (double)vhi[a] * vs + off;
+        c[a] = 0.5 * (lo + hi);
+        h[a] = 0.5 * (hi - lo);
+        P[a] = fmax(fabs(lo), fabs(hi));
+    }
+    double dmax = 0.0, mag = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double f = (double)e[12 + a] - c[a], w = 0.0, s = fabs((double)e[12 + a]) + P[a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double t = (double)e[4 * k + a];
+            f += t * c[k];
+            w += fabs(t - (a == k ? 1.0 : 0.0)) * h[k];
+            s += fabs(t) * P[k];
+        }
+        dmax = fmax(dmax, fabs(f) + w);
+        mag = fmax(mag, s);
+    }
+    const double D = dmax * (1.0 + 0x1p-40) + mag * 0x1p-16 + 0x1p-40;
+    if (!(D <= 1.0e38)) return kInfBits;
+    float fD = (float)D;
+    if ((double)fD < D) fD = __int_as_float(__float_as_int(fD) + 1);  // round up
+    return __float_as_uint(fD);
+}
+
+// A depth pixel as the screen sees it: the projective test rejects infinite
+// and too small depths (-inf here: never the maximum); NaN passes the test
+// (its sdf compares false both ways), so it counts as +inf.
+__device__ __forceinline__ float depth_key(float d) {
+    if (d != d) return kInf;
+    if (__builtin_isinf(d) || d < 1.e-4f) return -kInf;
+    return d;
+}
+
+// Tile maxima of one 128 x 128 pixel block of a volume's frame: the 64
+// 16-pixel tiles and the one 128-pixel tile.  One workgroup of kPrepThreads.
+__device__ void depth_tiles(const VolDesc* __restrict__ vols, int nvol, uint32_t blk, float* __restrict__ dmax) {
+    __shared__ uint32_t s_pb[kMaxBatch];
+    __shared__ float s_seg[kTile1][kTile1 / kTile0];
+    static_assert(kPrepThreads == kTile1 * (kTile1 / kTile0), "one thread per 16-pixel row segment");
+    const int tid = threadIdx.x;
+    for (int v = tid; v < nvol; v += blockDim.x) s_pb[v] = vols[v].pyr_blk;
+    __syncthreads();
+    int v = 0, hi = nvol - 1;  // last volume whose first block is <= blk
+    while (v < hi) {
+        const int mid = (v + hi + 1) >> 1;
+        if (s_pb[mid] <= blk)
+            v = mid;
+        else
+            hi = mid - 1;
+    }
+    const VolDesc& vd = vols[v];
+    const uint32_t local = blk - vd.pyr_blk;
+    const int32_t bx = (int32_t)(local % (uint32_t)vd.pw1), by = (int32_t)(local / (uint32_t)vd.pw1);
+    const int32_t W = vd.cam.W, H = vd.cam.H;
+    const int row = tid / (kTile1 / kTile0), seg = tid % (kTile1 / kTile0);
+    const int32_t y = by * kTile1 + row;
+    const int32_t x0 = bx * kTile1 + seg * kTile0;
+    float m = -kInf;
+    if (y < H) {
+        const float* rowp = vd.depth + (int64_t)y * W;
+#pragma unroll
+        for (int i = 0; i < kTile0; ++i)
+            if (x0 + i < W) m = fmaxf(m, depth_key(gload(rowp + x0 + i)));
+    }
+    s_seg[row][seg] = m;
+    __syncthreads();
+    if (tid < 64) {
+        const int ty = tid / (kTile1 / kTile0), tx = tid % (kTile1 / kTile0);
+        float t = -kInf;
+#pragma unroll
+        for (int r = 0; r < kTile0; ++r) t = fmaxf(t, s_seg[ty * kTile0 + r][tx]);
+        const int32_t gx = bx * (kTile1 / kTile0) + tx, gy = by * (kTile1 / kTile0) + ty;
+        if (gx < vd.pw0 && gy < vd.ph0) dmax[vd.pyr_base0 + (uint32_t)(gy * vd.pw0 + gx)] = t;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t = fmaxf(t, __shfl_xor(t, o));
+        if (tid == 0) dmax[vd.pyr_base1 + (uint32_t)(by * vd.pw1 + bx)] = t;
+    }
+}
+
+// The prep kernel: workgroups [0, pair_blocks) take the (node, brick) items;
+// workgroups [pair_blocks, gridDim.x) compute the depth-tile maxima of the
+// screened volumes' frames instead (one 128 x 128 pixel block each).
 template <class S>
 __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
-    const VolDesc* __restrict__ vols, int nvol, uint32_t total, NodeRec* __restrict__ nrec,
-    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, Ctrl* __restrict__ ctrl,
-    uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint32_t clear_tiles,
-    unsigned long long* __restrict__ stat) {
+    const VolDesc* __restrict__ vols, int nvol, uint32_t total, uint32_t pair_blocks, NodeRec* __restrict__ nrec,
+    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ brick_disp,
+    float* __restrict__ dmax, Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap,
+    uint32_t* __restrict__ tile_flag, uint32_t clear_tiles, unsigned long long* __restrict__ stat) {
+    if (blockIdx.x >= pair_blocks) {
+        depth_tiles(vols, nvol, blockIdx.x - pair_blocks, dmax);
+        return;
+    }
     __shared__ uint32_t s_k1[kMaxBatch];  // first (node, brick) item of each volume
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t nthreads = pair_blocks * blockDim.x;
     for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_k1[v] = vols[v].k1_base;
 
     // Clear the collision tiles the previous frame on this workspace flagged
@@ -274,6 +444,21 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
         const uint32_t gb = vd.brick_base + b;
         const uint32_t slot = atomicAdd(brick_count + gb, 1u);
         if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
+        if (vd.screen) {
+            // displacement bound of this node over its cube within this brick
+            const int32_t bb[3] = {bx, by, bz};
+            int32_t vlo[3], vhi[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int32_t o = bb[a] * bs[a];
+                vlo[a] = lo[a] > o ? lo[a] : o;
+                long long h = (long long)lo[a] + vd.cube[a] - 1;
+                if (h > o + bs[a] - 1) h = o + bs[a] - 1;
+                if (h > vd.g.dim[a] - 1) h = vd.g.dim[a] - 1;
+                vhi[a] = (int32_t)h;
+            }
+            atomicMax(brick_disp + gb, disp_bound_bits(vd, vd.T + 16ull * j, m, vd.radii[j], vlo, vhi));
+        }
     } while (false);
 }
 
@@ -282,19 +467,217 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
 // list append per workgroup (appends to one counter serialise, so a
 // workgroup takes many bricks).
 constexpr int kCompactPer = 2;
-__global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restrict__ vols, int nvol,
-                                                          uint32_t nbricks, const uint32_t* __restrict__ brick_count,
-                                                          uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
-                                                          uint64_t brick_cap, int parity_arg) {
-    constexpr int kWaves = kPrepThreads / 64;
-    __shared__ uint32_t s_cnt[kCompactPer * kWaves];  // per (round, wave): count, then first position
-    __shared__ uint32_t s_base;
-    __shared__ uint32_t s_bb[kMaxBatch];  // first brick of each volume
-    for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_bb[v] = vols[v].brick_base;
+constexpr int kCompactWaves = kPrepThreads / 64;
+
+// Order-preserving compaction of one workgroup's kCompactPer x kPrepThreads
+// slots (bit r of `flags`: slot r * kPrepThreads + thread is kept): one
+// atomic on *counter reserves the run, s_cnt receives the first position of
+// each (round, wave) within it.  Returns the run's first list position.
+__device__ __forceinline__ uint32_t compact_run(uint32_t flags, uint32_t* counter, uint32_t* s_cnt,
+                                                uint32_t* s_base, uint32_t* s_total = nullptr) {
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < kCompactPer; ++r) {
+        const unsigned long long m = __ballot((flags >> r) & 1u);
+        if (lane == 0) s_cnt[r * kCompactWaves + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive prefix over (round, wave), round-major: two entries a lane
+        static_assert(kCompactPer * kCompactWaves <= 128, "two table entries per lane of one wave");
+        const int e0 = 2 * lane;
+        const uint32_t c0 = e0 < kCompactPer * kCompactWaves ? s_cnt[e0] : 0u;
+        const uint32_t c1 = e0 + 1 < kCompactPer * kCompactWaves ? s_cnt[e0 + 1] : 0u;
+        const uint32_t incl = wave_inclusive_sum(c0 + c1);
+        const uint32_t excl = incl - (c0 + c1);
+        if (e0 < kCompactPer * kCompactWaves) s_cnt[e0] = excl;
+        if (e0 + 1 < kCompactPer * kCompactWaves) s_cnt[e0 + 1] = excl + c0;
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (lane == 0) {
+            *s_base = total ? atomicAdd(counter, total) : 0u;
+            if (s_total) *s_total = total;
+        }
+    }
+    __syncthreads();
+    return *s_base;
+}
+
+// List position of this thread's slot r after compact_run (whole waves call it).
+__device__ __forceinline__ uint32_t compact_pos(uint32_t base, const uint32_t* s_cnt, int r, bool keep) {
+    const unsigned long long m = __ballot(keep);
+    return base + s_cnt[r * kCompactWaves + (threadIdx.x >> 6)] +
+           (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+}
+
+constexpr uint32_t kScPend = 1u;   // may hold a pending voxel: kept, marks its target tiles
+constexpr uint32_t kScAll = 2u;    // kept, and every brick of its volume is kept
+constexpr uint32_t kScEmpty = 4u;  // no target inside the volume: dropped
+constexpr uint32_t kScKeep = 8u;   // screen off for the volume: kept
+constexpr int kMaxScreenTiles = 64;
+
+// Lanes that select one candidate together (k_select) and that screen one
+// brick together (k_compact).  Measured at 512^3 (prep, compaction and
+// selection together): 39.9 us with one lane per screened brick, 41.1, 44.5
+// and 51.6 us with 2, 4 and 8; the per-brick work is short, and fewer lanes
+// per brick mean fewer rounds in the workgroups whose run is dense.
+constexpr int kGrp = 8;
+constexpr int kGrpC = 1;
+
+// Largest of x over the aligned group of GL lanes (every lane of the group
+// active).
+template <int GL>
+__device__ __forceinline__ float group_max(float x) {
+#pragma unroll
+    for (int o = 1; o < GL; o <<= 1) x = fmaxf(x, __shfl_xor(x, o, GL));
+    return x;
+}
+
+// Any of the group's lanes (every lane of the group active).
+template <int GL>
+__device__ __forceinline__ bool group_any(bool x) {
+    const unsigned long long m = __ballot(x);
+    return ((m >> (__lane_id() & (64 - GL))) & ((1ull << GL) - 1ull)) != 0ull;
+}
+
+// The screen's verdict on brick b of vd with displacement bound dbits: its
+// target range in 8^3 tiles (lo | hi << 16 per axis in .x, .y, .z) and flags
+// in .w.  Called by the GL lanes of a group together (q: the lane's rank in
+// the group); each computes the same box, the depth-tile loads are split
+// over the group.
+template <class S, int GL>
+__device__ uint4 screen_brick(const VolDesc& vd, uint32_t b, uint32_t dbits, const float* __restrict__ dmax,
+                              int q) {
+    if (!vd.screen) return make_uint4(0u, 0u, 0u, kScKeep);
+    if (dbits >= kInfBits) return make_uint4(0u, 0u, 0u, kScPend | kScAll);
+    const uint32_t bd0 = (uint32_t)vd.bdim[0], bd1 = (uint32_t)vd.bdim[1];
+    const int32_t bi[3] = {(int32_t)(b % bd0), (int32_t)((b / bd0) % bd1), (int32_t)(b / (bd0 * bd1))};
+    const int32_t bs[3] = {S::BX, S::BY, S::BZ};
+    const double D = (double)__uint_as_float(dbits);
+    double lo[3], hi[3];  // box of the brick's warped voxel positions (camera frame)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t v0 = bi[a] * bs[a];
+        const int32_t v1 = v0 + bs[a] - 1 < vd.g.dim[a] - 1 ? v0 + bs[a] - 1 : vd.g.dim[a] - 1;
+        const double h = (double)vd.g.halfs[a], c = (double)vd.c[a], vs = (double)vd.g.vs[a];
+        lo[a] = ((double)v0 * vs - h + c) - D;
+        hi[a] = ((double)v1 * vs - h + c) + D;
+    }
+    // targets worldToVoxel(Rinv (p - c)), src/TSDFIntegration.cu:272-279,
+    // over the box by interval arithmetic, +-1 voxel for rounding
+    uint32_t tr[3];
+    bool empty = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double olo = 0.0, ohi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double r = (double)vd.rinv[i + 3 * j];
+            const double ql = r * (lo[j] - (double)vd.c[j]), qh = r * (hi[j] - (double)vd.c[j]);
+            olo += fmin(ql, qh);
+            ohi += fmax(ql, qh);
+        }
+        const double fl = (olo + (double)vd.g.halfs[i]) * (double)vd.g.inv[i];
+        const double fh = (ohi + (double)vd.g.halfs[i]) * (double)vd.g.inv[i];
+        const double vl = fmax(floor(fl - fabs(fl) * 0x1p-20) - 1.0, 0.0);
+        const double vh = fmin(ceil(fh + fabs(fh) * 0x1p-20) + 1.0, (double)(vd.g.dim[i] - 1));
+        if (!(vl <= vh)) {
+            empty = true;
+            tr[i] = 0u;
+        } else {
+            tr[i] = ((uint32_t)vl >> 3) | (((uint32_t)vh >> 3) << 16);
+        }
+    }
+    if (empty) return make_uint4(0u, 0u, 0u, kScEmpty);
+    // projective tests, src/TSDFIntegration.cu:16-58
+    const Camera& cam = vd.cam;
+    bool pend;
+    if (hi[2] < 0.99e-4) {
+        pend = false;  // every point in front of z = 1e-4
+    } else if (!(lo[2] > 1.0e-3)) {
+        pend = true;  // footprint unbounded near z = 0
+    } else {
+        const double fx = cam.fx, fy = cam.fy;
+        double u0 = fx * lo[0] / lo[2], u1 = u0, v0 = fy * lo[1] / lo[2], v1 = v0;
+        const double us[3] = {fx * lo[0] / hi[2], fx * hi[0] / lo[2], fx * hi[0] / hi[2]};
+        const double vs_[3] = {fy * lo[1] / hi[2], fy * hi[1] / lo[2], fy * hi[1] / hi[2]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            u0 = fmin(u0, us[q]);
+            u1 = fmax(u1, us[q]);
+            v0 = fmin(v0, vs_[q]);
+            v1 = fmax(v1, vs_[q]);
+        }
+        // pixel = int(u + 0.5f) (truncation), +-1 for rounding
+        const double px0 = fmax(floor(u0 + (double)cam.cx + 0.5) - 1.0, 0.0);
+        const double px1 = fmin(ceil(u1 + (double)cam.cx + 0.5) + 1.0, (double)cam.W - 1.0);
+        const double py0 = fmax(floor(v0 + (double)cam.cy + 0.5) - 1.0, 0.0);
+        const double py1 = fmin(ceil(v1 + (double)cam.cy + 0.5) + 1.0, (double)cam.H - 1.0);
+        if (!__builtin_isfinite(px0 + px1 + py0 + py1)) {
+            pend = true;
+        } else if (!(px0 <= px1 && py0 <= py1)) {
+            pend = false;  // no pixel of the frame
+        } else {
+            const int32_t x0 = (int32_t)px0, x1 = (int32_t)px1, y0 = (int32_t)py0, y1 = (int32_t)py1;
+            // the largest valid depth over the footprint: 16-pixel tiles when
+            // at most 16 of them cover it, else 128-pixel tiles (at most 40),
+            // else unbounded; lane q of the group loads tiles q, q + GL, ...,
+            // eight at a time
+            float md = -kInf;
+            int32_t tx0 = x0 / kTile0, ty0 = y0 / kTile0, nx = x1 / kTile0 - tx0 + 1, ny = y1 / kTile0 - ty0 + 1;
+            uint32_t base = vd.pyr_base0, pw = (uint32_t)vd.pw0;
+            if (nx * ny > 16) {
+                tx0 = x0 / kTile1, ty0 = y0 / kTile1, nx = x1 / kTile1 - tx0 + 1, ny = y1 / kTile1 - ty0 + 1;
+                base = vd.pyr_base1, pw = (uint32_t)vd.pw1;
+            }
+            if (nx * ny > 40) {
+                md = kInf;
+            } else {
+                for (int32_t t0 = q; t0 < nx * ny; t0 += 8 * GL) {
+                    float v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int32_t t = t0 + u * GL;
+                        v[u] = t < nx * ny ? dmax[base + (uint32_t)((ty0 + t / nx) * (int32_t)pw + tx0 + t % nx)]
+                                           : -kInf;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) md = fmaxf(md, v[u]);
+                }
+                md = group_max<GL>(md);
+            }
+            // every sdf = z - d > delta: z >= lo[2] and d <= md, and a margin
+            // above one ulp of delta keeps the float subtraction above delta
+            const double thr = (double)cam.delta + fabs((double)cam.delta) * 0x1p-20 + 0x1p-40;
+            pend = !(lo[2] - (double)md > thr);
+        }
+    }
+    return make_uint4(tr[0], tr[1], tr[2], pend ? kScPend : 0u);
+}
+
+__device__ __forceinline__ uint32_t tile_span(uint32_t r) { return (r >> 16) - (r & 0xffffu) + 1u; }
+
+// The frame's active bricks (those with a non-empty node list), in brick
+// order within each workgroup's run of kCompactPer * kPrepThreads bricks; one
+// list append per workgroup (appends to one counter serialise, so a
+// workgroup takes many bricks).  With SCREEN they go to the candidate list
+// with the screen's verdict, and pending-capable bricks mark their target
+// tiles (tag: this frame's number + 1) for k_select.  The run's candidates
+// are first packed in LDS, then screened by groups of kGrpC lanes, so that
+// they occupy the first waves of the workgroup rather than scattered lanes
+// of every wave.
+template <class S, bool SCREEN>
+__global__ __launch_bounds__(kPrepThreads) void k_compact(
+    const VolDesc* __restrict__ vols, int nvol, uint32_t nbricks, const uint32_t* __restrict__ brick_count,
+    const uint32_t* __restrict__ brick_disp, const float* __restrict__ dmax, uint32_t* __restrict__ active2,
+    uint32_t* __restrict__ cand, uint4* __restrict__ cand_info, uint32_t* __restrict__ tile_mark,
+    uint32_t* __restrict__ vol_mark, Ctrl* __restrict__ ctrl, uint64_t brick_cap, int parity_arg) {
+    __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];  // per (round, wave): count, then first position
+    __shared__ uint32_t s_base, s_total;
+    __shared__ uint32_t s_bb[kMaxBatch];  // first brick of each volume
+    __shared__ uint32_t s_item[SCREEN ? kCompactPer * kPrepThreads : 1];  // the run's candidates, in list order
+    for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_bb[v] = vols[v].brick_base;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
-    uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
     const uint64_t gb0 = (uint64_t)blockIdx.x * (kPrepThreads * kCompactPer) + threadIdx.x;
     uint32_t flags = 0;  // bit r: brick gb0 + r * kPrepThreads is active
 #pragma unroll
@@ -302,29 +685,13 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restr
         const uint64_t gb = gb0 + (uint64_t)(r * kPrepThreads);
         const bool act = gb < nbricks && brick_count[gb] != 0u;
         flags |= (uint32_t)act << r;
-        const unsigned long long m = __ballot(act);
-        if (lane == 0) s_cnt[r * kWaves + wave] = (uint32_t)__popcll(m);
     }
-    __syncthreads();
-    if (wave == 0) {
-        // exclusive prefix over (round, wave), round-major: two entries a lane
-        static_assert(kCompactPer * kWaves <= 128, "two table entries per lane of one wave");
-        const int e0 = 2 * lane;
-        const uint32_t c0 = e0 < kCompactPer * kWaves ? s_cnt[e0] : 0u;
-        const uint32_t c1 = e0 + 1 < kCompactPer * kWaves ? s_cnt[e0 + 1] : 0u;
-        const uint32_t incl = wave_inclusive_sum(c0 + c1);
-        const uint32_t excl = incl - (c0 + c1);
-        if (e0 < kCompactPer * kWaves) s_cnt[e0] = excl;
-        if (e0 + 1 < kCompactPer * kWaves) s_cnt[e0 + 1] = excl + c0;
-        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        if (lane == 0) s_base = total ? atomicAdd(&ctrl->active_count[parity], total) : 0u;
-    }
-    __syncthreads();
-    const uint32_t base = s_base;
+    const uint32_t base = compact_run(flags, SCREEN ? &ctrl->cand_count[parity] : &ctrl->active_count[parity],
+                                      s_cnt, &s_base, &s_total);
 #pragma unroll
     for (int r = 0; r < kCompactPer; ++r) {
         const bool act = (flags >> r) & 1u;
-        const unsigned long long m = __ballot(act);
+        const uint32_t pos = compact_pos(base, s_cnt, r, act);
         if (act) {
             const uint32_t gb = (uint32_t)(gb0 + (uint64_t)(r * kPrepThreads));
             int lo = 0, hi = nvol - 1;  // last volume whose first brick is <= gb
@@ -335,10 +702,133 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(const VolDesc* __restr
                 else
                     hi = mid - 1;
             }
-            active[base + s_cnt[r * kWaves + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
-                ((uint32_t)lo << 24) | (gb - s_bb[lo]);
+            const uint32_t item = ((uint32_t)lo << 24) | (gb - s_bb[lo]);
+            if (!SCREEN)
+                active2[(uint64_t)parity * brick_cap + pos] = item;
+            else
+                s_item[pos - base] = item;
         }
     }
+    if (!SCREEN) return;
+    __syncthreads();
+    const uint32_t tag = ctrl->frame + 1u;
+    const int q = (int)(threadIdx.x % kGrpC);
+    for (uint32_t c = threadIdx.x / kGrpC; c < s_total; c += kPrepThreads / kGrpC) {
+        const uint32_t item = s_item[c];
+        const uint32_t v = item >> 24, local = item & 0xffffffu;
+        const VolDesc& vd = vols[v];
+        const uint32_t gb = s_bb[v] + local;
+        const uint4 info = screen_brick<S, kGrpC>(vd, local, brick_disp[gb], dmax, q);
+        if (q == 0) {
+            cand[base + c] = item;
+            cand_info[base + c] = info;
+        }
+        if (info.w & kScPend) {
+            const uint32_t nx = tile_span(info.x), nxy = nx * tile_span(info.y);
+            const uint32_t nt = nxy * tile_span(info.z);
+            if ((info.w & kScAll) || nt > (uint32_t)kMaxScreenTiles) {
+                if (q == 0) vol_mark[v] = tag;
+            } else {
+                const uint32_t x0 = info.x & 0xffffu, y0 = info.y & 0xffffu, z0 = info.z & 0xffffu;
+                for (uint32_t t = (uint32_t)q; t < nt; t += kGrpC) {
+                    const uint32_t tz = z0 + t / nxy, ty = y0 + (t % nxy) / nx, tx = x0 + t % nx;
+                    tile_mark[vd.tile_base + (tz * (uint32_t)vd.tdim[1] + ty) * (uint32_t)vd.tdim[0] + tx] = tag;
+                }
+            }
+        }
+    }
+}
+
+// The active list from the candidates: a brick stays when the screen kept it
+// or when its target tiles meet a tile a pending-capable brick marked (or its
+// volume is marked whole).  Dropped bricks get their node-list count cleared
+// here (pass A clears the others); every candidate's displacement bound is
+// cleared for the next frame.  A group of kGrp lanes takes one candidate
+// (lane q tests tiles q, q + 8, ...); a workgroup takes kSelPerBlock
+// candidates at a time, in list order, so the active list keeps the
+// candidates' order.
+constexpr uint32_t kSelPerRound = kPrepThreads / kGrp;
+constexpr uint32_t kSelPerBlock = kCompactPer * kSelPerRound;
+
+__global__ __launch_bounds__(kPrepThreads) void k_select(
+    const VolDesc* __restrict__ vols, const uint32_t* __restrict__ cand, const uint4* __restrict__ cand_info,
+    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_disp, const uint32_t* __restrict__ tile_mark,
+    const uint32_t* __restrict__ vol_mark, uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
+    uint64_t brick_cap, int parity_arg, unsigned long long* __restrict__ stat,
+    unsigned long long* __restrict__ totals) {
+    __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];
+    __shared__ uint32_t s_base;
+    __shared__ unsigned long long s_drop;
+    if (threadIdx.x == 0) s_drop = 0ull;
+    const uint32_t parity = frame_parity(ctrl, parity_arg);
+    const uint32_t n = ctrl->cand_count[parity];
+    const uint32_t tag = ctrl->frame + 1u;
+    const int q = (int)(threadIdx.x % kGrp);
+    uint32_t dropped = 0;
+    // the loop bound is the same for every thread of the workgroup
+    for (uint64_t c0 = (uint64_t)blockIdx.x * kSelPerBlock; c0 < n; c0 += (uint64_t)gridDim.x * kSelPerBlock) {
+        uint32_t flags = 0, items[kCompactPer];
+#pragma unroll
+        for (int r = 0; r < kCompactPer; ++r) {
+            const uint64_t i = c0 + (uint64_t)r * kSelPerRound + threadIdx.x / kGrp;
+            items[r] = 0u;
+            if (i >= n) continue;
+            const uint32_t item = cand[i];
+            const uint4 info = cand_info[i];
+            const uint32_t v = item >> 24;
+            const VolDesc& vd = vols[v];
+            const uint32_t gb = vd.brick_base + (item & 0xffffffu);
+            bool keep;
+            if (info.w & (kScPend | kScAll | kScKeep)) {
+                keep = true;
+            } else if (info.w & kScEmpty) {
+                keep = false;
+            } else if (vol_mark[v] == tag) {
+                keep = true;
+            } else if (tile_span(info.x) * tile_span(info.y) * tile_span(info.z) > (uint32_t)kMaxScreenTiles) {
+                keep = true;
+            } else {
+                const uint32_t x0 = info.x & 0xffffu, y0 = info.y & 0xffffu, z0 = info.z & 0xffffu;
+                const uint32_t nx = tile_span(info.x), nxy = nx * tile_span(info.y);
+                const uint32_t nt = nxy * tile_span(info.z);
+                static_assert(kMaxScreenTiles <= 8 * kGrp, "eight tiles per lane cover the range");
+                uint32_t m[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t t = (uint32_t)q + (uint32_t)(u * kGrp);
+                    m[u] = 0u;
+                    if (t < nt) {
+                        const uint32_t tz = z0 + t / nxy, ty = y0 + (t % nxy) / nx, tx = x0 + t % nx;
+                        m[u] = tile_mark[vd.tile_base + (tz * (uint32_t)vd.tdim[1] + ty) * (uint32_t)vd.tdim[0] + tx];
+                    }
+                }
+                bool hit = false;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) hit = hit || m[u] == tag;
+                keep = group_any<kGrp>(hit);
+            }
+            if (q == 0) {
+                brick_disp[gb] = 0u;
+                if (!keep) {
+                    brick_count[gb] = 0u;
+                    ++dropped;
+                }
+                items[r] = item;
+                flags |= (uint32_t)keep << r;
+            }
+        }
+        const uint32_t base = compact_run(flags, &ctrl->active_count[parity], s_cnt, &s_base);
+#pragma unroll
+        for (int r = 0; r < kCompactPer; ++r) {
+            const bool keep = (flags >> r) & 1u;
+            const uint32_t pos = compact_pos(base, s_cnt, r, keep);
+            if (keep) active2[(uint64_t)parity * brick_cap + pos] = items[r];
+        }
+        __syncthreads();  // s_cnt and s_base are rewritten by the next round
+    }
+    if (dropped) atomicAdd(&s_drop, (unsigned long long)dropped);
+    __syncthreads();
+    if (threadIdx.x == 0) stat_add(stat, totals, kStScreened, s_drop);
 }
 
 // A node cube as the ranges [A, B] of brick-local coordinates it covers per
@@ -409,6 +899,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         // the other parity's list and counters belong to the previous frame,
         // whose kernels have finished
         ctrl->active_count[parity ^ 1u] = 0u;
+        ctrl->cand_count[parity ^ 1u] = 0u;
         for (uint32_t q = 0; q < kQueues; ++q) ctrl->queue[parity ^ 1u][q * kQueueStride] = 0u;
     }
     const uint32_t nactive = ctrl->active_count[parity];
@@ -509,6 +1000,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         voxel_to_world(vd.g, x, y, z, p);
         KnnSlots<K, kPassAThreads> ks;
         ks.init(s_keys, tid);
+        bool slow = true;  // the spilled path always takes IEEE divisions
 
         if (!spill) {
             // This brick's list with its node records, in list order (the
@@ -526,7 +1018,9 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 s_t2[tid] = pt2;
                 s_w[tid] = make_float2(pw.x, pw.y);
             }
-            __syncthreads();
+            // a node whose 2r^2 is outside the Markstein-safe range sends the
+            // brick's weights through IEEE divisions
+            slow = __syncthreads_or((uint32_t)tid < cnt && !den_safe(pw.x)) != 0;
             // Branch-free test of every list entry, the next entry's LDS
             // reads issued before this one's arithmetic (the arrays hold one
             // slot beyond kCap, so the read after the last entry stays inside).
@@ -634,7 +1128,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                     } else {
                         dw = s_w[kl.i[s] & kSlotMask];
                     }
-                    w[s] = warp_weight_rcp(dw.x, dw.y, kl.d[s], s_exp2);
+                    w[s] = slow ? warp_weight(dw.x, kl.d[s]) : warp_weight_rcp(dw.x, dw.y, kl.d[s], s_exp2);
                     sum = sum + w[s];
                 }
             }
@@ -658,7 +1152,16 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 acc[1] = pc[1];
                 acc[2] = pc[2];
             } else {
-            const float rs = rcp_rn(sum);
+            float rs = rcp_rn(sum), sm = sum;
+            if (slow) {
+                // IEEE quotients (src/TSDFWarp.cu:112-113); then
+                // div_rcp(w, 1, 1) below returns w unchanged
+#pragma unroll
+                for (int s = 0; s < K; ++s)
+                    if (s < n) w[s] = w[s] / sum;
+                sm = 1.0f;
+                rs = 1.0f;
+            }
             if (vd.blend == VF_BLEND_DQB) {
                 // dual-quaternion blend of the selected nodes' rigid parts
                 float4 br = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bd = br;
@@ -675,7 +1178,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                             qr = s_t0[kl.i[s] & kSlotMask];
                             qd = s_t1[kl.i[s] & kSlotMask];
                         }
-                        dqb_accumulate(qr, qd, f, div_rcp(w[s], sum, rs), &br, &bd);
+                        dqb_accumulate(qr, qd, f, div_rcp(w[s], sm, rs), &br, &bd);
                     }
                 }
                 dqb_transform(br, bd, pc, acc);
@@ -694,7 +1197,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                             r1 = s_t1[kl.i[s] & kSlotMask];
                             r2 = s_t2[kl.i[s] & kSlotMask];
                         }
-                        const float ws = div_rcp(w[s], sum, rs);
+                        const float ws = div_rcp(w[s], sm, rs);
                         float t[3];
                         affine_apply(r0, r1, r2, pc, t);
 #pragma unroll
@@ -963,7 +1466,14 @@ struct Buffers {
     unsigned long long* totals = nullptr;
     VolDesc* vols = nullptr;
     VolPtr* vptr = nullptr;
-    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0, arena_slots = 0;
+    // brick screen
+    uint32_t* brick_disp = nullptr;  // per brick: displacement bound (float bits), max over its (node, brick) pairs
+    uint32_t* cand = nullptr;        // bricks with a node list, before the screen
+    uint4* cand_info = nullptr;      // the screen's verdict per candidate
+    uint32_t* tile_mark = nullptr;   // per 8^3 tile: tag of the last frame that may have a pending target there
+    uint32_t* vol_mark = nullptr;    // per volume: tag of the last frame that keeps every brick
+    float* dmax = nullptr;           // depth-tile maxima
+    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0, arena_slots = 0, pyr_cap = 0;
     uint64_t bytes = 0;
 };
 
@@ -992,7 +1502,7 @@ struct vf_workspace {
     int timing = 0;
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
-    vf_tuning tuning = {0, 0, 0.0f};
+    vf_tuning tuning = {0, 0, 0.0f, 0};
     void* scratch = nullptr;  // general scratch for other entry points (point path)
     uint64_t scratch_bytes = 0;
 };
@@ -1004,8 +1514,10 @@ constexpr int kMaxTimed = 4096;
 constexpr int kTimedEvents = 6;  // per timed frame
 
 void free_buffers(Buffers& b) {
-    void* ptrs[] = {b.ctrl,    b.nrec,         b.brick_count, b.brick_nodes, b.active, b.bitmap, b.tile_flag,
-                    b.records, b.arena_counts, b.stat,        b.totals,      b.vols,   b.vptr};
+    void* ptrs[] = {b.ctrl,      b.nrec,         b.brick_count, b.brick_nodes, b.active,    b.bitmap,
+                    b.tile_flag, b.records,      b.arena_counts, b.stat,       b.totals,    b.vols,
+                    b.vptr,      b.brick_disp,   b.cand,        b.cand_info,   b.tile_mark, b.vol_mark,
+                    b.dmax};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = Buffers();
@@ -1029,14 +1541,15 @@ int dev_alloc(T** p, uint64_t count, uint64_t* bytes) {
 }
 
 struct Need {
-    uint64_t nodes = 0, bricks = 0, tiles = 0, records = 0, nvol = 0, arena_slots = 0;
+    uint64_t nodes = 0, bricks = 0, tiles = 0, records = 0, nvol = 0, arena_slots = 0, pyr = 0;
 };
 
 // (Re)allocates to at least the requested capacities; new buffers are zeroed.
 int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     Buffers& b = ws->buf;
     if (b.ctrl && need.nodes <= b.node_cap && need.bricks <= b.brick_cap && need.tiles <= b.tile_cap &&
-        need.records <= b.record_cap && need.nvol <= b.vol_cap && need.arena_slots <= b.arena_slots)
+        need.records <= b.record_cap && need.nvol <= b.vol_cap && need.arena_slots <= b.arena_slots &&
+        need.pyr <= b.pyr_cap)
         return VF_OK;
     if (b.ctrl) VF_HIP(hipStreamSynchronize(stream));
     // a capacity that has to grow gets half again as much headroom, so that
@@ -1051,6 +1564,7 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     nbuf.record_cap = grow(need.records, b.record_cap);
     nbuf.vol_cap = grow(need.nvol, b.vol_cap);
     nbuf.arena_slots = grow(need.arena_slots, b.arena_slots);
+    nbuf.pyr_cap = grow(need.pyr, b.pyr_cap);
     // the running totals survive the reallocation
     unsigned long long* totals = b.totals;
     b.totals = nullptr;
@@ -1074,6 +1588,12 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     }
     if (!st) st = dev_alloc(&nbuf.vols, nbuf.vol_cap * kDescSlots, &bytes);
     if (!st) st = dev_alloc(&nbuf.vptr, nbuf.vol_cap * kDescSlots, &bytes);
+    if (!st) st = dev_alloc(&nbuf.brick_disp, nbuf.brick_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.cand, nbuf.brick_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.cand_info, nbuf.brick_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.tile_mark, nbuf.tile_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.vol_mark, nbuf.vol_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.dmax, nbuf.pyr_cap, &bytes);
     if (st) {
         free_buffers(nbuf);
         return st;
@@ -1087,6 +1607,9 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     VF_HIP(hipMemsetAsync(nbuf.tile_flag, 0, nbuf.tile_cap * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.arena_counts, 0, nbuf.arena_slots * sizeof(uint32_t), stream));
     VF_HIP(hipMemsetAsync(nbuf.stat, 0, (uint64_t)kStCount * kStatSlots * 8, stream));
+    VF_HIP(hipMemsetAsync(nbuf.brick_disp, 0, nbuf.brick_cap * sizeof(uint32_t), stream));
+    VF_HIP(hipMemsetAsync(nbuf.tile_mark, 0, nbuf.tile_cap * sizeof(uint32_t), stream));
+    VF_HIP(hipMemsetAsync(nbuf.vol_mark, 0, nbuf.vol_cap * sizeof(uint32_t), stream));
     VF_HIP(hipStreamSynchronize(stream));
     b = nbuf;
     ws->frame = 0;
@@ -1099,7 +1622,7 @@ inline int32_t ceil_div(int32_t a, int32_t b) { return (a + b - 1) / b; }
 
 // Fills `vd` for one volume, with bricks of shape `shape`.
 int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, const vf_warp_params& p, bool knn_only,
-              int shape, VolDesc* vd, uint64_t* n_out) {
+              int shape, bool screen, VolDesc* vd, uint64_t* n_out) {
     uint64_t n;
     int st = check_volume(vol.dim, vol.voxel_size, &n);
     if (st) return st;
@@ -1156,6 +1679,18 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
             vd->bmag[i] = (uint32_t)((1ull << (24 + l)) / d + 1);
         }
     }
+    // the brick screen: LBS volumes (the blend's displacement bound needs a
+    // convex combination), tile coordinates that fit 16 bits
+    vd->screen = (screen && !knn_only && vd->blend == VF_BLEND_LBS && vd->tdim[0] <= 0xffff &&
+                  vd->tdim[1] <= 0xffff && vd->tdim[2] <= 0xffff)
+                     ? 1
+                     : 0;
+    if (vd->screen) {
+        vd->pw0 = ceil_div(fr.width, kTile0);
+        vd->ph0 = ceil_div(fr.height, kTile0);
+        vd->pw1 = ceil_div(fr.width, kTile1);
+        vd->ph1 = ceil_div(fr.height, kTile1);
+    }
     const bool no_cube = vd->cube[0] < 1 || vd->cube[1] < 1 || vd->cube[2] < 1;
     vd->G = (uint32_t)gr.count;
     const uint64_t k1 = no_cube ? 0 : (uint64_t)vd->G * (uint64_t)(vd->maxb[0] * vd->maxb[1] * vd->maxb[2]);
@@ -1165,12 +1700,15 @@ int make_desc(const vf_volume& vol, const vf_graph& gr, const vf_frame& fr, cons
     return VF_OK;
 }
 
-// Everything a frame's four launches need besides the device buffers.
+// Everything a frame's launches need besides the device buffers.
 struct FrameSetup {
     int nvol = 0;
     int k = 0;
     int shape = 0;
     bool knn_only = false;
+    bool screen = false;        // some volume is screened: k_compact feeds k_select
+    uint64_t blocks_pair = 0;   // k_prep_bin workgroups over (node, brick) pairs
+    uint64_t blocks_pyr = 0;    // k_prep_bin workgroups over depth blocks
     uint32_t k1 = 0;
     uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
     uint64_t tiles = 0;        // collision tiles of this batch
@@ -1229,6 +1767,7 @@ int check_tuning(const vf_tuning& t) {
         return fail(VF_ERR_INVALID_ARGUMENT, "brick shape %d: supported 0..%d", t.brick_shape, kNumShapes - 1);
     if (t.wg_per_cu < 0 || t.wg_per_cu > 64)
         return fail(VF_ERR_INVALID_ARGUMENT, "workgroups per CU %d: supported 0 (occupancy limit) .. 64", t.wg_per_cu);
+    if (t.flags & ~VF_TUNE_NO_SCREEN) return fail(VF_ERR_INVALID_ARGUMENT, "tuning flags 0x%x: unknown bits", t.flags);
     return VF_OK;
 }
 
@@ -1249,14 +1788,29 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     ws->vols_h.resize(nvol);
     ws->vptr_h.resize(nvol);
     Need need;
-    uint64_t voxels = 0, k1 = 0, rec_bound = 0;
+    uint64_t voxels = 0, k1 = 0, rec_bound = 0, pyr_blk = 0;
+    bool any_screen = false;
     for (int v = 0; v < nvol; ++v) {
         const vf_frame empty_frame = {nullptr, 0, 0, {{0}}};
         VolDesc& vd = ws->vols_h[v];
         memset(&vd, 0, sizeof(vd));  // padding too: descriptor sets are compared bytewise
         uint64_t n;
-        st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, shape, &vd, &n);
+        st = make_desc(vols[v], graphs[v], frames ? frames[v] : empty_frame, p[v], knn_only, shape,
+                       !(tuning.flags & VF_TUNE_NO_SCREEN), &vd, &n);
         if (st) return st;
+        if (vd.screen) {
+            any_screen = true;
+            vd.pyr_blk = (uint32_t)pyr_blk;
+            vd.pyr_base0 = (uint32_t)need.pyr;
+            need.pyr += (uint64_t)vd.pw0 * vd.ph0;
+            vd.pyr_base1 = (uint32_t)need.pyr;
+            need.pyr += (uint64_t)vd.pw1 * vd.ph1;
+            pyr_blk += (uint64_t)vd.pw1 * vd.ph1;
+        } else {
+            vd.pyr_blk = (uint32_t)pyr_blk;  // no blocks: keeps the block bases sorted
+        }
+        if (pyr_blk > 0x7fffffffull || need.pyr > 0xffffffffull)
+            return fail(VF_ERR_CAPACITY, "depth frames of the batch too large for the brick screen");
         vd.node_base = (uint32_t)need.nodes;
         vd.brick_base = (uint32_t)need.bricks;
         vd.tile_base = (uint32_t)need.tiles;
@@ -1340,7 +1894,10 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->shape = shape;
     fs->knn_only = knn_only;
     fs->k1 = (uint32_t)k1;
-    fs->blocks_prep = std::max<uint64_t>((k1 + kPrepThreads - 1) / kPrepThreads, 256);
+    fs->blocks_pair = std::max<uint64_t>((k1 + kPrepThreads - 1) / kPrepThreads, 256);
+    fs->blocks_pyr = pyr_blk;
+    fs->blocks_prep = fs->blocks_pair + fs->blocks_pyr;
+    fs->screen = any_screen;
     fs->blocks_a = blocks_a;
     fs->bricks = need.bricks;
     fs->tiles = need.tiles;
@@ -1360,14 +1917,27 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
         for (int i = 0; i < 6; ++i) e[i] = (fs.timing == 2 && (i < 2 || i > 3)) ? nullptr : ev[i];
     const VolDesc* const vols = fs.vols;
     launch(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), stream, e[0], nullptr,
-           vols, fs.nvol, fs.k1, B.nrec, B.brick_count, B.brick_nodes, B.ctrl, B.bitmap, B.tile_flag,
-           (uint32_t)fs.clear_tiles, B.stat);
+           vols, fs.nvol, fs.k1, (uint32_t)fs.blocks_pair, B.nrec, B.brick_count, B.brick_nodes, B.brick_disp,
+           B.dmax, B.ctrl, B.bitmap, B.tile_flag, (uint32_t)fs.clear_tiles, B.stat);
     VF_LAUNCH_CHECK("k_prep_bin");
-    launch(k_compact, dim3((unsigned)((fs.bricks + kPrepThreads * kCompactPer - 1) / (kPrepThreads * kCompactPer))),
-           dim3(kPrepThreads), stream,
-           nullptr, e[1], vols, fs.nvol, (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.active,
-           B.ctrl, B.brick_cap, parity_arg);
-    VF_LAUNCH_CHECK("k_compact");
+    const dim3 grid_c((unsigned)((fs.bricks + kPrepThreads * kCompactPer - 1) / (kPrepThreads * kCompactPer)));
+    if (fs.screen) {
+        launch(k_compact<S, true>, grid_c, dim3(kPrepThreads), stream, nullptr, nullptr, vols, fs.nvol,
+               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, (const uint32_t*)B.brick_disp,
+               (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
+               parity_arg);
+        VF_LAUNCH_CHECK("k_compact");
+        launch(k_select, grid_c, dim3(kPrepThreads), stream, nullptr, e[1], vols, (const uint32_t*)B.cand,
+               (const uint4*)B.cand_info, B.brick_count, B.brick_disp, (const uint32_t*)B.tile_mark,
+               (const uint32_t*)B.vol_mark, B.active, B.ctrl, B.brick_cap, parity_arg, B.stat, B.totals);
+        VF_LAUNCH_CHECK("k_select");
+    } else {
+        launch(k_compact<S, false>, grid_c, dim3(kPrepThreads), stream, nullptr, e[1], vols, fs.nvol,
+               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, (const uint32_t*)B.brick_disp,
+               (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
+               parity_arg);
+        VF_LAUNCH_CHECK("k_compact");
+    }
     Arena arena;
     arena.records = B.records;
     arena.counts = B.arena_counts;
@@ -1471,6 +2041,7 @@ int read_stats(vf_workspace* ws, bool totals, vf_frame_stats* out, hipStream_t s
     out->updated = s[kStUpdated];
     out->active_bricks = s[kStActive];
     out->overflow_bricks = s[kStSpilled];
+    out->screened_bricks = s[kStScreened];
     return VF_OK;
 }
 
@@ -1534,7 +2105,7 @@ struct vf_plan {
     std::vector<vf_graph> graphs;
     std::vector<vf_frame> frames;
     std::vector<vf_warp_params> params;
-    vf_tuning tuning = {0, 0, 0.0f};
+    vf_tuning tuning = {0, 0, 0.0f, 0};
     vf::FrameSetup fs;
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1609,13 +2180,13 @@ int plan_autotune(vf_plan* pl, int frames, vf_tuning* chosen, hipStream_t stream
     }
     VF_HIP(hipEventCreate(&guard.ev[0]));
     VF_HIP(hipEventCreate(&guard.ev[1]));
-    vf_tuning best = {-1, 0, 0.0f};
+    vf_tuning best = {-1, 0, 0.0f, pl->tuning.flags};
     for (int shape = 0; shape < kNumShapes; ++shape) {
         int per_cu = 1;
         int st = pass_a_occupancy(pl->params[0].k, false, shape, &per_cu);
         if (st) return st;
         for (int wg = 1; wg <= per_cu; ++wg) {
-            const vf_tuning t = {shape, wg, 0.0f};
+            const vf_tuning t = {shape, wg, 0.0f, pl->tuning.flags};
             FrameSetup fs;
             st = setup_frame(&tmp, n, vols.data(), graphs.data(), pl->frames.data(), pl->params.data(), false, t, &fs,
                              stream);
@@ -1632,7 +2203,7 @@ int plan_autotune(vf_plan* pl, int frames, vf_tuning* chosen, hipStream_t stream
             float ms = 0.0f;
             VF_HIP(hipEventElapsedTime(&ms, guard.ev[0], guard.ev[1]));
             ms /= (float)frames;
-            if (best.brick_shape < 0 || ms < best.ms_per_frame) best = vf_tuning{shape, wg, ms};
+            if (best.brick_shape < 0 || ms < best.ms_per_frame) best = vf_tuning{shape, wg, ms, t.flags};
         }
     }
     VF_HIP(hipStreamSynchronize(stream));

@@ -1,0 +1,985 @@
+// vf_gn.hip -- Gauss-Newton motion optimisation: the producer of the node
+// transforms the warped integration consumes.
+//
+// The reference runs, per ICP iteration (src/VolumeFusion.cu:434-692):
+// projective correspondences, a dense Jacobian J (rows x nodes of Vec6f),
+// an atomic enumeration of the node pairs that share a row, thrust sorts and
+// reduce_by_key into JTJ / JTr, a dense cuSOLVER LU factorisation and solve,
+// and the twist update T <- T - [w]x|t -- with JTJ rebuilt and refactorised on
+// every Gauss-Newton iteration although its value never changes inside one
+// ICP iteration (J depends on the canonical mesh, the weights, the normals and
+// the node positions only).
+//
+// Here:
+//   - the row structure (which nodes each row couples, and their Jacobian
+//     entries) is built once per ICP iteration from the sparse inputs; no
+//     dense J exists;
+//   - each node's rows are sorted by row index, and one wave per node sums
+//     its 6x6 blocks and its JTr entry over those rows in ascending order
+//     (deterministic; the reference's order is an atomic race);
+//   - JTJ is factorised once (rocSOLVER getrf, the LU with partial pivoting
+//     the reference calls through cuSOLVER), and each Gauss-Newton iteration
+//     only forms JTr from the new residuals and back-substitutes (getrs);
+//   - residuals, energy (binary64, fixed order), update and the mesh warp
+//     stay on the device; one read-back per iteration carries the energy the
+//     reference's stopping rule needs.
+// The reference's per-kernel entry points (residuals, dense J fills, the
+// update, the LU solve, correspondences) are kept for drop-in callers.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "vf_host.hpp"
+
+namespace vf {
+namespace {
+
+constexpr int kGnMaxK = 8;        // mesh knn slots (the reference's motionGraphKNN is 6)
+constexpr int kGnMaxR = 8;        // regularisation neighbours (the reference's energyMRegKNN is 4)
+constexpr int kGnMaxNbr = 128;    // distinct nodes sharing rows with one node
+constexpr int kGnMaxRows = 8192;  // rows of one node (sorted in LDS)
+constexpr uint32_t kGnNone = 0xffffffffu;
+
+inline unsigned blocks_for(uint64_t n, int b) { return (unsigned)((n + (uint64_t)b - 1) / (uint64_t)b); }
+
+__device__ __forceinline__ float sum3(float a0, float a1, float a2) { return a0 + (a1 + a2); }
+
+// Row a of gradTv(p), src/MotionOptimization.cu:10-27, as selects (no
+// branches: with a run-time row index, the if/else-if form was lowered for
+// gfx950 with g[1] = 0 on row 2, seen in the kernel's assembly).
+__device__ __forceinline__ void grad_row(const float p[3], int a, float g[6]) {
+    const bool r0 = a == 0, r1 = a == 1, r2 = a == 2;
+    g[0] = r1 ? -p[2] : (r2 ? p[1] : 0.0f);
+    g[1] = r0 ? p[2] : (r2 ? -p[0] : 0.0f);
+    g[2] = r0 ? -p[1] : (r1 ? p[0] : 0.0f);
+    g[3] = r0 ? 1.0f : 0.0f;
+    g[4] = r1 ? 1.0f : 0.0f;
+    g[5] = r2 ? 1.0f : 0.0f;
+}
+
+// (cCurrDepthNormal * weight).transpose() * gradTv * energyWeight
+// (src/MotionOptimization.cu:57): a three-term sum per column, then the weight.
+__device__ __forceinline__ void j_depth_value(const float n[3], float w, const float p[3], float wD, float out[6]) {
+    const float nw[3] = {n[0] * w, n[1] * w, n[2] * w};
+    float g0[6], g1[6], g2[6];
+    grad_row(p, 0, g0);
+    grad_row(p, 1, g1);
+    grad_row(p, 2, g2);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = sum3(nw[0] * g0[k], nw[1] * g1[k], nw[2] * g2[k]) * wD;
+}
+
+// (M * p4).head(3) with a materialised Vec4f: (a0 + a1) + (a2 + a3).
+__device__ __forceinline__ void mat4_apply4(const float* M, const float p[3], float o[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float a0 = M[0 * 4 + i] * p[0];
+        const float a1 = M[1 * 4 + i] * p[1];
+        const float a2 = M[2 * 4 + i] * p[2];
+        const float a3 = M[3 * 4 + i] * 1.0f;
+        o[i] = (a0 + a1) + (a2 + a3);
+    }
+}
+
+__device__ __forceinline__ void load3(const float* p, float o[3]) {
+    o[0] = p[0];
+    o[1] = p[1];
+    o[2] = p[2];
+}
+
+// ---- correspondences (src/ComputeCorrespondence.cu:46-82) --------------------
+
+struct CorrIn {
+    const float* warped;
+    uint64_t n;
+    float fx, fy, cx, cy;
+    const float* depth;
+    int32_t rows, cols;
+    const float* frame_mesh;
+    float thresh;
+};
+
+// The pixel of mesh point i when it forms a pair, else -1.
+__device__ __forceinline__ int64_t corr_pixel(const CorrIn& in, uint64_t i) {
+    float v[3];
+    load3(in.warped + 3 * i, v);
+    const float y0 = v[1] / v[2];
+    const float x0 = v[0] / v[2];
+    const int32_t y = cvt_i32_rz(y0 * in.fy + in.cy);
+    const int32_t x = cvt_i32_rz(x0 * in.fx + in.cx);
+    if (y < 0 || x < 0 || y >= in.rows || x >= in.cols) return -1;
+    const int64_t d = (int64_t)y * in.cols + x;
+    const float* fm = in.frame_mesh + 3 * d;
+    const float dx = fm[0] - v[0], dy = fm[1] - v[1], dz = fm[2] - v[2];
+    const float nrm = __builtin_sqrtf(sum3(dx * dx, dy * dy, dz * dz));
+    if (nrm > in.thresh) return -1;
+    return in.depth[d] > 1.e-4f ? d : -1;
+}
+
+constexpr int kCorrBlock = 256;
+
+__global__ __launch_bounds__(kCorrBlock) void k_corr_count(CorrIn in, uint32_t* __restrict__ block_count) {
+    const uint64_t i = (uint64_t)blockIdx.x * kCorrBlock + threadIdx.x;
+    const bool ok = i < in.n && corr_pixel(in, i) >= 0;
+    const unsigned long long m = __ballot(ok);
+    __shared__ uint32_t s_w[kCorrBlock / 64];
+    if (__lane_id() == 0) s_w[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kCorrBlock / 64; ++w) t += s_w[w];
+        block_count[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the block counts after *size; advances *size.
+__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ counts, uint32_t nb,
+                                                      uint32_t* __restrict__ size) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = b + per < nb ? b + per : nb;
+    uint32_t t = 0;
+    for (uint32_t i = b; i < e; ++i) t += counts[i];
+    s_part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = size ? *size : 0u;
+        for (int i = 0; i < 1024; ++i) {
+            const uint32_t v = s_part[i];
+            s_part[i] = run;
+            run += v;
+        }
+        if (size) *size = run;
+    }
+    __syncthreads();
+    uint32_t run = s_part[threadIdx.x];
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = counts[i];
+        counts[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kCorrBlock) void k_corr_write(CorrIn in, const uint32_t* __restrict__ block_off,
+                                                           int32_t* __restrict__ corr) {
+    const uint64_t i = (uint64_t)blockIdx.x * kCorrBlock + threadIdx.x;
+    const int64_t d = i < in.n ? corr_pixel(in, i) : -1;
+    const bool ok = d >= 0;
+    const unsigned long long m = __ballot(ok);
+    __shared__ uint32_t s_w[kCorrBlock / 64];
+    const int wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) s_w[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (ok) {
+        uint32_t pos = block_off[blockIdx.x];
+        for (int w = 0; w < wave; ++w) pos += s_w[w];
+        pos += (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        corr[2ull * pos] = (int32_t)i;
+        corr[2ull * pos + 1] = (int32_t)d;
+    }
+}
+
+// ---- residuals, dense Jacobians, update ----------------------------------------
+
+// fill_r_DepthKernel (src/MotionOptimization.cu:165-181).
+__global__ void k_r_depth(float* __restrict__ r, float wD, const int32_t* __restrict__ corr, uint32_t C,
+                          const float* __restrict__ warped, const float* __restrict__ dmesh,
+                          const float* __restrict__ dnormals) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float v[3], d[3], nn[3];
+    load3(warped + 3ull * (uint32_t)corr[2 * c], v);
+    load3(dmesh + 3ull * (uint32_t)corr[2 * c + 1], d);
+    load3(dnormals + 3ull * (uint32_t)corr[2 * c + 1], nn);
+    const float df0 = v[0] - d[0], df1 = v[1] - d[1], df2 = v[2] - d[2];
+    r[c] = wD * sum3(nn[0] * df0, nn[1] * df1, nn[2] * df2);
+}
+
+// fill_r_MRegKernel (src/MotionOptimization.cu:201-225), one thread per
+// (node, neighbour slot).
+__global__ void k_r_mreg(float* __restrict__ r, float wR, const float* __restrict__ graph,
+                         const float* __restrict__ T, uint32_t G, const uint32_t* __restrict__ gknn, int R) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G * (uint32_t)R) return;
+    const uint32_t g = t / (uint32_t)R;
+    const uint32_t nb = gknn[t];
+    if (nb >= G) return;
+    float p[3], tc[3], tn[3];
+    load3(graph + 3ull * g, p);
+    mat4_apply4(T + 16ull * g, p, tc);
+    mat4_apply4(T + 16ull * nb, p, tn);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r[3ull * t + a] = (tc[a] - tn[a]) * wR;
+}
+
+// fill_J_DepthKernel (src/MotionOptimization.cu:29-60) into the dense J.
+__global__ void k_j_depth(float* __restrict__ J, float wD, uint32_t jW, const int32_t* __restrict__ corr, uint32_t C,
+                          const float* __restrict__ canon, int K, const uint32_t* __restrict__ knn,
+                          const float* __restrict__ kw, const float* __restrict__ dnormals) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t ci = (uint32_t)corr[2 * c], di = (uint32_t)corr[2 * c + 1];
+    float p[3], nn[3];
+    load3(canon + 3ull * ci, p);
+    load3(dnormals + 3ull * di, nn);
+    for (int s = 0; s < K; ++s) {
+        const float w = kw[(uint64_t)K * ci + s];
+        if (w < 1.e-4f) continue;
+        const uint32_t j = knn[(uint64_t)K * ci + s];
+        float v[6];
+        j_depth_value(nn, w, p, wD, v);
+        float* o = J + 6ull * ((uint64_t)c * jW + j);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = v[k];
+    }
+}
+
+// fill_J_MRegKernel (src/MotionOptimization.cu:80-145) into the dense J.
+__global__ void k_j_mreg(float* __restrict__ J, float wR, uint32_t jW, const float* __restrict__ graph, uint32_t G,
+                         const uint32_t* __restrict__ gknn, int R) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G * (uint32_t)R) return;
+    const uint32_t g = t / (uint32_t)R;
+    const uint32_t nb = gknn[t];
+    if (nb >= G) return;
+    float p[3];
+    load3(graph + 3ull * g, p);
+    for (int a = 0; a < 3; ++a) {
+        float gr[6];
+        grad_row(p, a, gr);
+        float* jc = J + 6ull * ((3ull * t + a) * jW + g);
+        float* jn = J + 6ull * ((3ull * t + a) * jW + nb);
+        for (int k = 0; k < 6; ++k) jc[k] = gr[k] * wR;
+        for (int k = 0; k < 6; ++k) jn[k] = -(gr[k] * wR);
+    }
+}
+
+// updateTransformsKernel (src/MotionOptimization.cu:481-498): T <- T - U.
+__global__ void k_update(float* __restrict__ T, const float* __restrict__ u, uint32_t G) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const float* x = u + 6ull * g;
+    float* M = T + 16ull * g;
+    // column-major M(i, j) = M[4 j + i]
+    M[4 * 1 + 0] = M[4 * 1 + 0] - (-x[2]);
+    M[4 * 2 + 0] = M[4 * 2 + 0] - x[1];
+    M[4 * 3 + 0] = M[4 * 3 + 0] - x[3];
+    M[4 * 0 + 1] = M[4 * 0 + 1] - x[2];
+    M[4 * 2 + 1] = M[4 * 2 + 1] - (-x[0]);
+    M[4 * 3 + 1] = M[4 * 3 + 1] - x[4];
+    M[4 * 0 + 2] = M[4 * 0 + 2] - (-x[1]);
+    M[4 * 1 + 2] = M[4 * 1 + 2] - x[0];
+    M[4 * 3 + 2] = M[4 * 3 + 2] - x[5];
+    // the zero entries of U: T - 0 is T except for -0 (-0 - 0 = -0): unchanged
+}
+
+// ---- normal equations from the sparse rows --------------------------------------
+//
+// Row r < C is a depth row: its members are the knn slots of mesh point
+// corr[r].x with a non-zero weight (the pairs of
+// computeJTJAndJTrContribJDepthKernel, src/MotionOptimization.cu:227-259),
+// each with the Jacobian entry fill_J_DepthKernel writes (zero below weight
+// 1e-4).  Row r >= C is regularisation row 3 (g R + i) + a: members g and its
+// neighbour nb (computeJTJAndJTrContribJMRegKernel, :284-314); when nb == g the
+// three pairs coincide (the diagonal block takes the product three times, JTr
+// the entry twice, with the entry of the last write, -grad).
+
+struct GnRows {
+    const int32_t* corr;
+    uint32_t C;
+    const float* canon;
+    int K;
+    const uint32_t* knn;
+    const float* kw;
+    const float* dnormals;
+    const float* graph;
+    uint32_t G;
+    const uint32_t* gknn;
+    int R;
+    float wD, wR;
+};
+
+struct Member {
+    uint32_t node;
+    int diag;  // multiplicity on the diagonal block
+    int jtr;   // multiplicity in JTr
+    float j[6];
+};
+
+// Members of row `row` (up to kGnMaxK); returns their count.
+__device__ int row_members(const GnRows& g, uint32_t row, Member* m) {
+    if (row < g.C) {
+        const uint32_t ci = (uint32_t)g.corr[2 * row], di = (uint32_t)g.corr[2 * row + 1];
+        float p[3], nn[3];
+        load3(g.canon + 3ull * ci, p);
+        load3(g.dnormals + 3ull * di, nn);
+        int n = 0;
+        for (int s = 0; s < g.K; ++s) {
+            const float w = g.kw[(uint64_t)g.K * ci + s];
+            if (w == 0.0f) continue;
+            Member& e = m[n++];
+            e.node = g.knn[(uint64_t)g.K * ci + s];
+            e.diag = e.jtr = 1;
+            if (w < 1.e-4f) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) e.j[k] = 0.0f;
+            } else {
+                j_depth_value(nn, w, p, g.wD, e.j);
+            }
+        }
+        return n;
+    }
+    const uint32_t mr = (row - g.C) / 3u, a = (row - g.C) % 3u;
+    const uint32_t node = mr / (uint32_t)g.R;
+    const uint32_t nb = g.gknn[mr];
+    if (nb >= g.G) return 0;
+    float p[3], gr[6];
+    load3(g.graph + 3ull * node, p);
+    grad_row(p, (int)a, gr);
+    if (nb == node) {
+        m[0].node = node;
+        m[0].diag = 3;
+        m[0].jtr = 2;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) m[0].j[k] = -(gr[k] * g.wR);
+        return 1;
+    }
+    m[0].node = node;
+    m[1].node = nb;
+    m[0].diag = m[0].jtr = m[1].diag = m[1].jtr = 1;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        m[0].j[k] = gr[k] * g.wR;
+        m[1].j[k] = -(gr[k] * g.wR);
+    }
+    return 2;
+}
+
+// Counts each node's rows; flags a member node outside the graph.
+__global__ void k_rows_count(GnRows g, uint32_t* __restrict__ cnt, uint32_t* __restrict__ bad) {
+    const uint32_t rows = g.C + 3u * g.G * (uint32_t)g.R;
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    if (row < g.C) {
+        const uint32_t ci = (uint32_t)g.corr[2 * row];
+        for (int s = 0; s < g.K; ++s) {
+            if (g.kw[(uint64_t)g.K * ci + s] == 0.0f) continue;
+            const uint32_t j = g.knn[(uint64_t)g.K * ci + s];
+            if (j >= g.G) {
+                *bad = 1u;
+                continue;
+            }
+            atomicAdd(cnt + j, 1u);
+        }
+        return;
+    }
+    const uint32_t mr = (row - g.C) / 3u;
+    const uint32_t node = mr / (uint32_t)g.R, nb = g.gknn[mr];
+    if (nb >= g.G) return;
+    atomicAdd(cnt + node, 1u);
+    if (nb != node) atomicAdd(cnt + nb, 1u);
+}
+
+__global__ void k_rows_fill(GnRows g, const uint32_t* __restrict__ off, uint32_t* __restrict__ fill,
+                            uint32_t* __restrict__ rows_of) {
+    const uint32_t rows = g.C + 3u * g.G * (uint32_t)g.R;
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    if (row < g.C) {
+        const uint32_t ci = (uint32_t)g.corr[2 * row];
+        for (int s = 0; s < g.K; ++s) {
+            if (g.kw[(uint64_t)g.K * ci + s] == 0.0f) continue;
+            const uint32_t j = g.knn[(uint64_t)g.K * ci + s];
+            if (j >= g.G) continue;
+            rows_of[off[j] + atomicAdd(fill + j, 1u)] = row;
+        }
+        return;
+    }
+    const uint32_t mr = (row - g.C) / 3u;
+    const uint32_t node = mr / (uint32_t)g.R, nb = g.gknn[mr];
+    if (nb >= g.G) return;
+    rows_of[off[node] + atomicAdd(fill + node, 1u)] = row;
+    if (nb != node) rows_of[off[nb] + atomicAdd(fill + nb, 1u)] = row;
+}
+
+// Sorts each node's row list (bitonic sort in LDS, one workgroup per node).
+__global__ __launch_bounds__(1024) void k_rows_sort(const uint32_t* __restrict__ off, uint32_t* __restrict__ rows_of,
+                                                    uint32_t* __restrict__ bad) {
+    __shared__ uint32_t s[kGnMaxRows];
+    const uint32_t node = blockIdx.x;
+    const uint32_t b = off[node], n = off[node + 1] - b;
+    if (n > (uint32_t)kGnMaxRows) {
+        if (threadIdx.x == 0) *bad = 2u;
+        return;
+    }
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) s[i] = i < n ? rows_of[b + i] : kGnNone;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = s[i], y = s[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        s[i] = y;
+                        s[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) rows_of[b + i] = s[i];
+}
+
+// Zero, then the identity diagonal (runInitIdentityKernel before the fill).
+__global__ void k_identity(float* __restrict__ A, uint64_t n) {
+    const uint64_t total = n * n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        A[i] = (i / n == i % n) ? 1.0f : 0.0f;
+}
+
+// One wave per node i: its 6x6 blocks JTJ(i, j) for every node j sharing a
+// row with it, summed over i's rows in ascending order (each accumulator
+// starts at -0, the additive identity, so the first term is taken as is),
+// and, when r is given, its JTr entry.  Lanes 0..35 own the 36 elements of a
+// block; lanes 0..5 the JTr entry.
+__global__ __launch_bounds__(64) void k_blocks(GnRows g, const uint32_t* __restrict__ off,
+                                               const uint32_t* __restrict__ rows_of, const float* __restrict__ r,
+                                               float* __restrict__ JTJ, float* __restrict__ JTr,
+                                               uint32_t* __restrict__ bad) {
+    __shared__ uint32_t s_ids[kGnMaxNbr];
+    __shared__ float s_acc[kGnMaxNbr][36];
+    __shared__ Member s_m[kGnMaxK];
+    __shared__ int s_n, s_me;
+    const uint32_t i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t b = off[i], n = off[i + 1] - b;
+    int nids = 0;
+    float jtr = -0.0f;
+    if (JTJ) {
+        // distinct nodes of i's rows, in first-seen order
+        for (uint32_t e = 0; e < n; ++e) {
+            if (lane == 0) s_n = row_members(g, rows_of[b + e], s_m);
+            __syncthreads();
+            for (int q = 0; q < s_n; ++q) {
+                const uint32_t j = s_m[q].node;
+                const bool seen = __ballot((lane < nids && s_ids[lane] == j) ||
+                                           (lane + 64 < nids && s_ids[lane + 64] == j)) != 0;
+                if (!seen) {
+                    if (nids >= kGnMaxNbr) {
+                        if (lane == 0) *bad = 3u;
+                        return;
+                    }
+                    if (lane == 0) s_ids[nids] = j;
+                    ++nids;
+                }
+                __syncthreads();
+            }
+        }
+        for (int t = 0; t < nids; ++t)
+            if (lane < 36) s_acc[t][lane] = -0.0f;
+        __syncthreads();
+    }
+    for (uint32_t e = 0; e < n; ++e) {
+        const uint32_t row = rows_of[b + e];
+        if (lane == 0) {
+            s_n = row_members(g, row, s_m);
+            s_me = -1;
+            for (int q = 0; q < s_n; ++q)
+                if (s_m[q].node == i) s_me = q;
+        }
+        __syncthreads();
+        if (s_me < 0) {  // cannot happen: the row is in i's list because i is a member
+            if (lane == 0) *bad = 4u;
+            return;
+        }
+        const Member& mi = s_m[s_me];
+        if (JTr && lane < 6) {
+            const float rv = r[row];
+            for (int t = 0; t < mi.jtr; ++t) jtr = jtr + mi.j[lane] * rv;
+        }
+        if (JTJ) {
+            for (int q = 0; q < s_n; ++q) {
+                const Member& mj = s_m[q];
+                // the slot of mj.node: lanes test two list entries each
+                const unsigned long long h0 = __ballot(lane < nids && s_ids[lane] == mj.node);
+                const unsigned long long h1 = __ballot(lane + 64 < nids && s_ids[lane + 64] == mj.node);
+                const int slot = h0 ? __ffsll((long long)h0) - 1 : 64 + __ffsll((long long)h1) - 1;
+                const int mult = mj.node == i ? mi.diag : 1;
+                if (lane < 36) {
+                    const float v = mi.j[lane / 6] * mj.j[lane % 6];
+                    for (int t = 0; t < mult; ++t) s_acc[slot][lane] = s_acc[slot][lane] + v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (JTr && lane < 6) JTr[6ull * i + lane] = n ? jtr : 0.0f;
+    if (JTJ && lane < 36) {
+        const uint64_t W = 6ull * g.G;
+        const int a = lane / 6, c = lane % 6;
+        for (int t = 0; t < nids; ++t)
+            JTJ[(6ull * i + a) * W + 6ull * s_ids[t] + c] = s_acc[t][lane];
+    }
+}
+
+// Sum of squares in binary64, fixed order: each workgroup sums a contiguous
+// range, then one thread adds the workgroups' sums.
+constexpr int kEnergyBlocks = 256;
+__global__ __launch_bounds__(256) void k_energy_part(const float* __restrict__ r, uint64_t n,
+                                                     double* __restrict__ part) {
+    __shared__ double s[256];
+    const uint64_t per = (n + kEnergyBlocks - 1) / kEnergyBlocks;
+    const uint64_t b = blockIdx.x * per, e = b + per < n ? b + per : n;
+    double t = 0.0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += 256) t += (double)r[i] * (double)r[i];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void k_energy_sum(const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kEnergyBlocks; ++i) t += part[i];
+        *out = t;
+    }
+}
+
+}  // namespace
+}  // namespace vf
+
+// ---- host side -------------------------------------------------------------------
+
+struct vf_gn_solver {
+    int device = 0;
+    vf::GnRows rows;
+    uint32_t nrows = 0;
+    uint64_t n = 0;  // 6 G
+    // device buffers
+    uint32_t* cnt = nullptr;   // G + 1: counts, then offsets
+    uint32_t* fill = nullptr;  // G
+    uint32_t* rows_of = nullptr;
+    uint32_t* bad = nullptr;
+    float* JTJ = nullptr;  // LU factors after the factorisation
+    int* ipiv = nullptr;
+    int* info = nullptr;
+    float* r = nullptr;    // residuals, nrows
+    float* x = nullptr;    // JTr, then the update
+    double* part = nullptr;
+    double* energy = nullptr;
+    rocblas_handle blas = nullptr;
+};
+
+namespace vf {
+namespace {
+
+void gn_free(vf_gn_solver* s) {
+    void* ptrs[] = {s->cnt, s->fill, s->rows_of, s->bad, s->JTJ, s->ipiv, s->info, s->r, s->x, s->part, s->energy};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->blas) (void)rocblas_destroy_handle(s->blas);
+}
+
+template <typename T>
+int gn_alloc(T** p, uint64_t count) {
+    const hipError_t e = hipMalloc((void**)p, std::max<uint64_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? VF_ERR_OUT_OF_MEMORY : VF_ERR_HIP, "gn allocation of %llu B: %s",
+                    (unsigned long long)(count * sizeof(T)), hipGetErrorString(e));
+    }
+    return VF_OK;
+}
+
+int check_problem(const vf_gn_problem* p) {
+    if (!p) return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn: NULL problem");
+    if (p->k < 1 || p->k > kGnMaxK) return fail(VF_ERR_UNSUPPORTED_K, "gn: k=%d, supported 1..%d", p->k, kGnMaxK);
+    if (p->reg_k < 1 || p->reg_k > kGnMaxR)
+        return fail(VF_ERR_INVALID_ARGUMENT, "gn: reg_k=%d, supported 1..%d", p->reg_k, kGnMaxR);
+    if (p->graph_size < 1 || p->graph_size > (1u << 20))
+        return fail(VF_ERR_INVALID_ARGUMENT, "gn: graph of %u nodes, supported 1..2^20", p->graph_size);
+    if (!p->graph || !p->graph_knn || (p->corr_size && (!p->corr || !p->canon || !p->knn_idxs || !p->knn_weights ||
+                                                         !p->normals)))
+        return fail(VF_ERR_INVALID_ARGUMENT, "gn: NULL problem buffer");
+    const uint64_t rows = (uint64_t)p->corr_size + 3ull * p->graph_size * (uint64_t)p->reg_k;
+    if (rows > 0x7fffffffull) return fail(VF_ERR_CAPACITY, "gn: %llu rows exceed 2^31", (unsigned long long)rows);
+    const uint64_t n = 6ull * p->graph_size;
+    if (n * n > (1ull << 34))
+        return fail(VF_ERR_CAPACITY, "gn: the dense normal matrix of %u nodes exceeds 64 GiB", p->graph_size);
+    return VF_OK;
+}
+
+GnRows make_rows(const vf_gn_problem* p) {
+    GnRows g;
+    g.corr = (const int32_t*)p->corr;
+    g.C = p->corr_size;
+    g.canon = (const float*)p->canon;
+    g.K = p->k;
+    g.knn = p->knn_idxs;
+    g.kw = p->knn_weights;
+    g.dnormals = (const float*)p->normals;
+    g.graph = (const float*)p->graph;
+    g.G = p->graph_size;
+    g.gknn = p->graph_knn;
+    g.R = p->reg_k;
+    g.wD = p->weight_depth;
+    g.wR = p->weight_reg;
+    return g;
+}
+
+// Builds each node's sorted row list; returns the host-read status.
+int build_rows(vf_gn_solver* s, hipStream_t st) {
+    const GnRows& g = s->rows;
+    VF_HIP(hipMemsetAsync(s->cnt, 0, sizeof(uint32_t) * (g.G + 1ull), st));
+    VF_HIP(hipMemsetAsync(s->fill, 0, sizeof(uint32_t) * g.G, st));
+    VF_HIP(hipMemsetAsync(s->bad, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_rows_count, dim3(blocks_for(s->nrows, 256)), dim3(256), 0, st, g, s->cnt, s->bad);
+    VF_LAUNCH_CHECK("k_rows_count");
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, s->cnt, g.G + 1u, (uint32_t*)nullptr);
+    VF_LAUNCH_CHECK("k_scan_counts");
+    uint32_t total = 0;
+    VF_HIP(hipMemcpyAsync(&total, s->cnt + g.G, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VF_HIP(hipStreamSynchronize(st));
+    VF_HIP(hipFree(s->rows_of));
+    s->rows_of = nullptr;
+    int e = gn_alloc(&s->rows_of, total);
+    if (e) return e;
+    hipLaunchKernelGGL(k_rows_fill, dim3(blocks_for(s->nrows, 256)), dim3(256), 0, st, g, (const uint32_t*)s->cnt,
+                       s->fill, s->rows_of);
+    VF_LAUNCH_CHECK("k_rows_fill");
+    hipLaunchKernelGGL(k_rows_sort, dim3(g.G), dim3(1024), 0, st, (const uint32_t*)s->cnt, s->rows_of, s->bad);
+    VF_LAUNCH_CHECK("k_rows_sort");
+    return VF_OK;
+}
+
+int read_bad(vf_gn_solver* s, hipStream_t st) {
+    uint32_t bad = 0;
+    VF_HIP(hipMemcpyAsync(&bad, s->bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VF_HIP(hipStreamSynchronize(st));
+    switch (bad) {
+        case 0: return VF_OK;
+        case 1: return fail(VF_ERR_INVALID_ARGUMENT, "gn: a mesh point's knn names a node outside the graph");
+        case 2: return fail(VF_ERR_CAPACITY, "gn: a node belongs to more than %d rows", kGnMaxRows);
+        case 3: return fail(VF_ERR_CAPACITY, "gn: a node shares rows with more than %d nodes", kGnMaxNbr);
+        default: return fail(VF_ERR_HIP, "gn: inconsistent row lists");
+    }
+}
+
+int launch_residuals(vf_gn_solver* s, const float* warped, const vf_float3* dmesh, const vf_mat4* T, hipStream_t st) {
+    const GnRows& g = s->rows;
+    VF_HIP(hipMemsetAsync(s->r, 0, sizeof(float) * s->nrows, st));
+    if (g.C) {
+        hipLaunchKernelGGL(k_r_depth, dim3(blocks_for(g.C, 256)), dim3(256), 0, st, s->r, g.wD, g.corr, g.C, warped,
+                           (const float*)dmesh, g.dnormals);
+        VF_LAUNCH_CHECK("k_r_depth");
+    }
+    hipLaunchKernelGGL(k_r_mreg, dim3(blocks_for((uint64_t)g.G * g.R, 256)), dim3(256), 0, st, s->r + g.C, g.wR,
+                       g.graph, (const float*)T, g.G, g.gknn, g.R);
+    VF_LAUNCH_CHECK("k_r_mreg");
+    hipLaunchKernelGGL(k_energy_part, dim3(kEnergyBlocks), dim3(256), 0, st, (const float*)s->r, (uint64_t)s->nrows,
+                       s->part);
+    VF_LAUNCH_CHECK("k_energy_part");
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(64), 0, st, (const double*)s->part, s->energy);
+    VF_LAUNCH_CHECK("k_energy_sum");
+    return VF_OK;
+}
+
+}  // namespace
+}  // namespace vf
+
+using namespace vf;
+
+extern "C" {
+
+int vf_correspondence_pairs(vf_workspace* ws, vf_int2* corr, uint32_t* corr_size, const vf_float3* warped,
+                            size_t mesh_size, vf_mat3 K, const float* depth, vf_int2 depth_dim,
+                            const vf_float3* frame_mesh, float thresh, vf_stream stream) {
+    if (!mesh_size) return VF_OK;
+    if (!corr || !corr_size || !warped || !depth || !frame_mesh)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_correspondence_pairs: NULL argument");
+    if (depth_dim.x < 0 || depth_dim.y < 0 || mesh_size > 0x7fffffffull)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_correspondence_pairs: bad sizes");
+    CorrIn in;
+    in.warped = (const float*)warped;
+    in.n = mesh_size;
+    in.fx = K.m[0];
+    in.fy = K.m[4];
+    in.cx = K.m[6];
+    in.cy = K.m[7];
+    in.depth = depth;
+    in.rows = depth_dim.x;
+    in.cols = depth_dim.y;
+    in.frame_mesh = (const float*)frame_mesh;
+    in.thresh = thresh;
+    const unsigned nb = blocks_for(mesh_size, kCorrBlock);
+    void* mem = nullptr;
+    int st = workspace_scratch(ws, sizeof(uint32_t) * (nb + 1ull), &mem, (hipStream_t)stream);
+    if (st) return st;
+    uint32_t* counts = (uint32_t*)mem;
+    hipLaunchKernelGGL(k_corr_count, dim3(nb), dim3(kCorrBlock), 0, (hipStream_t)stream, in, counts);
+    VF_LAUNCH_CHECK("k_corr_count");
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, (hipStream_t)stream, counts, nb, corr_size);
+    VF_LAUNCH_CHECK("k_scan_counts");
+    hipLaunchKernelGGL(k_corr_write, dim3(nb), dim3(kCorrBlock), 0, (hipStream_t)stream, in, (const uint32_t*)counts,
+                       (int32_t*)corr);
+    VF_LAUNCH_CHECK("k_corr_write");
+    return VF_OK;
+}
+
+int vf_fill_r_depth(float* r, float weight, const vf_int2* corr, uint32_t corr_size, const vf_float3* warped,
+                    const vf_float3* depth_mesh, const vf_float3* normals, vf_stream stream) {
+    if (!corr_size) return VF_OK;
+    if (!r || !corr || !warped || !depth_mesh || !normals) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_r_depth: NULL");
+    hipLaunchKernelGGL(k_r_depth, dim3(blocks_for(corr_size, 256)), dim3(256), 0, (hipStream_t)stream, r, weight,
+                       (const int32_t*)corr, corr_size, (const float*)warped, (const float*)depth_mesh,
+                       (const float*)normals);
+    VF_LAUNCH_CHECK("k_r_depth");
+    return VF_OK;
+}
+
+int vf_fill_r_mreg(float* r, float weight, const vf_float3* graph, const vf_mat4* transforms, uint32_t graph_size,
+                   const uint32_t* graph_knn, int reg_k, vf_stream stream) {
+    if (!graph_size) return VF_OK;
+    if (reg_k < 1 || reg_k > kGnMaxR) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_r_mreg: reg_k=%d", reg_k);
+    if (!r || !graph || !transforms || !graph_knn) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_r_mreg: NULL");
+    hipLaunchKernelGGL(k_r_mreg, dim3(blocks_for((uint64_t)graph_size * reg_k, 256)), dim3(256), 0,
+                       (hipStream_t)stream, r, weight, (const float*)graph, (const float*)transforms, graph_size,
+                       graph_knn, reg_k);
+    VF_LAUNCH_CHECK("k_r_mreg");
+    return VF_OK;
+}
+
+int vf_fill_j_depth(float* J, float weight, uint32_t j_height, uint32_t j_width, const vf_int2* corr,
+                    uint32_t corr_size, const vf_float3* canon, int k, const uint32_t* knn_idxs,
+                    const float* knn_weights, const vf_float3* depth_mesh, const vf_float3* normals,
+                    vf_stream stream) {
+    (void)depth_mesh;
+    if (!corr_size) return VF_OK;
+    if (k < 1 || k > kGnMaxK) return fail(VF_ERR_UNSUPPORTED_K, "vf_fill_j_depth: k=%d", k);
+    if (corr_size > j_height) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_j_depth: more pairs than J rows");
+    if (!J || !corr || !canon || !knn_idxs || !knn_weights || !normals)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_j_depth: NULL");
+    hipLaunchKernelGGL(k_j_depth, dim3(blocks_for(corr_size, 256)), dim3(256), 0, (hipStream_t)stream, J, weight,
+                       j_width, (const int32_t*)corr, corr_size, (const float*)canon, k, knn_idxs, knn_weights,
+                       (const float*)normals);
+    VF_LAUNCH_CHECK("k_j_depth");
+    return VF_OK;
+}
+
+int vf_fill_j_mreg(float* J, float weight, uint32_t j_height, uint32_t j_width, const vf_float3* graph,
+                   const vf_mat4* transforms, uint32_t graph_size, const uint32_t* graph_knn, int reg_k,
+                   vf_stream stream) {
+    (void)transforms;
+    if (!graph_size) return VF_OK;
+    if (reg_k < 1 || reg_k > kGnMaxR) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_j_mreg: reg_k=%d", reg_k);
+    if ((uint64_t)graph_size * reg_k * 3 > j_height || graph_size > j_width)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_j_mreg: J too small");
+    if (!J || !graph || !graph_knn) return fail(VF_ERR_INVALID_ARGUMENT, "vf_fill_j_mreg: NULL");
+    hipLaunchKernelGGL(k_j_mreg, dim3(blocks_for((uint64_t)graph_size * reg_k, 256)), dim3(256), 0,
+                       (hipStream_t)stream, J, weight, j_width, (const float*)graph, graph_size, graph_knn, reg_k);
+    VF_LAUNCH_CHECK("k_j_mreg");
+    return VF_OK;
+}
+
+int vf_update_transforms(vf_mat4* transforms, const float* updates, uint32_t graph_size, vf_stream stream) {
+    if (!graph_size) return VF_OK;
+    if (!transforms || !updates) return fail(VF_ERR_INVALID_ARGUMENT, "vf_update_transforms: NULL");
+    hipLaunchKernelGGL(k_update, dim3(blocks_for(graph_size, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (float*)transforms, updates, graph_size);
+    VF_LAUNCH_CHECK("k_update");
+    return VF_OK;
+}
+
+int vf_lu_solve(int n, const float* A, int lda, const float* b, float* x, vf_stream stream) {
+    if (n < 0 || lda < std::max(n, 1)) return fail(VF_ERR_INVALID_ARGUMENT, "vf_lu_solve: n=%d lda=%d", n, lda);
+    if (!n) return VF_OK;
+    if (!A || !b || !x) return fail(VF_ERR_INVALID_ARGUMENT, "vf_lu_solve: NULL");
+    hipStream_t st = (hipStream_t)stream;
+    float* Ac = nullptr;
+    int* ipiv = nullptr;
+    int* info = nullptr;
+    rocblas_handle h = nullptr;
+    int rc = VF_OK;
+    auto cleanup = [&]() {
+        if (Ac) (void)hipFree(Ac);
+        if (ipiv) (void)hipFree(ipiv);
+        if (info) (void)hipFree(info);
+        if (h) (void)rocblas_destroy_handle(h);
+    };
+    if (!rc) rc = gn_alloc(&Ac, (uint64_t)lda * n);
+    if (!rc) rc = gn_alloc(&ipiv, (uint64_t)n);
+    if (!rc) rc = gn_alloc(&info, 1);
+    if (!rc && rocblas_create_handle(&h) != rocblas_status_success) rc = fail(VF_ERR_HIP, "rocblas_create_handle");
+    if (!rc && rocblas_set_stream(h, st) != rocblas_status_success) rc = fail(VF_ERR_HIP, "rocblas_set_stream");
+    // getrf overwrites A with its factors: factor a copy, solve into x
+    if (!rc && hipMemcpyAsync(Ac, A, sizeof(float) * (uint64_t)lda * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = fail(VF_ERR_HIP, "vf_lu_solve: copy of A");
+    if (!rc && hipMemcpyAsync(x, b, sizeof(float) * (uint64_t)n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = fail(VF_ERR_HIP, "vf_lu_solve: copy of b");
+    if (!rc && rocsolver_sgetrf(h, n, n, Ac, lda, ipiv, info) != rocblas_status_success)
+        rc = fail(VF_ERR_HIP, "rocsolver_sgetrf");
+    int hinfo = 0;
+    if (!rc && (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess))
+        rc = fail(VF_ERR_HIP, "vf_lu_solve: info read-back");
+    if (!rc && hinfo != 0) rc = fail(VF_ERR_SINGULAR, "vf_lu_solve: zero pivot at column %d", hinfo);
+    if (!rc && rocsolver_sgetrs(h, rocblas_operation_none, n, 1, Ac, lda, ipiv, x, n) != rocblas_status_success)
+        rc = fail(VF_ERR_HIP, "rocsolver_sgetrs");
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(VF_ERR_HIP, "vf_lu_solve: synchronize");
+    cleanup();
+    return rc;
+}
+
+int vf_gn_normal_equations(vf_workspace* ws, float* JTJ, float* JTr, const float* r, const vf_gn_problem* problem,
+                           vf_stream stream) {
+    (void)ws;
+    int st = check_problem(problem);
+    if (st) return st;
+    if (!JTJ && !JTr) return VF_OK;
+    if (JTr && !r) return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_normal_equations: JTr needs r");
+    vf_gn_solver s;
+    VF_HIP(hipGetDevice(&s.device));
+    s.rows = make_rows(problem);
+    s.nrows = s.rows.C + 3u * s.rows.G * (uint32_t)s.rows.R;
+    s.n = 6ull * s.rows.G;
+    hipStream_t hs = (hipStream_t)stream;
+    if (!st) st = gn_alloc(&s.cnt, s.rows.G + 1ull);
+    if (!st) st = gn_alloc(&s.fill, s.rows.G);
+    if (!st) st = gn_alloc(&s.bad, 1);
+    if (!st) st = build_rows(&s, hs);
+    if (!st && JTJ) {
+        hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, JTJ, s.n);
+        if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "k_identity launch");
+    }
+    if (!st) {
+        hipLaunchKernelGGL(k_blocks, dim3(s.rows.G), dim3(64), 0, hs, s.rows, (const uint32_t*)s.cnt,
+                           (const uint32_t*)s.rows_of, r, JTJ, JTr, s.bad);
+        if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "k_blocks launch");
+    }
+    if (!st) st = read_bad(&s, hs);
+    gn_free(&s);
+    return st;
+}
+
+int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_stream stream) {
+    if (!out) return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_create: out is NULL");
+    *out = nullptr;
+    int st = check_problem(problem);
+    if (st) return st;
+    vf_gn_solver* s = new vf_gn_solver();
+    hipStream_t hs = (hipStream_t)stream;
+    (void)hipGetDevice(&s->device);
+    s->rows = make_rows(problem);
+    s->nrows = s->rows.C + 3u * s->rows.G * (uint32_t)s->rows.R;
+    s->n = 6ull * s->rows.G;
+    if (!st) st = gn_alloc(&s->cnt, s->rows.G + 1ull);
+    if (!st) st = gn_alloc(&s->fill, s->rows.G);
+    if (!st) st = gn_alloc(&s->bad, 1);
+    if (!st) st = gn_alloc(&s->JTJ, s->n * s->n);
+    if (!st) st = gn_alloc(&s->ipiv, s->n);
+    if (!st) st = gn_alloc(&s->info, 1);
+    if (!st) st = gn_alloc(&s->r, s->nrows);
+    if (!st) st = gn_alloc(&s->x, s->n);
+    if (!st) st = gn_alloc(&s->part, kEnergyBlocks);
+    if (!st) st = gn_alloc(&s->energy, 1);
+    if (!st && rocblas_create_handle(&s->blas) != rocblas_status_success)
+        st = fail(VF_ERR_HIP, "rocblas_create_handle");
+    if (!st && rocblas_set_stream(s->blas, hs) != rocblas_status_success) st = fail(VF_ERR_HIP, "rocblas_set_stream");
+    if (!st) st = build_rows(s, hs);
+    if (!st) {
+        hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, s->JTJ, s->n);
+        hipLaunchKernelGGL(k_blocks, dim3(s->rows.G), dim3(64), 0, hs, s->rows, (const uint32_t*)s->cnt,
+                           (const uint32_t*)s->rows_of, (const float*)nullptr, s->JTJ, (float*)nullptr, s->bad);
+        if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "normal-equation kernels launch");
+    }
+    if (!st) st = read_bad(s, hs);
+    // JTJ does not change within the ICP iteration: factorise it once
+    if (!st && rocsolver_sgetrf(s->blas, (int)s->n, (int)s->n, s->JTJ, (int)s->n, s->ipiv, s->info) !=
+                   rocblas_status_success)
+        st = fail(VF_ERR_HIP, "rocsolver_sgetrf");
+    if (!st) {
+        int hinfo = 0;
+        if (hipMemcpyAsync(&hinfo, s->info, sizeof(int), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            st = fail(VF_ERR_HIP, "gn: info read-back");
+        else if (hinfo != 0)
+            st = fail(VF_ERR_SINGULAR, "gn: the normal matrix has a zero pivot at column %d", hinfo);
+    }
+    if (st) {
+        gn_free(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return VF_OK;
+}
+
+int vf_gn_solver_destroy(vf_gn_solver* s) {
+    if (!s) return VF_OK;
+    gn_free(s);
+    delete s;
+    return VF_OK;
+}
+
+int vf_gn_solver_iterate(vf_gn_solver* s, vf_mat4* transforms, vf_mat4* pending, vf_float3* warped,
+                         const vf_float3* canon, size_t mesh_size, const vf_float3* depth_mesh, int max_iters,
+                         float min_step, vf_gn_report* report, vf_stream stream) {
+    if (!s || !transforms || !pending || !warped || !canon || !report)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_iterate: NULL argument");
+    if (s->rows.C && !depth_mesh) return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_iterate: NULL depth mesh");
+    if (max_iters < 0 || max_iters > 1000)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_iterate: max_iters %d", max_iters);
+    hipStream_t hs = (hipStream_t)stream;
+    if (rocblas_set_stream(s->blas, hs) != rocblas_status_success) return fail(VF_ERR_HIP, "rocblas_set_stream");
+    memset(report, 0, sizeof(*report));
+    const uint32_t G = s->rows.G;
+    // the loop of src/VolumeFusion.cu:541-691: pending starts as the transforms
+    VF_HIP(hipMemcpyAsync(pending, transforms, sizeof(vf_mat4) * G, hipMemcpyDeviceToDevice, hs));
+    double energy = 1.0e300;
+    for (int it = 0; it < max_iters; ++it) {
+        int st = launch_residuals(s, (const float*)warped, depth_mesh, pending, hs);
+        if (st) return st;
+        double e = 0.0;
+        VF_HIP(hipMemcpyAsync(&e, s->energy, sizeof(double), hipMemcpyDeviceToHost, hs));
+        VF_HIP(hipStreamSynchronize(hs));
+        if (it < (int)(sizeof(report->energy) / sizeof(report->energy[0]))) report->energy[it] = e;
+        report->iterations = it;
+        if (energy - e < (double)min_step) {
+            report->stopped = 1;
+            break;
+        }
+        energy = e;
+        report->final_energy = e;
+        if (it != 0) VF_HIP(hipMemcpyAsync(transforms, pending, sizeof(vf_mat4) * G, hipMemcpyDeviceToDevice, hs));
+        // JTr from the new residuals, then the back substitution
+        hipLaunchKernelGGL(k_blocks, dim3(G), dim3(64), 0, hs, s->rows, (const uint32_t*)s->cnt,
+                           (const uint32_t*)s->rows_of, (const float*)s->r, (float*)nullptr, s->x, s->bad);
+        VF_LAUNCH_CHECK("k_blocks");
+        if (rocsolver_sgetrs(s->blas, rocblas_operation_none, (int)s->n, 1, s->JTJ, (int)s->n, s->ipiv, s->x,
+                             (int)s->n) != rocblas_status_success)
+            return fail(VF_ERR_HIP, "rocsolver_sgetrs");
+        hipLaunchKernelGGL(k_update, dim3(blocks_for(G, 256)), dim3(256), 0, hs, (float*)pending, (const float*)s->x,
+                           G);
+        VF_LAUNCH_CHECK("k_update");
+        st = vf_warp_mesh(s->rows.K, warped, canon, pending, s->rows.knn, s->rows.kw, mesh_size, G, stream);
+        if (st) return st;
+        report->iterations = it + 1;
+    }
+    VF_HIP(hipStreamSynchronize(hs));
+    return VF_OK;
+}
+
+}  // extern "C"

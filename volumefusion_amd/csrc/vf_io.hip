@@ -8,11 +8,11 @@
 //       (src/TSDFVolume.cpp:339-358): int32 dims[3], a 24-byte size field,
 //       float delta, tsdf[N], weight[N], colour R/G/B (uint8 each) and colour
 //       weight (float) [N].  The reference writes its Vec3f size with
-//       sizeof(double) * 3 (:344, :324 on load), i.e. 12 bytes of size and 12
-//       bytes of whatever follows it in the object; here the field holds the
-//       float size and 12 zero bytes, and loading reads only its first 12
-//       bytes.  Colours are written as zeros (the device volumes carry none)
-//       and skipped on load.
+//       sizeof(double) * 3 (:344, :324 on load), i.e. 12 bytes of size and the
+//       12 bytes of the voxel size that follows it in the object; here the
+//       field holds the float size and size / dim, and loading reads only its
+//       first 12 bytes.  Colours are written as zeros (the device volumes
+//       carry none) and skipped on load.
 // Device volumes are streamed through a bounded host buffer.
 #include <cstdio>
 #include <cstring>
@@ -124,9 +124,16 @@ int save(const char* path, int format, const vf_tsdf_info* info, const float* ts
     } else if (format == VF_TSDF_FORMAT_LEGACY) {
         const int32_t dim[3] = {info->dim.x, info->dim.y, info->dim.z};
         const float size[3] = {info->size.x, info->size.y, info->size.z};
-        if (fwrite(dim, 4, 3, fh.f) != 3 || fwrite(size, 4, 3, fh.f) != 3) return io_fail("short write", path);
-        st = write_zeros(fh.f, 12);  // the rest of the 24-byte size field
-        if (!st && fwrite(&info->delta, 4, 1, fh.f) != 1) return io_fail("short write", path);
+        // The 24-byte size field is sizeof(double) * 3 bytes of the Vec3f
+        // m_size and of the Vec3f m_voxelSize after it (include/af/TSDFVolume.h:93-94):
+        // TSDFVolume::load reads both from it and does not recompute the
+        // voxel size, so the second half carries size / dim as the
+        // constructor and setSize compute it (src/TSDFVolume.cpp:11, 391).
+        const float vsz[3] = {size[0] / (float)dim[0], size[1] / (float)dim[1], size[2] / (float)dim[2]};
+        if (fwrite(dim, 4, 3, fh.f) != 3 || fwrite(size, 4, 3, fh.f) != 3 || fwrite(vsz, 4, 3, fh.f) != 3)
+            return io_fail("short write", path);
+        st = VF_OK;
+        if (fwrite(&info->delta, 4, 1, fh.f) != 1) return io_fail("short write", path);
         if (!st) st = write_floats(fh.f, tsdf, n, device, buf);
         if (!st) st = write_floats(fh.f, weights, n, device, buf);
         if (!st) st = write_zeros(fh.f, 3 * n + 4 * n);  // colour R, G, B and colour weights

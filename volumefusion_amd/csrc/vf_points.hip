@@ -598,6 +598,36 @@ using namespace vf;
         case 8: MACRO(8); break; \
     }
 
+namespace vf {
+namespace {
+
+int check_warp_points(int k, int blend, vf_float3* warped, const vf_float3* points, size_t n, const vf_float3* nodes,
+                      const float* radii, const vf_mat4* transforms, size_t graph_size) {
+    int st = check_k(k);
+    if (st) return st;
+    if (blend != VF_BLEND_LBS && blend != VF_BLEND_DQB)
+        return fail(VF_ERR_INVALID_ARGUMENT, "blend mode %d: supported VF_BLEND_LBS (0), VF_BLEND_DQB (1)", blend);
+    if (!n) return VF_OK;
+    if (!warped || !points || (graph_size && (!nodes || !radii || !transforms)))
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_warp_points: NULL argument");
+    if (graph_size > 0xffffffffull) return fail(VF_ERR_INVALID_ARGUMENT, "graph_size exceeds 2^32-1");
+    return VF_OK;
+}
+
+// Bytes of scratch the point warp uses for G nodes: the nodes' dual
+// quaternions, then the node grid.
+uint64_t warp_points_scratch_bytes(uint32_t G) {
+    const uint64_t g1 = std::max<uint32_t>(G, 1);
+    return align256(sizeof(float4) * g1 * 2) + grid_layout(nullptr, G).bytes;
+}
+
+int warp_points_on(void* mem, int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
+                   const vf_float3* nodes, const float* radii, const vf_mat4* transforms, uint32_t G,
+                   hipStream_t stream);
+
+}  // namespace
+}  // namespace vf
+
 extern "C" {
 
 int vf_compute_warp_weights(int k, float* knn_weights, const float* graph_radii, const uint32_t* knn_idxs,
@@ -645,11 +675,11 @@ int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, c
         return fail(VF_ERR_INVALID_ARGUMENT, "vf_mesh_knn: NULL argument");
     if (graph_size > 0xffffffffull) return fail(VF_ERR_INVALID_ARGUMENT, "graph_size exceeds 2^32-1");
     const uint32_t G = (uint32_t)graph_size;
-    void* scratch = nullptr;
+    void* mem = nullptr;
     const GridBufs probe = grid_layout(nullptr, G);
-    st = workspace_scratch(ws, probe.bytes, &scratch, (hipStream_t)stream);
+    st = workspace_scratch(ws, probe.bytes, &mem, (hipStream_t)stream);
     if (st) return st;
-    const GridBufs gb = grid_layout((char*)scratch, G);
+    const GridBufs gb = grid_layout((char*)mem, G);
     st = build_grid(gb, (const float*)nodes, G, (hipStream_t)stream);
     if (st) return st;
 #define VF_KNN(KK)                                                                                                   \
@@ -670,44 +700,49 @@ int vf_mesh_knn(vf_workspace* ws, int k, uint32_t* knn_idxs, float* knn_dists, c
 int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
                    const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
                    vf_stream stream) {
-    int st = check_k(k);
+    int st = check_warp_points(k, blend, warped, points, n, nodes, radii, transforms, graph_size);
+    if (st || !n) return st;
+    void* mem = nullptr;
+    st = workspace_scratch(ws, warp_points_scratch_bytes((uint32_t)graph_size), &mem, (hipStream_t)stream);
     if (st) return st;
-    if (blend != VF_BLEND_LBS && blend != VF_BLEND_DQB)
-        return fail(VF_ERR_INVALID_ARGUMENT, "blend mode %d: supported VF_BLEND_LBS (0), VF_BLEND_DQB (1)", blend);
-    if (!n) return VF_OK;
-    if (!warped || !points || (graph_size && (!nodes || !radii || !transforms)))
-        return fail(VF_ERR_INVALID_ARGUMENT, "vf_warp_points: NULL argument");
-    if (graph_size > 0xffffffffull) return fail(VF_ERR_INVALID_ARGUMENT, "graph_size exceeds 2^32-1");
-    const uint32_t G = (uint32_t)graph_size;
-    void* scratch = nullptr;
+    return warp_points_on(mem, k, blend, warped, points, n, nodes, radii, transforms, (uint32_t)graph_size,
+                          (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace vf {
+namespace {
+
+int warp_points_on(void* mem, int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
+                   const vf_float3* nodes, const float* radii, const vf_mat4* transforms, uint32_t G,
+                   hipStream_t stream) {
     const uint64_t g1 = std::max<uint32_t>(G, 1);
     const uint64_t dq_bytes = align256(sizeof(float4) * g1 * 2);
-    const GridBufs probe = grid_layout(nullptr, G);
-    st = workspace_scratch(ws, dq_bytes + probe.bytes, &scratch, (hipStream_t)stream);
-    if (st) return st;
-    float4* qr = (float4*)scratch;
+    float4* qr = (float4*)mem;
     float4* qd = qr + g1;
-    const GridBufs gb = grid_layout((char*)scratch + dq_bytes, G);
-    st = build_grid(gb, (const float*)nodes, G, (hipStream_t)stream);
+    const GridBufs gb = grid_layout((char*)mem + dq_bytes, G);
+    int st = build_grid(gb, (const float*)nodes, G, stream);
     if (st) return st;
     if (G && blend == VF_BLEND_DQB) {
-        hipLaunchKernelGGL(k_node_dq, dim3(grid_for(G, 256)), dim3(256), 0, (hipStream_t)stream,
-                           (const float*)transforms, G, qr, qd);
+        hipLaunchKernelGGL(k_node_dq, dim3(grid_for(G, 256)), dim3(256), 0, stream, (const float*)transforms, G, qr,
+                           qd);
         VF_LAUNCH_CHECK("k_node_dq");
     }
 #define VF_WP(KK)                                                                                                 \
     if (blend == VF_BLEND_DQB)                                                                                    \
         hipLaunchKernelGGL((k_warp_points<KK, VF_BLEND_DQB>), dim3(grid_for(n, kPtThreads)), dim3(kPtThreads), 0, \
-                           (hipStream_t)stream, (float*)warped, (const float*)points, (uint64_t)n, gb.hdr,         \
-                           gb.start, gb.sorted, radii, (const float*)transforms, qr, qd, G);                      \
+                           stream, (float*)warped, (const float*)points, (uint64_t)n, gb.hdr, gb.start, gb.sorted, \
+                           radii, (const float*)transforms, qr, qd, G);                                           \
     else                                                                                                          \
         hipLaunchKernelGGL((k_warp_points<KK, VF_BLEND_LBS>), dim3(grid_for(n, kPtThreads)), dim3(kPtThreads), 0, \
-                           (hipStream_t)stream, (float*)warped, (const float*)points, (uint64_t)n, gb.hdr,         \
-                           gb.start, gb.sorted, radii, (const float*)transforms, qr, qd, G)
+                           stream, (float*)warped, (const float*)points, (uint64_t)n, gb.hdr, gb.start, gb.sorted, \
+                           radii, (const float*)transforms, qr, qd, G)
     VF_PT_K_SWITCH(k, VF_WP)
 #undef VF_WP
     VF_LAUNCH_CHECK("k_warp_points");
     return VF_OK;
 }
 
-}  // extern "C"
+}  // namespace
+}  // namespace vf

@@ -69,6 +69,7 @@ const char* vf_status_string(int status) {
         case VF_ERR_OUT_OF_MEMORY: return "out of device memory";
         case VF_ERR_WORKSPACE: return "workspace too small";
         case VF_ERR_CAPACITY: return "capacity exceeded";
+        case VF_ERR_SINGULAR: return "singular linear system";
         default: return "unknown status";
     }
 }

@@ -660,6 +660,7 @@ int vf_server_get_stats(vf_server* sv, vf_server_stats* out) {
         out->totals.updated += t.updated;
         out->totals.active_bricks += t.active_bricks;
         out->totals.overflow_bricks += t.overflow_bricks;
+        out->totals.screened_bricks += t.screened_bricks;
     }
     return VF_OK;
 }

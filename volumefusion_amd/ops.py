@@ -377,10 +377,12 @@ class Workspace:
         L.call("vf_workspace_kernel_ms", self.handle, ms, C.byref(n))
         return (float(ms[0]), float(ms[1]), float(ms[2])), int(n.value)
 
-    def set_tuning(self, brick_shape: int = 0, wg_per_cu: int = 0) -> None:
-        """Brick shape (index into _lib.BRICK_SHAPES) and pass-A workgroups
-        per CU (0 = occupancy limit) for later calls; results do not change."""
-        t = L.Tuning(int(brick_shape), int(wg_per_cu), 0.0)
+    def set_tuning(self, brick_shape: int = 0, wg_per_cu: int = 0, screen: bool = True) -> None:
+        """Brick shape (index into _lib.BRICK_SHAPES), pass-A workgroups per
+        CU (0 = occupancy limit) and the brick screen for later calls; tsdf,
+        weights and node positions do not change (with the screen, the
+        supported and in-bounds counts cover the swept bricks only)."""
+        t = L.Tuning(int(brick_shape), int(wg_per_cu), 0.0, 0 if screen else L.TUNE_NO_SCREEN)
         L.call("vf_workspace_set_tuning", self.handle, C.byref(t))
 
     def close(self) -> None:
@@ -475,9 +477,15 @@ def warped_integrate_batch(ws: Workspace, tsdfs, graphs, depths, intrinsics_list
         _done(stream)
 
 
+def _tuning_dict(t) -> dict:
+    return {"brick_shape": int(t.brick_shape), "brick": L.BRICK_SHAPES[t.brick_shape],
+            "wg_per_cu": int(t.wg_per_cu), "ms_per_frame": float(t.ms_per_frame),
+            "screen": not (int(t.flags) & L.TUNE_NO_SCREEN)}
+
+
 class Plan:
     """A fixed batch of volumes replayed every frame (vf_plan): descriptors are
-    uploaded once and, with graph=True, the four kernels run as one recorded
+    uploaded once and, with graph=True, the five kernels run as one recorded
     HIP graph.  The device buffers of every argument must stay alive and in
     place; their contents may change between launches."""
 
@@ -502,7 +510,8 @@ class Plan:
         self.handle = C.c_void_p(None)
         t = None
         if tuning is not None:
-            t = L.Tuning(int(tuning[0]), int(tuning[1]), 0.0)
+            screen = tuning[2] if len(tuning) > 2 else True
+            t = L.Tuning(int(tuning[0]), int(tuning[1]), 0.0, 0 if screen else L.TUNE_NO_SCREEN)
         L.call("vf_plan_create", C.byref(self.handle), n, self.vols, self.graphs, self.frames, self.params,
                C.byref(t) if t is not None else None, L.PLAN_GRAPH if graph else 0, _s(stream))
 
@@ -514,14 +523,12 @@ class Plan:
     def autotune(self, frames: int = 3, stream: Stream | None = None) -> dict:
         t = L.Tuning()
         L.call("vf_plan_autotune", self.handle, int(frames), C.byref(t), _s(stream))
-        return {"brick_shape": int(t.brick_shape), "brick": L.BRICK_SHAPES[t.brick_shape],
-                "wg_per_cu": int(t.wg_per_cu), "ms_per_frame": float(t.ms_per_frame)}
+        return _tuning_dict(t)
 
     def tuning(self) -> dict:
         t = L.Tuning()
         L.call("vf_plan_get_tuning", self.handle, C.byref(t))
-        return {"brick_shape": int(t.brick_shape), "brick": L.BRICK_SHAPES[t.brick_shape],
-                "wg_per_cu": int(t.wg_per_cu), "ms_per_frame": float(t.ms_per_frame)}
+        return _tuning_dict(t)
 
     def stats(self, reset: bool = False, stream: Stream | None = None) -> tuple:
         last, tot = L.FrameStats(), L.FrameStats()

@@ -426,3 +426,76 @@ def render_frames(jobs: list, workers: int | None = None) -> list:
     ctx = multiprocessing.get_context("fork")
     with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
         return list(ex.map(_render_job, jobs, chunksize=1))
+
+
+@dataclass
+class GNProblem:
+    """One linearisation of the reference's motion optimisation
+    (src/VolumeFusion.cu:434-692): a motion graph with its regularisation
+    neighbours, a canonical mesh bound to the graph by k nearest nodes and
+    normalised Gaussian weights, and correspondences of mesh points to target
+    points with unit normals."""
+    graph: np.ndarray    # [G, 3] float32 node positions
+    T: np.ndarray        # [G, 16] float32 column-major transforms
+    gknn: np.ndarray     # [G, R] uint32 regularisation neighbours (>= G: none)
+    canon: np.ndarray    # [M, 3] float32 canonical mesh
+    knn: np.ndarray      # [M, K] uint32 nearest nodes of each mesh point
+    kw: np.ndarray       # [M, K] float32 normalised weights (0 beyond the radius)
+    corr: np.ndarray     # [C, 2] int32 (mesh index, target index)
+    dmesh: np.ndarray    # [D, 3] float32 target points
+    dnormals: np.ndarray # [D, 3] float32 unit normals
+    wD: float = 1.0
+    wR: float = 1.0
+
+
+def gn_problem(seed: int, G: int = 24, M: int = 600, K: int = 6, R: int = 4, radius: float = 0.05,
+               motion=None, warp_angle_deg: float = 2.0, warp_translation: float = 0.004,
+               wD: float = 1.0, wR: float = 0.5, drop_gknn: float = 0.1) -> GNProblem:
+    """Nodes on a sphere patch, mesh points near the surface, targets on the
+    mesh points moved by `motion` (a twist u [G, 6] applied as T - [u], or a
+    random rigid motion of each node) with noise-free normals."""
+    rng = np.random.default_rng([seed, 60])
+    K = min(K, G)
+    dirs = rng.normal(size=(G, 3))
+    dirs[:, 2] = np.abs(dirs[:, 2]) + 1.0
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    graph = (0.3 * dirs + np.array([0.0, 0.0, 0.8])).astype(F32)
+    # regularisation neighbours: the R nearest other nodes, some dropped
+    d2 = ((graph[:, None, :].astype(np.float64) - graph[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    gknn = np.full((G, R), 0xFFFFFFFF, np.uint32)  # fewer than R other nodes: the rest absent
+    gknn[:, :min(R, G - 1)] = np.argsort(d2, axis=1)[:, :min(R, G - 1)]
+    gknn[rng.random(gknn.shape) < drop_gknn] = np.uint32(0xFFFFFFFF)
+    # mesh points around the nodes
+    src = rng.integers(0, G, M)
+    canon = (graph[src] + rng.normal(scale=0.4 * radius, size=(M, 3))).astype(F32)
+    dm = ((canon[:, None, :].astype(np.float64) - graph[None, :, :]) ** 2).sum(-1)
+    knn = np.argsort(dm, axis=1)[:, :K].astype(np.uint32)
+    dk = np.sqrt(np.take_along_axis(dm, knn.astype(np.int64), 1))
+    w = np.where(dk < radius, np.exp(-(dk ** 2) / (2 * radius ** 2)), 0.0)
+    s = w.sum(1, keepdims=True)
+    w = np.where(s > 0, w / np.where(s > 0, s, 1.0), 0.0)
+    kw = w.astype(F32)
+    T = np.tile(np.eye(4, dtype=F32).reshape(16), (G, 1))
+    if motion is None:
+        motion = smooth_random_warps(graph, rng, warp_angle_deg, warp_translation)
+        T_true = motion
+    else:
+        T_true = T.copy()
+        for g in range(G):
+            x = motion[g]
+            U = np.array([[0, -x[2], x[1], x[3]], [x[2], 0, -x[0], x[4]], [-x[1], x[0], 0, x[5]], [0, 0, 0, 0]])
+            T_true[g] = (T[g].reshape(4, 4).T - U).T.reshape(16)
+    # targets: every mesh point warped by the true transforms
+    tgt = np.zeros((M, 3))
+    for i in range(M):
+        p = np.append(canon[i].astype(np.float64), 1.0)
+        for s_ in range(K):
+            if kw[i, s_] > 0:
+                tgt[i] += kw[i, s_] * (T_true[knn[i, s_]].reshape(4, 4).T.astype(np.float64) @ p)[:3]
+    normals = rng.normal(size=(M, 3))
+    normals /= np.linalg.norm(normals, axis=1, keepdims=True)
+    has_w = kw.sum(1) > 0
+    idx = np.nonzero(has_w)[0].astype(np.int32)
+    corr = np.stack([idx, idx], 1).astype(np.int32)
+    return GNProblem(graph, T, gknn, canon, knn, kw, corr, tgt.astype(F32), normals.astype(F32), wD, wR)
