@@ -158,6 +158,19 @@ int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const 
                    const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
                    vf_stream stream);
 
+/* vf_warp_points without a workspace, allocating nothing: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch (the node grid and dual quaternions) is a static 4 MiB device
+ * array of the library, one per GPU, held until the process ends; graphs of
+ * at most 65536 nodes (VF_ERR_CAPACITY beyond).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (on different
+ * streams or host threads) race and must be serialised by the caller.
+ * Same results as vf_warp_points. */
+int vf_warp_points_static(int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
+                          const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
+                          vf_stream stream);
+
 /* ---- depth-frame ingest ---------------------------------------------------- */
 /* af::runFilterDepthKernel, include/af/VertexManipulation.cuh:11
  * (src/VertexManipulation.cu:6-23): values above threshold, below 1e-4 or NaN

@@ -211,6 +211,7 @@ _SIGS = {
     "vf_warp_mesh": [C.c_int, vp, vp, vp, vp, vp, sz, sz, vp],
     "vf_mesh_knn": [vp, C.c_int, vp, vp, vp, sz, vp, sz, C.c_int, vp],
     "vf_warp_points": [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
+    "vf_warp_points_static": [C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, sz, vp],
     "vf_correspondence_pairs": [vp, vp, vp, vp, sz, Mat3, vp, Int2, vp, f32, vp],
     "vf_fill_r_depth": [vp, f32, vp, u32, vp, vp, vp, vp],
     "vf_fill_r_mreg": [vp, f32, vp, vp, u32, vp, C.c_int, vp],

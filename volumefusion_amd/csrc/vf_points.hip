@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 
 #include "vf_host.hpp"
 
@@ -524,6 +525,12 @@ __global__ __launch_bounds__(kPtThreads) void k_warp_points(float* out, const fl
 
 inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// The static scratch of vf_warp_points_static: 4 MiB per device holds the
+// dual quaternions and node grid of graphs up to 65536 nodes.
+constexpr uint64_t kStaticScratchBytes = 4ull << 20;
+constexpr uint32_t kStaticMaxNodes = 65536;
+constexpr int kStaticMaxDevices = 64;
+
 // Cells per axis for G nodes: about two nodes per cell of a full grid
 // (nodes on a surface fill fewer cells, more densely).
 inline int32_t grid_dim(uint32_t G) {
@@ -598,8 +605,39 @@ using namespace vf;
         case 8: MACRO(8); break; \
     }
 
+// Scratch of vf_warp_points_static: a file-scope device array, one copy per
+// GPU, so that the call allocates nothing.  Its address on each device is
+// taken once (hipGetSymbolAddress) and kept; every call on a device shares
+// that device's copy, so calls that overlap on one device race.  The kernels
+// re-initialise every byte they read (grid counters, header, node lists, dual
+// quaternions) on each call.
+static __device__ __align__(256) unsigned char scratch[vf::kStaticScratchBytes];
+
 namespace vf {
 namespace {
+
+std::mutex g_static_mutex;
+void* g_static_ptr[kStaticMaxDevices] = {};
+
+// The current device's address of `scratch` (taken on the first call there).
+int static_scratch(void** out) {
+    int dev = 0;
+    VF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStaticMaxDevices)
+        return fail(VF_ERR_INVALID_ARGUMENT, "device %d: the static scratch serves devices 0..%d", dev,
+                    kStaticMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (!g_static_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VF_ERR_HIP, "hipGetSymbolAddress of the static scratch on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_static_ptr[dev] = p;
+    }
+    *out = g_static_ptr[dev];
+    return VF_OK;
+}
 
 int check_warp_points(int k, int blend, vf_float3* warped, const vf_float3* points, size_t n, const vf_float3* nodes,
                       const float* radii, const vf_mat4* transforms, size_t graph_size) {
@@ -704,6 +742,25 @@ int vf_warp_points(vf_workspace* ws, int k, int blend, vf_float3* warped, const 
     if (st || !n) return st;
     void* mem = nullptr;
     st = workspace_scratch(ws, warp_points_scratch_bytes((uint32_t)graph_size), &mem, (hipStream_t)stream);
+    if (st) return st;
+    return warp_points_on(mem, k, blend, warped, points, n, nodes, radii, transforms, (uint32_t)graph_size,
+                          (hipStream_t)stream);
+}
+
+int vf_warp_points_static(int k, int blend, vf_float3* warped, const vf_float3* points, size_t n,
+                          const vf_float3* nodes, const float* radii, const vf_mat4* transforms, size_t graph_size,
+                          vf_stream stream) {
+    int st = check_warp_points(k, blend, warped, points, n, nodes, radii, transforms, graph_size);
+    if (st || !n) return st;
+    const uint64_t need = warp_points_scratch_bytes((uint32_t)graph_size);
+    if (graph_size > kStaticMaxNodes || need > kStaticScratchBytes)
+        return fail(VF_ERR_CAPACITY,
+                    "vf_warp_points_static: a graph of %llu nodes needs %llu B of scratch; the library's static "
+                    "scratch holds %llu B (graphs up to %u nodes)",
+                    (unsigned long long)graph_size, (unsigned long long)need, (unsigned long long)kStaticScratchBytes,
+                    kStaticMaxNodes);
+    void* mem = nullptr;
+    st = static_scratch(&mem);
     if (st) return st;
     return warp_points_on(mem, k, blend, warped, points, n, nodes, radii, transforms, (uint32_t)graph_size,
                           (hipStream_t)stream);
