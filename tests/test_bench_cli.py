@@ -36,3 +36,5 @@ def test_traffic_file_of_the_default_workload_is_committed():
     assert abs(ka["hbm_bytes_per_launch"] - ka["hbm_read_bytes_per_launch"] - ka["hbm_write_bytes_per_launch"]) <= 1
     assert ka["hbm_bytes_per_launch"] > ka["algorithmic_bytes_per_launch"] > 0
     assert ka["valu_insts_per_launch"] > 0
+    # keyed to the kernel sources it was measured on (bench.py compares the key)
+    assert len(t["kernel_sources_sha256"]) == 64 and t["kernel_sources"]
