@@ -47,6 +47,7 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "warped voxel updates/s + % HBM roofline; p50/p99 frame latency @32 streams"
 HBM_PEAK = 8.0e12
+FP32_PEAK = 157.3e12  # dense fp32 matrix peak of MI355X
 SIMDS = 1024          # MI355X: 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 VALU_CYCLES = 4       # a wave64 VALU instruction on a 16-lane SIMD
@@ -69,6 +70,7 @@ def parse_args():
     ap.add_argument("--tuning", default=None, metavar="SHAPE,WG",
                     help="fixed brick shape index and workgroups per CU instead of autotuning (e.g. 2,0)")
     ap.add_argument("--no-extras", action="store_true", help="skip marching cubes, ingest and point warp")
+    ap.add_argument("--no-gn", action="store_true", help="skip the Gauss-Newton (gn4096) measurement")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the binary64 error report of sphere128, sphere256_warped and ellipsoids512")
     return ap.parse_args()
@@ -349,6 +351,77 @@ def bench_plain_sphere128(stream) -> dict:
     return {"workload": "sphere128: 128^3 at 4 mm, one 640x576 frame, plain method", "ms_per_frame": ms,
             "voxels_per_s": n / (ms * 1e-3), "hbm_bytes_per_frame": 16.0 * n,
             "achieved_GBps": 16.0 * n / (ms * 1e-3) / 1e9}
+
+
+def bench_gauss_newton(stream, G: int = 4096, M: int = 300_000, iters: int = 5) -> dict:
+    """One ICP iteration's Gauss-Newton solve (src/VolumeFusion.cu:434-692) on
+    the gn4096 workload: 4096 nodes, 300k mesh points with 6 knn slots, 4
+    regularisation neighbours, targets made by a smooth rigid motion of every
+    node.  vf_gn_solver_create builds the normal equations from the sparse
+    rows and factorises them (rocSOLVER getrf); vf_gn_solver_iterate then
+    runs `iters` Gauss-Newton iterations (residuals, energy, JTr, getrs,
+    update, mesh warp).  The reference rebuilds and refactorises JTJ on every
+    iteration, i.e. one create per iteration."""
+    from volumefusion_amd import _lib as L
+    from volumefusion_amd import scenes
+    from volumefusion_amd.device import DeviceArray
+
+    p = scenes.gn_problem(7, G=G, M=M)
+    K, R = p.knn.shape[1], p.gknn.shape[1]
+    dev = lambda a: DeviceArray.from_numpy(np.ascontiguousarray(a), stream)
+    corr, canon, knn, kw = dev(p.corr), dev(p.canon), dev(p.knn), dev(p.kw)
+    dmesh, normals, graph, gknn = dev(p.dmesh), dev(p.dnormals), dev(p.graph), dev(p.gknn)
+    desc = L.GNProblem(corr.ptr, p.corr.shape[0], canon.ptr, K, knn.ptr, kw.ptr, normals.ptr, graph.ptr, G,
+                       gknn.ptr, R, p.wD, p.wR)
+    n = 6 * G
+    JTJ = DeviceArray((n, n), np.float32, stream)
+    T0 = np.tile(np.eye(4, dtype=np.float32).reshape(16), (G, 1))
+    dT, dP, dw = dev(T0), DeviceArray((G, 16), np.float32, stream), DeviceArray((M, 3), np.float32, stream)
+
+    def reset():
+        dT.upload(T0)
+        L.call("vf_warp_mesh", K, dw.ptr, canon.ptr, dT.ptr, knn.ptr, kw.ptr, M, G, stream.ptr)
+        stream.synchronize()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        stream.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    ne = [timed(lambda: L.call("vf_gn_normal_equations", None, JTJ.ptr, None, None, C.byref(desc), stream.ptr))
+          for _ in range(3)]
+    del JTJ
+    solver = C.c_void_p()
+    create, per_iter, rep = [], [], None
+    for r in range(3):
+        reset()
+        create.append(timed(lambda: L.call("vf_gn_solver_create", C.byref(solver), C.byref(desc), stream.ptr)))
+        rep = L.GNReport()
+        ms = timed(lambda: L.call("vf_gn_solver_iterate", solver, dT.ptr, dP.ptr, dw.ptr, canon.ptr, M, dmesh.ptr,
+                                  iters, -1e30, C.byref(rep), stream.ptr))
+        per_iter.append(ms / iters)
+        L.call("vf_gn_solver_destroy", solver)
+    c_ms, ne_ms, it_ms = min(create), min(ne), min(per_iter)
+    lu_ms = max(c_ms - ne_ms, 1e-6)
+    flops = 2.0 / 3.0 * float(n) ** 3
+    out = {
+        "workload": f"gn4096: {G} nodes, {M} mesh points ({p.corr.shape[0]} correspondences), k={K}, "
+                    f"{R} regularisation neighbours, {n}x{n} normal matrix",
+        "rows": int(p.corr.shape[0] + 3 * G * R),
+        "normal_equations_ms": ne_ms,
+        "solver_create_ms": c_ms,
+        "lu_factor_ms_est": lu_ms,
+        "lu_factor_tflops_est": flops / (lu_ms * 1e-3) / 1e12,
+        "lu_factor_frac_fp32_peak": flops / (lu_ms * 1e-3) / FP32_PEAK,
+        "gn_iteration_ms": it_ms,
+        "iterations": iters,
+        "icp_iteration_ms": c_ms + iters * it_ms,
+        "refactorised_every_iteration_ms": iters * (c_ms + it_ms),
+        "energy_first_last": [rep.energy[0], rep.energy[iters - 1]],
+    }
+    out["speedup_vs_refactorising"] = out["refactorised_every_iteration_ms"] / out["icp_iteration_ms"]
+    return out
 
 
 def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
@@ -791,6 +864,12 @@ def main() -> int:
         print(f"serving: p50 {sv['poisson']['p50_ms']:.2f} ms, p99 {sv['poisson']['p99_ms']:.2f} ms, "
               f"saturated {sv['saturated']['fps']:.0f} fps vs plain {sv['plain']['with_upload']['fps']:.0f} fps",
               file=sys.stderr, flush=True)
+    if not args.no_gn:
+        result["gauss_newton"] = bench_gauss_newton(stream)
+        g = result["gauss_newton"]
+        print(f"gauss-newton: normal equations {g['normal_equations_ms']:.2f} ms, solver create "
+              f"{g['solver_create_ms']:.2f} ms (LU ~{g['lu_factor_tflops_est']:.1f} TFLOP/s), "
+              f"{g['gn_iteration_ms']:.3f} ms per iteration", file=sys.stderr, flush=True)
     if not args.no_fp64:
         result["fp64_error"] = fp64_error((L.BRICK_SHAPES.index(tuple(tuning["brick"])), tuning["wg_per_cu"]),
                                           min(16, os.cpu_count() or 1))

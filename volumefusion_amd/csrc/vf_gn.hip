@@ -446,31 +446,84 @@ __global__ void k_identity(float* __restrict__ A, uint64_t n) {
         A[i] = (i / n == i % n) ? 1.0f : 0.0f;
 }
 
+// The members of every row, computed once per ICP iteration (one thread per
+// row, row_members): slot q < 8 of row r holds node | diag << 24 | jtr << 28
+// in node8[8 r + q] (kGnNone past the row's members) and its Jacobian
+// entries in j48[48 r + 6 q + k].  A member outside the graph (a knn index
+// >= G with a non-zero weight) is dropped and flagged.
+__global__ void k_row_data(GnRows g, uint32_t rows, uint32_t* __restrict__ node8, float* __restrict__ j48,
+                           uint32_t* __restrict__ bad) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    Member m[kGnMaxK];
+    const int n0 = row_members(g, row, m);
+    int n = 0;
+    for (int q = 0; q < n0; ++q) {
+        if (m[q].node >= g.G) {
+            *bad = 1u;
+            continue;
+        }
+        m[n++] = m[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kGnMaxK; ++q) {
+        uint32_t v = kGnNone;
+        float j[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (q < n) {
+            v = m[q].node | ((uint32_t)m[q].diag << 24) | ((uint32_t)m[q].jtr << 28);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) j[k] = m[q].j[k];
+        }
+        node8[8ull * row + q] = v;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) j48[48ull * row + 6 * q + k] = j[k];
+    }
+}
+
 // One wave per node i: its 6x6 blocks JTJ(i, j) for every node j sharing a
 // row with it, summed over i's rows in ascending order (each accumulator
 // starts at -0, the additive identity, so the first term is taken as is),
 // and, when r is given, its JTr entry.  Lanes 0..35 own the 36 elements of a
-// block; lanes 0..5 the JTr entry.
-__global__ __launch_bounds__(64) void k_blocks(GnRows g, const uint32_t* __restrict__ off,
-                                               const uint32_t* __restrict__ rows_of, const float* __restrict__ r,
-                                               float* __restrict__ JTJ, float* __restrict__ JTr,
-                                               uint32_t* __restrict__ bad) {
+// block; lanes 0..5 the JTr entry.  Each row's members come from the row
+// data (node8, j48), the next row's loaded while the current one is summed;
+// the row indices come 64 at a time, one per lane.
+__global__ __launch_bounds__(64) void k_blocks(uint32_t G, const uint32_t* __restrict__ off,
+                                               const uint32_t* __restrict__ rows_of,
+                                               const uint32_t* __restrict__ node8, const float* __restrict__ j48,
+                                               const float* __restrict__ r, float* __restrict__ JTJ,
+                                               float* __restrict__ JTr, uint32_t* __restrict__ bad) {
     __shared__ uint32_t s_ids[kGnMaxNbr];
     __shared__ float s_acc[kGnMaxNbr][36];
-    __shared__ Member s_m[kGnMaxK];
-    __shared__ int s_n, s_me;
+    __shared__ float s_j[2][48];
     const uint32_t i = blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t b = off[i], n = off[i + 1] - b;
+    // row e of i's list: rows_of[b + e], held by lane e % 64 of `rowreg`
+    // while chunk e / 64 is loaded
+    uint32_t rowreg = 0u, chunk = 0xffffffffu;
+    const auto row_at = [&](uint32_t e) -> uint32_t {
+        if ((e >> 6) != chunk) {
+            chunk = e >> 6;
+            const uint32_t k = (chunk << 6) + (uint32_t)lane;
+            rowreg = k < n ? rows_of[b + k] : 0u;
+        }
+        return (uint32_t)__shfl((int)rowreg, (int)(e & 63u));
+    };
     int nids = 0;
-    float jtr = -0.0f;
     if (JTJ) {
         // distinct nodes of i's rows, in first-seen order
+        uint32_t pn = kGnNone;
+        if (n > 0 && lane < 8) pn = node8[8ull * row_at(0) + lane];
         for (uint32_t e = 0; e < n; ++e) {
-            if (lane == 0) s_n = row_members(g, rows_of[b + e], s_m);
-            __syncthreads();
-            for (int q = 0; q < s_n; ++q) {
-                const uint32_t j = s_m[q].node;
+            const uint32_t cn = pn;
+            if (e + 1 < n) {
+                const uint32_t r1 = row_at(e + 1);
+                pn = lane < 8 ? node8[8ull * r1 + lane] : kGnNone;
+            }
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t mq = (uint32_t)__shfl((int)cn, q);
+                if (mq == kGnNone) break;
+                const uint32_t j = mq & 0xffffffu;
                 const bool seen = __ballot((lane < nids && s_ids[lane] == j) ||
                                            (lane + 64 < nids && s_ids[lane + 64] == j)) != 0;
                 if (!seen) {
@@ -481,50 +534,64 @@ __global__ __launch_bounds__(64) void k_blocks(GnRows g, const uint32_t* __restr
                     if (lane == 0) s_ids[nids] = j;
                     ++nids;
                 }
-                __syncthreads();
             }
         }
         for (int t = 0; t < nids; ++t)
             if (lane < 36) s_acc[t][lane] = -0.0f;
-        __syncthreads();
+    }
+    float jtr = -0.0f;
+    uint32_t pn = kGnNone;
+    float pj = 0.0f, pr = 0.0f;
+    if (n > 0) {
+        const uint32_t r0 = row_at(0);
+        if (lane < 8) pn = node8[8ull * r0 + lane];
+        if (lane < 48) pj = j48[48ull * r0 + lane];
+        if (r) pr = r[r0];
     }
     for (uint32_t e = 0; e < n; ++e) {
-        const uint32_t row = rows_of[b + e];
-        if (lane == 0) {
-            s_n = row_members(g, row, s_m);
-            s_me = -1;
-            for (int q = 0; q < s_n; ++q)
-                if (s_m[q].node == i) s_me = q;
+        const uint32_t cn = pn;
+        const float cr = pr;
+        float* sj = s_j[e & 1u];
+        if (lane < 48) sj[lane] = pj;
+        if (e + 1 < n) {
+            const uint32_t r1 = row_at(e + 1);
+            if (lane < 8) pn = node8[8ull * r1 + lane];
+            if (lane < 48) pj = j48[48ull * r1 + lane];
+            if (r) pr = r[r1];
         }
-        __syncthreads();
-        if (s_me < 0) {  // cannot happen: the row is in i's list because i is a member
+        // this node's member: the last slot naming i
+        const unsigned long long mine = __ballot(lane < 8 && cn != kGnNone && (cn & 0xffffffu) == i);
+        if (!mine) {  // cannot happen: the row is in i's list because i is a member
             if (lane == 0) *bad = 4u;
             return;
         }
-        const Member& mi = s_m[s_me];
+        const int qi = 63 - __clzll((long long)mine);
+        const uint32_t mi = (uint32_t)__shfl((int)cn, qi);
         if (JTr && lane < 6) {
-            const float rv = r[row];
-            for (int t = 0; t < mi.jtr; ++t) jtr = jtr + mi.j[lane] * rv;
+            const int mult = (int)(mi >> 28);
+            const float v = sj[6 * qi + lane] * cr;
+            for (int t = 0; t < mult; ++t) jtr = jtr + v;
         }
         if (JTJ) {
-            for (int q = 0; q < s_n; ++q) {
-                const Member& mj = s_m[q];
-                // the slot of mj.node: lanes test two list entries each
-                const unsigned long long h0 = __ballot(lane < nids && s_ids[lane] == mj.node);
-                const unsigned long long h1 = __ballot(lane + 64 < nids && s_ids[lane + 64] == mj.node);
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t mq = (uint32_t)__shfl((int)cn, q);
+                if (mq == kGnNone) break;
+                const uint32_t node = mq & 0xffffffu;
+                // the slot of node: lanes test two list entries each
+                const unsigned long long h0 = __ballot(lane < nids && s_ids[lane] == node);
+                const unsigned long long h1 = __ballot(lane + 64 < nids && s_ids[lane + 64] == node);
                 const int slot = h0 ? __ffsll((long long)h0) - 1 : 64 + __ffsll((long long)h1) - 1;
-                const int mult = mj.node == i ? mi.diag : 1;
+                const int mult = node == i ? (int)((mi >> 24) & 15u) : 1;
                 if (lane < 36) {
-                    const float v = mi.j[lane / 6] * mj.j[lane % 6];
+                    const float v = sj[6 * qi + lane / 6] * sj[6 * q + lane % 6];
                     for (int t = 0; t < mult; ++t) s_acc[slot][lane] = s_acc[slot][lane] + v;
                 }
             }
         }
-        __syncthreads();
     }
     if (JTr && lane < 6) JTr[6ull * i + lane] = n ? jtr : 0.0f;
     if (JTJ && lane < 36) {
-        const uint64_t W = 6ull * g.G;
+        const uint64_t W = 6ull * G;
         const int a = lane / 6, c = lane % 6;
         for (int t = 0; t < nids; ++t)
             JTJ[(6ull * i + a) * W + 6ull * s_ids[t] + c] = s_acc[t][lane];
@@ -558,6 +625,110 @@ __global__ void k_energy_sum(const double* __restrict__ part, double* __restrict
     }
 }
 
+// ---- the solve with the factorised JTJ ---------------------------------------
+//
+// getrs (src/LinearSolver.cu:179-225 calls it through cuSOLVER) as blocked
+// substitutions for the solver loop, which solves with the same factors on
+// every Gauss-Newton iteration: the row interchanges as one gather (the
+// permutation composed once from ipiv), then L y = P b and U x = y in blocks
+// of 64 rows, one launch per block step.  Step k subtracts the block column
+// k of the factor times the solved block k from every block still to solve
+// (a GEMV over a column panel, coalesced along the rows; one wave per row
+// block), and the wave of the next block then solves its 64 x 64 diagonal
+// block by substitution in registers.  Rounding differs from rocSOLVER's
+// getrs (another summation order), not the method.
+constexpr int kTB = 64;
+
+// Lane j's value of x, broadcast (j a constant of the unrolled loops).
+__device__ __forceinline__ float lane_value(float x, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+}
+
+__global__ __launch_bounds__(256) void k_gather_perm(const float* __restrict__ b, const int* __restrict__ perm,
+                                                     float* __restrict__ y, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = b[perm[i]];
+}
+
+// Forward step k (k = -1: the first diagonal block alone).  Wave w takes row
+// block k + 1 + w; the wave of block k + 1 also solves its diagonal block.
+// LU is column-major with leading dimension n; L has a unit diagonal.
+__global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict__ LU, int n, int k,
+                                                        float* __restrict__ y) {
+    const int lane = threadIdx.x;
+    const int b = k + 1 + (int)blockIdx.x;
+    const int i = b * kTB + lane;
+    const bool row = i < n;
+    float r = row ? y[i] : 0.0f;
+    if (k >= 0) {
+        const int c0 = k * kTB;
+        const float yk = y[c0 + lane];  // block k is full: k < b <= last block
+        float a[kTB];
+#pragma unroll
+        for (int j = 0; j < kTB; ++j) a[j] = row ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(yk, j);
+        r = r - acc;
+    }
+    if (b == k + 1) {
+        const int c0 = b * kTB;
+        const int m = n - c0 < kTB ? n - c0 : kTB;  // rows of this block
+        float a[kTB];
+#pragma unroll
+        for (int j = 0; j < kTB; ++j)
+            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < kTB; ++j) {
+            const float yj = lane_value(r, j);  // final: lane j has subtracted columns 0..j-1
+            if (lane > j) r = r - a[j] * yj;
+        }
+    }
+    if (row) y[i] = r;
+}
+
+// Backward step k (k = nb: the last diagonal block alone).  Wave w takes row
+// block k - 1 - w; the wave of block k - 1 also solves its diagonal block (U
+// upper, divisions by its diagonal).
+__global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict__ LU, int n, int k, int nb,
+                                                        float* __restrict__ x) {
+    const int lane = threadIdx.x;
+    const int b = k - 1 - (int)blockIdx.x;
+    const int i = b * kTB + lane;
+    const bool row = i < n;
+    float r = row ? x[i] : 0.0f;
+    if (k < nb) {
+        const int c0 = k * kTB;
+        const int m = n - c0 < kTB ? n - c0 : kTB;
+        const float xk = lane < m ? x[c0 + lane] : 0.0f;
+        float a[kTB];
+#pragma unroll
+        for (int j = 0; j < kTB; ++j)
+            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(xk, j);
+        r = r - acc;
+    }
+    if (b == k - 1) {
+        const int c0 = b * kTB;
+        const int m = n - c0 < kTB ? n - c0 : kTB;
+        float a[kTB];
+#pragma unroll
+        for (int j = 0; j < kTB; ++j)
+            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+#pragma unroll
+        for (int j = kTB - 1; j >= 0; --j) {
+            if (j < m) {
+                if (lane == j) r = r / a[j];  // a[j] of lane j is the diagonal U(j, j)
+                const float xj = lane_value(r, j);
+                if (lane < j) r = r - a[j] * xj;
+            }
+        }
+    }
+    if (row) x[i] = r;
+}
+
 }  // namespace
 }  // namespace vf
 
@@ -572,12 +743,16 @@ struct vf_gn_solver {
     uint32_t* cnt = nullptr;   // G + 1: counts, then offsets
     uint32_t* fill = nullptr;  // G
     uint32_t* rows_of = nullptr;
+    uint32_t* node8 = nullptr;  // row data: members of each row
+    float* j48 = nullptr;       // row data: their Jacobian entries
     uint32_t* bad = nullptr;
     float* JTJ = nullptr;  // LU factors after the factorisation
     int* ipiv = nullptr;
     int* info = nullptr;
     float* r = nullptr;    // residuals, nrows
-    float* x = nullptr;    // JTr, then the update
+    float* x = nullptr;    // JTr
+    float* y = nullptr;    // the permuted JTr, solved in place into the update
+    int* perm = nullptr;   // row interchanges of the factorisation as one permutation
     double* part = nullptr;
     double* energy = nullptr;
     rocblas_handle blas = nullptr;
@@ -587,7 +762,8 @@ namespace vf {
 namespace {
 
 void gn_free(vf_gn_solver* s) {
-    void* ptrs[] = {s->cnt, s->fill, s->rows_of, s->bad, s->JTJ, s->ipiv, s->info, s->r, s->x, s->part, s->energy};
+    void* ptrs[] = {s->cnt, s->fill, s->rows_of, s->node8, s->j48,  s->bad,  s->JTJ,   s->ipiv,
+                    s->info, s->r,   s->x,       s->y,     s->perm, s->part, s->energy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (s->blas) (void)rocblas_destroy_handle(s->blas);
@@ -662,6 +838,14 @@ int build_rows(vf_gn_solver* s, hipStream_t st) {
     VF_LAUNCH_CHECK("k_rows_fill");
     hipLaunchKernelGGL(k_rows_sort, dim3(g.G), dim3(1024), 0, st, (const uint32_t*)s->cnt, s->rows_of, s->bad);
     VF_LAUNCH_CHECK("k_rows_sort");
+    if (!s->node8) {
+        e = gn_alloc(&s->node8, 8ull * s->nrows);
+        if (!e) e = gn_alloc(&s->j48, 48ull * s->nrows);
+        if (e) return e;
+    }
+    hipLaunchKernelGGL(k_row_data, dim3(blocks_for(s->nrows, 256)), dim3(256), 0, st, g, s->nrows, s->node8, s->j48,
+                       s->bad);
+    VF_LAUNCH_CHECK("k_row_data");
     return VF_OK;
 }
 
@@ -676,6 +860,27 @@ int read_bad(vf_gn_solver* s, hipStream_t st) {
         case 3: return fail(VF_ERR_CAPACITY, "gn: a node shares rows with more than %d nodes", kGnMaxNbr);
         default: return fail(VF_ERR_HIP, "gn: inconsistent row lists");
     }
+}
+
+// y = JTJ^-1 x with the factors of vf_gn_solver_create (P, L, U): the gather,
+// then the forward and backward block steps.
+int lu_solve_factored(vf_gn_solver* s, hipStream_t st) {
+    const int n = (int)s->n;
+    const int nb = (n + kTB - 1) / kTB;
+    hipLaunchKernelGGL(k_gather_perm, dim3(blocks_for((uint64_t)n, 256)), dim3(256), 0, st, (const float*)s->x,
+                       (const int*)s->perm, s->y, n);
+    VF_LAUNCH_CHECK("k_gather_perm");
+    for (int k = -1; k <= nb - 2; ++k) {
+        hipLaunchKernelGGL(k_trsv_lower_step, dim3(k < 0 ? 1u : (unsigned)(nb - 1 - k)), dim3(kTB), 0, st,
+                           (const float*)s->JTJ, n, k, s->y);
+        VF_LAUNCH_CHECK("k_trsv_lower_step");
+    }
+    for (int k = nb; k >= 1; --k) {
+        hipLaunchKernelGGL(k_trsv_upper_step, dim3(k == nb ? 1u : (unsigned)k), dim3(kTB), 0, st,
+                           (const float*)s->JTJ, n, k, nb, s->y);
+        VF_LAUNCH_CHECK("k_trsv_upper_step");
+    }
+    return VF_OK;
 }
 
 int launch_residuals(vf_gn_solver* s, const float* warped, const vf_float3* dmesh, const vf_mat4* T, hipStream_t st) {
@@ -865,8 +1070,9 @@ int vf_gn_normal_equations(vf_workspace* ws, float* JTJ, float* JTr, const float
         if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "k_identity launch");
     }
     if (!st) {
-        hipLaunchKernelGGL(k_blocks, dim3(s.rows.G), dim3(64), 0, hs, s.rows, (const uint32_t*)s.cnt,
-                           (const uint32_t*)s.rows_of, r, JTJ, JTr, s.bad);
+        hipLaunchKernelGGL(k_blocks, dim3(s.rows.G), dim3(64), 0, hs, s.rows.G, (const uint32_t*)s.cnt,
+                           (const uint32_t*)s.rows_of, (const uint32_t*)s.node8, (const float*)s.j48, r, JTJ, JTr,
+                           s.bad);
         if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "k_blocks launch");
     }
     if (!st) st = read_bad(&s, hs);
@@ -893,6 +1099,8 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     if (!st) st = gn_alloc(&s->info, 1);
     if (!st) st = gn_alloc(&s->r, s->nrows);
     if (!st) st = gn_alloc(&s->x, s->n);
+    if (!st) st = gn_alloc(&s->y, s->n);
+    if (!st) st = gn_alloc(&s->perm, s->n);
     if (!st) st = gn_alloc(&s->part, kEnergyBlocks);
     if (!st) st = gn_alloc(&s->energy, 1);
     if (!st && rocblas_create_handle(&s->blas) != rocblas_status_success)
@@ -901,12 +1109,17 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     if (!st) st = build_rows(s, hs);
     if (!st) {
         hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, s->JTJ, s->n);
-        hipLaunchKernelGGL(k_blocks, dim3(s->rows.G), dim3(64), 0, hs, s->rows, (const uint32_t*)s->cnt,
-                           (const uint32_t*)s->rows_of, (const float*)nullptr, s->JTJ, (float*)nullptr, s->bad);
+        hipLaunchKernelGGL(k_blocks, dim3(s->rows.G), dim3(64), 0, hs, s->rows.G, (const uint32_t*)s->cnt,
+                           (const uint32_t*)s->rows_of, (const uint32_t*)s->node8, (const float*)s->j48,
+                           (const float*)nullptr, s->JTJ, (float*)nullptr, s->bad);
         if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "normal-equation kernels launch");
     }
     if (!st) st = read_bad(s, hs);
-    // JTJ does not change within the ICP iteration: factorise it once
+    // JTJ does not change within the ICP iteration: factorise it once (the
+    // reference's LU with partial pivoting, linearSolverLUStable).  Partial
+    // pivoting does interchange rows on these matrices (the rotation
+    // columns' couplings exceed their diagonals), so the factorisation
+    // without pivot searches is not an option.
     if (!st && rocsolver_sgetrf(s->blas, (int)s->n, (int)s->n, s->JTJ, (int)s->n, s->ipiv, s->info) !=
                    rocblas_status_success)
         st = fail(VF_ERR_HIP, "rocsolver_sgetrf");
@@ -917,6 +1130,25 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
             st = fail(VF_ERR_HIP, "gn: info read-back");
         else if (hinfo != 0)
             st = fail(VF_ERR_SINGULAR, "gn: the normal matrix has a zero pivot at column %d", hinfo);
+    }
+    if (!st) {
+        // the interchanges (row i with row ipiv[i] - 1, in order) as one gather
+        std::vector<int> piv(s->n), perm(s->n);
+        if (hipMemcpyAsync(piv.data(), s->ipiv, sizeof(int) * s->n, hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess) {
+            st = fail(VF_ERR_HIP, "gn: pivot read-back");
+        } else {
+            for (uint64_t i = 0; i < s->n; ++i) perm[i] = (int)i;
+            for (uint64_t i = 0; i < s->n && !st; ++i) {
+                const int t = piv[i] - 1;
+                if (t < 0 || (uint64_t)t >= s->n) st = fail(VF_ERR_HIP, "gn: pivot %d out of range", piv[i]);
+                else std::swap(perm[i], perm[(size_t)t]);
+            }
+            if (!st && (hipMemcpyAsync(s->perm, perm.data(), sizeof(int) * s->n, hipMemcpyHostToDevice, hs) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(hs) != hipSuccess))
+                st = fail(VF_ERR_HIP, "gn: permutation upload");
+        }
     }
     if (st) {
         gn_free(s);
@@ -965,13 +1197,13 @@ int vf_gn_solver_iterate(vf_gn_solver* s, vf_mat4* transforms, vf_mat4* pending,
         report->final_energy = e;
         if (it != 0) VF_HIP(hipMemcpyAsync(transforms, pending, sizeof(vf_mat4) * G, hipMemcpyDeviceToDevice, hs));
         // JTr from the new residuals, then the back substitution
-        hipLaunchKernelGGL(k_blocks, dim3(G), dim3(64), 0, hs, s->rows, (const uint32_t*)s->cnt,
-                           (const uint32_t*)s->rows_of, (const float*)s->r, (float*)nullptr, s->x, s->bad);
+        hipLaunchKernelGGL(k_blocks, dim3(G), dim3(64), 0, hs, G, (const uint32_t*)s->cnt,
+                           (const uint32_t*)s->rows_of, (const uint32_t*)s->node8, (const float*)s->j48,
+                           (const float*)s->r, (float*)nullptr, s->x, s->bad);
         VF_LAUNCH_CHECK("k_blocks");
-        if (rocsolver_sgetrs(s->blas, rocblas_operation_none, (int)s->n, 1, s->JTJ, (int)s->n, s->ipiv, s->x,
-                             (int)s->n) != rocblas_status_success)
-            return fail(VF_ERR_HIP, "rocsolver_sgetrs");
-        hipLaunchKernelGGL(k_update, dim3(blocks_for(G, 256)), dim3(256), 0, hs, (float*)pending, (const float*)s->x,
+        st = lu_solve_factored(s, hs);
+        if (st) return st;
+        hipLaunchKernelGGL(k_update, dim3(blocks_for(G, 256)), dim3(256), 0, hs, (float*)pending, (const float*)s->y,
                            G);
         VF_LAUNCH_CHECK("k_update");
         st = vf_warp_mesh(s->rows.K, warped, canon, pending, s->rows.knn, s->rows.kw, mesh_size, G, stream);

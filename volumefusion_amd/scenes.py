@@ -467,11 +467,12 @@ def gn_problem(seed: int, G: int = 24, M: int = 600, K: int = 6, R: int = 4, rad
     gknn[:, :min(R, G - 1)] = np.argsort(d2, axis=1)[:, :min(R, G - 1)]
     gknn[rng.random(gknn.shape) < drop_gknn] = np.uint32(0xFFFFFFFF)
     # mesh points around the nodes
+    from scipy.spatial import cKDTree
+
     src = rng.integers(0, G, M)
     canon = (graph[src] + rng.normal(scale=0.4 * radius, size=(M, 3))).astype(F32)
-    dm = ((canon[:, None, :].astype(np.float64) - graph[None, :, :]) ** 2).sum(-1)
-    knn = np.argsort(dm, axis=1)[:, :K].astype(np.uint32)
-    dk = np.sqrt(np.take_along_axis(dm, knn.astype(np.int64), 1))
+    dk, knn = cKDTree(graph.astype(np.float64)).query(canon.astype(np.float64), k=K)
+    dk, knn = dk.reshape(M, K), knn.reshape(M, K).astype(np.uint32)
     w = np.where(dk < radius, np.exp(-(dk ** 2) / (2 * radius ** 2)), 0.0)
     s = w.sum(1, keepdims=True)
     w = np.where(s > 0, w / np.where(s > 0, s, 1.0), 0.0)
@@ -486,13 +487,15 @@ def gn_problem(seed: int, G: int = 24, M: int = 600, K: int = 6, R: int = 4, rad
             x = motion[g]
             U = np.array([[0, -x[2], x[1], x[3]], [x[2], 0, -x[0], x[4]], [-x[1], x[0], 0, x[5]], [0, 0, 0, 0]])
             T_true[g] = (T[g].reshape(4, 4).T - U).T.reshape(16)
-    # targets: every mesh point warped by the true transforms
+    # targets: every mesh point warped by the true transforms (binary64,
+    # slots in order)
+    Mt = T_true.reshape(G, 4, 4).astype(np.float64)  # Mt[g, j, a] = T_g(a, j)
+    p64 = canon.astype(np.float64)
     tgt = np.zeros((M, 3))
-    for i in range(M):
-        p = np.append(canon[i].astype(np.float64), 1.0)
-        for s_ in range(K):
-            if kw[i, s_] > 0:
-                tgt[i] += kw[i, s_] * (T_true[knn[i, s_]].reshape(4, 4).T.astype(np.float64) @ p)[:3]
+    for s_ in range(K):
+        Ts = Mt[knn[:, s_]]
+        moved = (Ts[:, 0, :3] * p64[:, 0:1] + Ts[:, 1, :3] * p64[:, 1:2]) + (Ts[:, 2, :3] * p64[:, 2:3] + Ts[:, 3, :3])
+        tgt += np.where(kw[:, s_:s_ + 1] > 0, kw[:, s_:s_ + 1].astype(np.float64) * moved, 0.0)
     normals = rng.normal(size=(M, 3))
     normals /= np.linalg.norm(normals, axis=1, keepdims=True)
     has_w = kw.sum(1) > 0
