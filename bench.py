@@ -525,10 +525,14 @@ def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
     }
 
 
-def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int = 5) -> dict:
+def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int = 5, max_batch: int = 8,
+                  lanes: int = 8, rounds: int = 32, ramp_rates=(120.0, 240.0, 300.0, 330.0, 360.0, 480.0)) -> dict:
     """Poisson arrivals into the serving loop at 30 fps per stream and at
     rising rates, the streams saturated, and plain per-volume calls (each
-    frame uploaded, then integrated) on resident copies of the same streams."""
+    frame uploaded, then integrated) on resident copies of the same streams.
+    The saturated server and the plain calls both run `rounds` frames per
+    stream (32: the server's fill and drain are then a small part of the
+    run; with 8 rounds they cost it about an eighth)."""
     from volumefusion_amd import _lib as L
     from volumefusion_amd import ops, serve
     from volumefusion_amd.device import DeviceArray, Event, PinnedArray, Stream
@@ -536,7 +540,7 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
     n = len(vols)
     # eight lanes of at most eight frames: a lane uploads its batch while the
     # others compute (measured 1.4x the saturated rate of two lanes of 32)
-    srv = serve.Server(200 << 30, k=8, max_batch=8, lanes=8)
+    srv = serve.Server(200 << 30, k=8, max_batch=max_batch, lanes=lanes)
     ids, keep = [], []
     for v in vols:
         d, kp = serve.scene_desc(v)
@@ -588,12 +592,11 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
                 "submit_lateness_p99_ms": _pct([x * 1e3 for x in late], 99)}
 
     poisson = poisson_run(fps, seconds, 0)
-    ramp = [poisson_run(rate, 1.0, r + 1) for r, rate in enumerate((120.0, 240.0, 300.0, 330.0, 360.0, 480.0))]
+    ramp = [poisson_run(rate, 1.0, r + 1) for r, rate in enumerate(ramp_rates)]
     within = [x for x in [poisson] + ramp if x["p99_ms"] is not None and x["p99_ms"] < 33.0]
     capacity = max(within, key=lambda x: x["achieved_fps"]) if within else None
 
     # saturated: every stream has frames queued
-    rounds = 8
     upd1 = srv.stats()["totals"]["updated"]
     t0 = time.perf_counter()
     for r in range(rounds):
@@ -657,6 +660,7 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
                     f"640x576 frames from page-locked host memory, Poisson arrivals",
         "poisson": poisson, "ramp": ramp,
         "capacity_within_p99_33ms_fps": capacity["achieved_fps"] if capacity else None,
+        "server": {"max_batch": max_batch, "lanes": lanes, "rounds_per_stream": rounds},
         "saturated": saturated, "plain": plain,
         "saturated_vs_plain": saturated["fps"] / plain["with_upload"]["fps"], "hbm": hbm,
     }

@@ -601,8 +601,12 @@ typedef struct vf_server_stats {
     vf_frame_stats totals;    /* kernel counters summed over every batch     */
 } vf_server_stats;
 
-/* Device bytes a stream with this description is charged at admission:
- * its buffers plus its share of `lanes` workspaces. */
+/* The most a stream with this description adds to a server's admission
+ * charge: its buffers plus `lanes` times what it adds to a workspace whose
+ * batch holds it (with half again for the workspace's growth).  A server
+ * charges its open streams' buffers plus `lanes` workspaces sized for the
+ * batch of its max_batch largest workspace shares; a stream is admitted
+ * when that charge, with it, fits the HBM budget. */
 uint64_t vf_server_stream_bytes(const vf_stream_desc* desc, int lanes);
 int vf_server_create(vf_server** server, const vf_server_config* config);
 /* Finishes every submitted frame, then frees everything. */

@@ -100,6 +100,27 @@ def test_admission_by_budget(vf):
     srv.close_server()
 
 
+def test_admission_charges_lane_workspaces_once(vf):
+    """The charge is every open stream's buffers plus `lanes` workspaces sized
+    for the max_batch largest workspace shares, not a share of every lane per
+    stream: with max_batch 2 and four streams, the two largest shares count."""
+    sc = _scenes()[:4]
+    descs = [serve.scene_desc(s) for s in sc]
+    ws = [serve.stream_bytes(d, 2) - serve.stream_bytes(d, 1) for d, _ in descs]
+    own = [serve.stream_bytes(d, 1) - w for (d, _), w in zip(descs, ws)]
+    lanes, max_batch = 3, 2
+    srv = serve.Server(1 << 40, k=8, max_batch=max_batch, lanes=lanes)
+    ids = [srv.open(d) for d, _ in descs]
+    top = sorted(ws, reverse=True)[:max_batch]
+    assert srv.stats()["bytes_in_use"] == sum(own) + lanes * sum(top)
+    assert srv.stats()["bytes_in_use"] < sum(serve.stream_bytes(d, lanes) for d, _ in descs)
+    big = int(np.argmax(ws))
+    srv.close(ids[big])
+    rest = [w for i, w in enumerate(ws) if i != big]
+    assert srv.stats()["bytes_in_use"] == sum(own) - own[big] + lanes * sum(sorted(rest, reverse=True)[:max_batch])
+    srv.close_server()
+
+
 def test_borrowed_submit_equals_copied(vf):
     """Frames submitted from page-locked memory without a copy give the bits of
     copied submissions."""

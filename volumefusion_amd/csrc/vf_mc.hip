@@ -157,6 +157,39 @@ __device__ __forceinline__ unsigned long long edge_codes(const McGeom& g, const 
     return c;
 }
 
+// No edge point of the cube can equal a corner (a screen before the exact
+// test of edge_codes).  mc_interp returns a corner when a corner value is
+// within 1e-7 of iso, or when the lerp's displacement mu * L along the edge
+// is at most half an ulp of the corner coordinate (<= 2^-24 P, P the largest
+// |coordinate| of the cube's corners, L the edge length as the float
+// difference of the corners).  With m and M the smallest and largest
+// |t - iso| over the corners, every crossed edge has mu and 1 - mu at least
+// (1 - 2^-22) m / (2 M) after float rounding; m >= 2e-7, m >= 2^-16 M and
+// m L >= 2^-19 P M then put the displacement above 2^-21 P on both ends.
+// NaN or infinite corner values fail the screen.
+__device__ __forceinline__ bool no_snap(const McGeom& g, const Cube& cb) {
+    float m = 3.0e38f, M = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const float d = fabsf(cb.t[v] - g.iso);
+        if (!(d <= 3.0e38f)) return false;
+        m = fminf(m, d);
+        M = fmaxf(M, d);
+    }
+    if (!(m >= 2.0e-7f)) return false;
+    float p0[3], p1[3];
+    mc_world(g, cb.i, cb.j, cb.k, p0);
+    mc_world(g, cb.i + 1, cb.j + 1, cb.k + 1, p1);
+    double L = 3.0e38, P = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        L = fmin(L, (double)(p1[a] - p0[a]));
+        P = fmax(P, fmax(fabs((double)p0[a]), fabs((double)p1[a])));
+    }
+    if (!(L > 0.0)) return false;
+    return (double)m >= 0x1p-16 * (double)M && (double)m * L >= 0x1p-19 * P * (double)M;
+}
+
 // Output vertex count of a cube: 3 per triangle of its case (FULL, dropping
 // triangles with two equal points, src/MarchingCubes.cu:400-410) or 1 per
 // crossed edge.  With FULL, bit q of keep marks the kept triangles.
@@ -169,7 +202,11 @@ __device__ __forceinline__ int cube_outputs(const McGeom& g, const Cube& cb, uns
     const unsigned long long t = kMcTri[cb.cs];
     const int nt = (int)(t & 15);
     int n = 0;
-    if (corners_distinct(g, cb)) {
+    if (no_snap(g, cb)) {
+        // every edge point lies strictly inside its edge: no two coincide
+        keep = (1u << nt) - 1u;
+        n = 3 * nt;
+    } else if (corners_distinct(g, cb)) {
         const unsigned long long codes = edge_codes(g, cb, mask);
         for (int q = 0; q < nt; ++q) {
             const int c0 = (int)(codes >> (4 * ((t >> (4 + 12 * q)) & 15))) & 15;
@@ -292,12 +329,13 @@ __device__ __forceinline__ uint32_t slice_bits(const Slice& s, float iso) {
     return (s.t0 > iso ? 1u : 0u) | (s.t1 > iso ? 2u : 0u) | ((s.w0 == 0.0f || s.w1 == 0.0f) ? 4u : 0u);
 }
 
-// FULL needs about 120 VGPRs for surface cubes (edge points of the
-// degenerate-triangle test).  Four waves per SIMD keep them in registers;
-// capping the count at eight waves (64 VGPRs) spilled 152 B per lane and
-// measured 0.76 ms for the full 512^3 extraction against 0.70 ms.
+// FULL needs about 120 VGPRs for the exact degenerate-triangle test of the
+// cubes no_snap cannot clear.  With the screen that path is rare, and six
+// waves per SIMD measured best for the full 512^3 extraction: 0.572 ms,
+// against 0.638 ms at four waves (no spills) and 0.617 ms at eight (168 B of
+// scratch per lane).
 template <bool FULL>
-__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_mc_count(
+__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_mc_count(
     McGeom g, const float* __restrict__ tsdf, const float* __restrict__ wt, uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_lut[256];
     static_assert(kMcBlock == 256, "one lookup entry per thread");

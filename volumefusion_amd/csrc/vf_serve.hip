@@ -26,6 +26,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -67,7 +68,8 @@ struct StreamState {
     float* radii = nullptr;
     vf_mat4* T = nullptr;
     float* depth = nullptr;
-    uint64_t bytes = 0;
+    uint64_t own = 0;  // its buffers
+    uint64_t ws = 0;   // what it adds to a workspace whose batch holds it
     std::deque<Pending> queue;
     bool busy = false;  // a frame in flight
 };
@@ -114,7 +116,9 @@ struct vf_server {
     uint64_t next_ticket = 1;
     uint64_t next_batch = 0;
     uint64_t submitted = 0, completed = 0, batches = 0;
-    uint64_t bytes_in_use = 0;
+    uint64_t bytes_in_use = 0;            // the admission charge of the open (and opening) streams
+    uint64_t own_total = 0;               // their buffers
+    std::multiset<uint64_t> ws_open;      // their workspace shares
     int max_batch_seen = 0;
     int inflight = 0;
     bool stop = false;
@@ -297,19 +301,41 @@ Staging* take_staging(vf_server* sv, uint64_t bytes) {
     return nullptr;
 }
 
-uint64_t stream_bytes(const vf_stream_desc& d, int lanes) {
+// A stream's buffers: tsdf and weight, node positions, radii, transforms,
+// depth frame.
+uint64_t stream_own_bytes(const vf_stream_desc& d) {
+    const uint64_t n = voxel_count(d.dim);
+    return 8 * n + (uint64_t)d.nodes * (12 + 4 + 64) + (uint64_t)std::max(d.width, 0) * std::max(d.height, 0) * 4;
+}
+
+// What a stream adds to the workspace of a batch that holds it (workspace
+// needs add up over a batch's volumes): brick lists (<= 4 B x 128 per 256
+// voxels), collision bitmap (1/4 B per voxel), node records (96 B), pending
+// records and the overflow arena (2 x 16 B per possible update), plus the
+// arenas' per-workgroup slack (<= 1024 workgroups x 512 records x 16 B); and
+// half again for the workspace's geometric growth.
+uint64_t stream_ws_bytes(const vf_stream_desc& d) {
     const uint64_t n = voxel_count(d.dim);
     const uint64_t g = d.nodes;
     const uint64_t cube = (uint64_t)std::max(d.voxels_per_node.x, 0) * std::max(d.voxels_per_node.y, 0) *
                           std::max(d.voxels_per_node.z, 0);
-    const uint64_t own = 8 * n + g * (12 + 4 + 64) + (uint64_t)std::max(d.width, 0) * std::max(d.height, 0) * 4;
-    // per workspace: brick lists (<= 4 B x 128 per 256 voxels), collision
-    // bitmap (1/4 B per voxel), node records (96 B), pending records and
-    // the overflow arena (2 x 16 B per possible update), plus the arenas'
-    // per-workgroup slack (<= 1024 workgroups x 512 records x 16 B)
     const uint64_t records = std::min<uint64_t>(n, g * cube);
     const uint64_t ws = 2 * n + n / 4 + g * 96 + 32 * records + (16u << 20);
-    return own + (uint64_t)std::max(lanes, 1) * ws;
+    return ws + ws / 2;
+}
+
+uint64_t stream_bytes(const vf_stream_desc& d, int lanes) {
+    return stream_own_bytes(d) + (uint64_t)std::max(lanes, 1) * stream_ws_bytes(d);
+}
+
+// The server's charge: every open stream's buffers, and `lanes` workspaces
+// each large enough for the batch of the max_batch largest workspace shares
+// (no batch holds more).  Caller holds sv->mu.
+uint64_t server_charge(const vf_server* sv, uint64_t own_total, const std::multiset<uint64_t>& ws) {
+    uint64_t top = 0;
+    int k = 0;
+    for (auto it = ws.rbegin(); it != ws.rend() && k < sv->cfg.max_batch; ++it, ++k) top += *it;
+    return own_total + (uint64_t)std::max(sv->cfg.lanes, 1) * top;
 }
 
 }  // namespace
@@ -399,25 +425,42 @@ int vf_server_open(vf_server* sv, const vf_stream_desc* d, int wait_ms, int32_t*
     if (d->width < 0 || d->height < 0) return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_open: negative frame size");
     if (d->nodes > 0 && (!d->points || !d->radii))
         return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_open: node positions and radii must not be NULL");
-    const uint64_t bytes = stream_bytes(*d, sv->cfg.lanes);
+    const uint64_t own = stream_own_bytes(*d), wsb = stream_ws_bytes(*d);
     {
         std::unique_lock<std::mutex> lk(sv->mu);
-        const auto fits = [&] { return sv->bytes_in_use + bytes <= sv->cfg.hbm_budget; };
+        uint64_t charge = 0;
+        const auto fits = [&] {
+            std::multiset<uint64_t> with = sv->ws_open;
+            with.insert(wsb);
+            charge = server_charge(sv, sv->own_total + own, with);
+            return charge <= sv->cfg.hbm_budget;
+        };
         if (!fits()) {
             if (wait_ms > 0) sv->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), fits);
             if (!fits())
-                return fail(VF_ERR_CAPACITY, "stream needs %llu B; %llu of %llu B in use",
-                            (unsigned long long)bytes, (unsigned long long)sv->bytes_in_use,
+                return fail(VF_ERR_CAPACITY, "stream would bring the charge to %llu B; %llu of %llu B in use",
+                            (unsigned long long)charge, (unsigned long long)sv->bytes_in_use,
                             (unsigned long long)sv->cfg.hbm_budget);
         }
-        sv->bytes_in_use += bytes;  // reserved while allocating
+        // reserved while allocating
+        sv->own_total += own;
+        sv->ws_open.insert(wsb);
+        sv->bytes_in_use = charge;
     }
+    const auto unreserve = [&] {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->own_total -= own;
+        sv->ws_open.erase(sv->ws_open.find(wsb));
+        sv->bytes_in_use = server_charge(sv, sv->own_total, sv->ws_open);
+        sv->cv_done.notify_all();
+    };
     StreamState s;
     s.open = true;
     s.desc = *d;
     s.desc.points = nullptr;
     s.desc.radii = nullptr;
-    s.bytes = bytes;
+    s.own = own;
+    s.ws = wsb;
     const uint64_t g = d->nodes;
     const uint64_t px = (uint64_t)d->width * (uint64_t)d->height;
     hipError_t e = hipSetDevice(sv->cfg.device);
@@ -438,9 +481,7 @@ int vf_server_open(vf_server* sv, const vf_stream_desc* d, int wait_ms, int32_t*
     if (hs) (void)hipStreamDestroy(hs);
     if (e != hipSuccess || fst) {
         free_stream(s);
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->bytes_in_use -= bytes;
-        sv->cv_done.notify_all();
+        unreserve();
         if (fst) return fst;
         return fail(e == hipErrorOutOfMemory ? VF_ERR_OUT_OF_MEMORY : VF_ERR_HIP, "vf_server_open: %s",
                     hipGetErrorString(e));
@@ -470,8 +511,10 @@ int vf_server_close(vf_server* sv, int32_t sid) {
     StreamState& s = *sv->streams[sid];
     (void)hipSetDevice(sv->cfg.device);
     free_stream(s);
-    sv->bytes_in_use -= s.bytes;
-    s.bytes = 0;
+    sv->own_total -= s.own;
+    sv->ws_open.erase(sv->ws_open.find(s.ws));
+    sv->bytes_in_use = server_charge(sv, sv->own_total, sv->ws_open);
+    s.own = s.ws = 0;
     sv->cv_done.notify_all();
     return VF_OK;
 }
