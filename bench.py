@@ -611,7 +611,10 @@ def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int =
                  "updates_per_s": (st["totals"]["updated"] - upd1) / sat_wall,
                  "mean_batch_frames": float(np.mean([r_["batch_frames"] for r_ in res])),
                  "p50_ms": _pct(sat_lat, 50), "p99_ms": _pct(sat_lat, 99)}
-    hbm = {"bytes_admitted": st["bytes_in_use"], "budget": st["hbm_budget"], "streams": st["streams_open"]}
+    resident = sum(8 * v.n_voxels for v in vols)  # the streams' tsdf and weights
+    hbm = {"bytes_admitted": st["bytes_in_use"], "budget": st["hbm_budget"], "streams": st["streams_open"],
+           "tsdf_weight_bytes": resident, "admitted_vs_tsdf_weight": st["bytes_in_use"] / resident,
+           "redo_bricks": st["totals"]["redo_bricks"]}
     srv.close_server()
 
     # plain per-volume calls: upload the frame, integrate, next volume

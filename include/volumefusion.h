@@ -333,6 +333,8 @@ typedef struct vf_frame_stats {
     uint64_t overflow_bricks; /* bricks whose node list spilled            */
     uint64_t screened_bricks; /* bricks with a node list the screen proved
                                  unable to change the frame (not swept)    */
+    uint64_t redo_bricks;     /* bricks whose pending updates exceeded the
+                                 record budget (recomputed by the redo pass) */
 } vf_frame_stats;
 
 
@@ -405,6 +407,17 @@ typedef struct vf_tuning {
 
 /* Tuning used by later vf_warped_integrate* calls on this workspace. */
 int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning);
+
+/* Pending-record budget of later frames on this workspace (no counterpart in
+ * the reference, whose per-voxel buffers hold every voxel's result): room for
+ * ceil(records_per_voxel x the batch's voxels) pending updates, at most the
+ * worst case min(N, G x cube volume) per volume; 0, the default, sizes for the
+ * worst case.  The bricks whose updates do not fit are recomputed by one more
+ * pass after pass A, which writes them in place; results and statistics are
+ * the same.  With a budget the workspace's buffers grow to exactly what a
+ * frame needs, without headroom, so that the owner can bound the workspace
+ * (the serving loop does).  Range 0..1. */
+int vf_workspace_set_record_budget(vf_workspace* ws, float records_per_voxel);
 
 /* ------------------------------------------------------------------------ */
 /* Frame plans: a fixed batch, replayed every frame                         */

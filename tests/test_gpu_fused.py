@@ -15,15 +15,17 @@ def dev(a):
 
 
 def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True, tuning=None, blend=0,
-               screen=True, depth_fn=None, transforms_fn=None, f64=False):
+               screen=True, depth_fn=None, transforms_fn=None, f64=False, budget=None, per_frame=None):
     """Frames through the fused call against the oracle, bit for bit after
     every frame.  With the brick screen off the supported and in-bounds
     counts equal the oracle's too; with it they cover the swept bricks only.
     With f64 the gather oracle also carries the binary64 companion state, and
     the device tsdf's max and RMS error against it (in units of the
     truncation distance: the stored tsdf is sdf / delta) is checked after
-    every frame and reported.  Returns the last frame's (oracle stats, device
-    stats)."""
+    every frame and reported.  With budget, the workspace gets that
+    pending-record budget (records per voxel).  per_frame, a list, receives
+    each frame's device statistics.  Returns the last frame's (oracle stats,
+    device stats)."""
     n = s.n_voxels
     t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
     t64 = w64 = None
@@ -37,6 +39,8 @@ def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True
     ws = ops.Workspace()
     tuning = tuple(tuning) if tuning is not None else (0, 0)
     ws.set_tuning(tuning[0], tuning[1], screen)
+    if budget is not None:
+        ws.set_record_budget(budget)
     gs = None
     for f in frames:
         depth = depth_fn(f) if depth_fn else s.depth(f)
@@ -52,6 +56,8 @@ def run_frames(vf, oracle, s, frames, gather=False, threads=16, check_stats=True
         ops.warped_integrate(ws, tsdf, graph, dev(depth), s.intrinsics(), s.k, s.delta, s.max_weight, s.cube,
                              blend=blend)
         gs = ws.last_stats()
+        if per_frame is not None:
+            per_frame.append(gs)
         if check_stats:
             if not screen:
                 assert gs["supported"] == st["supported"]
@@ -93,6 +99,61 @@ def test_fused_spill_path(vf, oracle):
     s = scenes.small_scene(seed=12, k=8, nodes=400, radius=0.03)
     run_frames(vf, oracle, s, [0, 1])
     run_frames(vf, oracle, s, [0, 1], screen=False)
+
+
+@pytest.mark.parametrize("screen", [True, False])
+@pytest.mark.parametrize("budget", [1e-7, 0.02, 1.0])
+def test_fused_record_budget(vf, oracle, budget, screen):
+    """A pending-record budget: with room for one record every brick with a
+    pending voxel goes to the redo pass; with room for about 1,300 (a few
+    records per workgroup arena, the rest in the overflow arena) some bricks
+    are stored, some overflow reservations are left as null records by
+    bricks that then go to the redo pass, and the rest are redone; a budget
+    of one record per voxel holds everything.  Every frame's bits equal the
+    oracle's, and every statistic the default workspace's."""
+    s = scenes.small_scene(seed=11, k=8, nodes=70)
+    frames = [0, 1, 2]
+    ref, got = [], []
+    run_frames(vf, oracle, s, frames, screen=screen, per_frame=ref)
+    run_frames(vf, oracle, s, frames, screen=screen, budget=budget, per_frame=got)
+    for f, (a, b) in enumerate(zip(ref, got)):
+        assert a["redo_bricks"] == 0
+        assert {k: v for k, v in a.items() if k != "redo_bricks"} == \
+               {k: v for k, v in b.items() if k != "redo_bricks"}, f"frame {f}"
+    redo = [g["redo_bricks"] for g in got]
+    if budget < 1e-6:
+        assert min(redo) > 10, redo
+    elif budget < 0.5:
+        assert min(redo) > 0 and max(redo) < max(g["active_bricks"] for g in got), redo
+    else:
+        assert max(redo) == 0, redo
+
+
+def test_batch_record_budget_with_dqb_and_spill(vf, oracle):
+    """A batch on a workspace with a small record budget: volumes of uneven
+    size, LBS and DQB, one with a node cluster whose brick lists spill; the
+    redo pass finds each brick's volume and list.  Bits equal the oracle's."""
+    vols = [scenes.small_scene(seed=20 + i, k=8, nodes=30 + 25 * i, dim=(24 + 8 * i, 32, 20 + 12 * i))
+            for i in range(3)]
+    vols.append(scenes.small_scene(seed=12, k=8, nodes=400, radius=0.03))
+    blends = [0, 1, 0, 1]
+    tsdfs = [ops.DeviceTsdf(s.size, s.dim, center=s.center, delta=s.delta) for s in vols]
+    graphs = [ops.DeviceMotionGraph(s.nodes, s.transforms(0), s.radii) for s in vols]
+    depths = [dev(s.depth(0)) for s in vols]
+    ws = ops.Workspace()
+    ws.set_record_budget(1e-3)
+    ops.warped_integrate_batch(ws, tsdfs, graphs, depths, [s.intrinsics() for s in vols], 8, 0.012, 64.0,
+                               [s.cube for s in vols], blend=blends)
+    st = ws.last_stats()
+    assert st["redo_bricks"] > 0 and st["overflow_bricks"] > 0, st
+    for s, t, g, b in zip(vols, tsdfs, graphs, blends):
+        n = s.n_voxels
+        t_or, w_or, nodes_or = np.full(n, -1, F), np.zeros(n, F), s.nodes.copy()
+        oracle.warped_frame(s.k, t_or, w_or, nodes_or, s.radii, s.transforms(0), s.dim, s.voxel_size, s.size,
+                            s.cube, s.center, s.depth(0), s.delta, s.intrinsics(), s.max_weight, blend=b)
+        np.testing.assert_array_equal(t.tsdf.download(), t_or)
+        np.testing.assert_array_equal(t.weights.download(), w_or)
+        np.testing.assert_array_equal(g.points.download(), nodes_or)
 
 
 def _special_depth(s, f, nan):

@@ -85,6 +85,7 @@ class FrameStats(C.Structure):
         ("active_bricks", C.c_uint64),
         ("overflow_bricks", C.c_uint64),
         ("screened_bricks", C.c_uint64),
+        ("redo_bricks", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -249,6 +250,7 @@ _SIGS = {
     "vf_workspace_set_timing": [vp, C.c_int],
     "vf_workspace_kernel_ms": [vp, C.POINTER(f32), C.POINTER(C.c_int)],
     "vf_workspace_set_tuning": [vp, C.POINTER(Tuning)],
+    "vf_workspace_set_record_budget": [vp, f32],
     "vf_plan_create": [C.POINTER(vp), C.c_int, C.POINTER(Volume), C.POINTER(Graph), C.POINTER(Frame),
                        C.POINTER(WarpParams), C.POINTER(Tuning), C.c_int, vp],
     "vf_plan_destroy": [vp],

@@ -7,16 +7,17 @@
 //
 //   k_prep_bin  one thread per (node, brick overlapped by the node's cube):
 //               node record (translated position, cube corner, 2r, 2r^2 and
-//               its reciprocal, transform rows); append the node to the
-//               brick's list; the brick's displacement bound.  Also clears
-//               the collision tiles the previous frame flagged, and (extra
+//               its reciprocal, transform rows); a slot in the brick's node
+//               list; the brick's displacement bound.  Also clears the
+//               collision tiles the previous frame flagged, and (extra
 //               workgroups) takes the depth-tile maxima of the frame.
 //   k_compact   two bricks per thread: the bricks with a non-empty node list
-//               in brick order within each workgroup's run, each with the
-//               brick screen's verdict; bricks that may hold a pending voxel
-//               mark their target tiles.
+//               in brick order within each workgroup's run, each with its
+//               place in the node-list pool and the brick screen's verdict;
+//               bricks that may hold a pending voxel mark their target tiles.
 //   k_select    the active list: the candidates the screen keeps (see the
-//               brick screen below).
+//               brick screen below); fills the node lists (k_fill_lists
+//               does that without the screen).
 //   k_pass_a    one workgroup per active brick at a time, bricks drawn from
 //               per-XCD queues (one voxel per thread, each wave a 4x4x4
 //               block): stage the brick's node list in LDS (each cube as
@@ -28,7 +29,11 @@
 //               merged into the global bitmap once per brick, a word at a
 //               time.  Voxels that pass the projective tests append a 16 B
 //               pending record (voxel, target, new tsdf, new weight) to the
-//               workgroup's own record arena.
+//               workgroup's own record arena (a brick's records together:
+//               there, else in a shared overflow arena, else nowhere and
+//               the brick joins the redo list).
+//   (redo)      only with a record budget: pass A again over the redo list,
+//               writing each pending voxel whose target was reached once.
 //   k_pass_b    one thread per pending record: drop it if its target was
 //               reached twice, else write tsdf and weight.
 //
@@ -97,7 +102,8 @@ enum {
     kStActive,
     kStSpilled,
     kStScreened,
-    kStCount = 8
+    kStRedo,
+    kStCount = 9
 };
 
 // Depth-tile maxima for the brick screen: tiles of 16 and of 128 pixels.
@@ -159,7 +165,9 @@ struct Ctrl {
     uint32_t active_count[2];  // by frame parity
     uint32_t cand_count[2];    // bricks with a node list before the screen, by frame parity
     uint32_t queue[2][kQueues * kQueueStride];  // pass-A queue counters by frame parity
-    uint32_t overflow_records;
+    uint32_t overflow_records;  // reserved in the shared overflow arena (may exceed its capacity)
+    uint32_t redo_count;        // bricks whose records fit no arena, recomputed by the redo pass
+    uint32_t list_total;        // node-list pool entries handed out by k_compact
     uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
 };
 
@@ -336,7 +344,7 @@ __device__ void depth_tiles(const VolDesc* __restrict__ vols, int nvol, uint32_t
 template <class S>
 __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     const VolDesc* __restrict__ vols, int nvol, uint32_t total, uint32_t pair_blocks, NodeRec* __restrict__ nrec,
-    uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_nodes, uint32_t* __restrict__ brick_disp,
+    uint32_t* __restrict__ brick_count, uint2* __restrict__ items, uint32_t* __restrict__ brick_disp,
     float* __restrict__ dmax, Ctrl* __restrict__ ctrl, uint32_t* __restrict__ bitmap,
     uint32_t* __restrict__ tile_flag, uint32_t clear_tiles, unsigned long long* __restrict__ stat) {
     if (blockIdx.x >= pair_blocks) {
@@ -359,10 +367,17 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
         }
     }
     if (gid < kStCount * kStatSlots) stat[gid] = 0ull;
-    if (gid == 0) ctrl->overflow_records = 0u;
+    if (gid == 0) {
+        ctrl->overflow_records = 0u;
+        ctrl->redo_count = 0u;
+        ctrl->list_total = 0u;
+    }
     __syncthreads();
 
-    // This thread's (node, brick) pair: the node joins the brick's list.
+    // This thread's (node, brick) pair: the node takes a slot in the brick's
+    // list, recorded as (brick, node << 7 | slot) for the list fill once
+    // k_compact has placed every list in the pool ({kNone, 0}: no entry).
+    uint2 entry = make_uint2(kNone, 0u);
     do {
         if (gid >= total) break;
         int v = 0, hi = nvol - 1;  // last volume whose first item is <= gid
@@ -443,7 +458,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
         const uint32_t b = ((uint32_t)bz * (uint32_t)vd.bdim[1] + (uint32_t)by) * (uint32_t)vd.bdim[0] + (uint32_t)bx;
         const uint32_t gb = vd.brick_base + b;
         const uint32_t slot = atomicAdd(brick_count + gb, 1u);
-        if (slot < (uint32_t)kCap) brick_nodes[(uint64_t)gb * kCap + slot] = j;
+        if (slot < (uint32_t)kCap) entry = make_uint2(gb, (j << kSlotBits) | slot);
         if (vd.screen) {
             // displacement bound of this node over its cube within this brick
             const int32_t bb[3] = {bx, by, bz};
@@ -460,6 +475,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
             atomicMax(brick_disp + gb, disp_bound_bits(vd, vd.T + 16ull * j, m, vd.radii[j], vlo, vhi));
         }
     } while (false);
+    if (gid < total) items[gid] = entry;
 }
 
 // The frame's active bricks (those with a non-empty node list), in brick
@@ -501,6 +517,26 @@ __device__ __forceinline__ uint32_t compact_run(uint32_t flags, uint32_t* counte
     }
     __syncthreads();
     return *s_base;
+}
+
+// Exclusive prefix of v over the workgroup's threads, plus a base taken from
+// *counter for the workgroup's total (the whole workgroup calls it).
+__device__ __forceinline__ uint32_t block_alloc(uint32_t v, uint32_t* counter, uint32_t* s_w, uint32_t* s_base) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_sum(v);
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        static_assert(kCompactWaves <= 64, "one wave total a lane");
+        const uint32_t x = lane < kCompactWaves ? s_w[lane] : 0u;
+        const uint32_t inc = wave_inclusive_sum(x);
+        if (lane < kCompactWaves) s_w[lane] = inc - x;
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        if (lane == 0) *s_base = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    return *s_base + s_w[wave] + (incl - v);
 }
 
 // List position of this thread's slot r after compact_run (whole waves call it).
@@ -660,7 +696,9 @@ __device__ __forceinline__ uint32_t tile_span(uint32_t r) { return (r >> 16) - (
 // The frame's active bricks (those with a non-empty node list), in brick
 // order within each workgroup's run of kCompactPer * kPrepThreads bricks; one
 // list append per workgroup (appends to one counter serialise, so a
-// workgroup takes many bricks).  With SCREEN they go to the candidate list
+// workgroup takes many bricks).  Each brick whose list holds at most kCap
+// nodes gets its place in the node-list pool (one pool allocation per
+// workgroup); the lists are filled after this kernel.  With SCREEN they go to the candidate list
 // with the screen's verdict, and pending-capable bricks mark their target
 // tiles (tag: this frame's number + 1) for k_select.  The run's candidates
 // are first packed in LDS, then screened by groups of kGrpC lanes, so that
@@ -669,22 +707,36 @@ __device__ __forceinline__ uint32_t tile_span(uint32_t r) { return (r >> 16) - (
 template <class S, bool SCREEN>
 __global__ __launch_bounds__(kPrepThreads) void k_compact(
     const VolDesc* __restrict__ vols, int nvol, uint32_t nbricks, const uint32_t* __restrict__ brick_count,
-    const uint32_t* __restrict__ brick_disp, const float* __restrict__ dmax, uint32_t* __restrict__ active2,
+    uint32_t* __restrict__ brick_off, const uint32_t* __restrict__ brick_disp, const float* __restrict__ dmax,
+    uint32_t* __restrict__ active2,
     uint32_t* __restrict__ cand, uint4* __restrict__ cand_info, uint32_t* __restrict__ tile_mark,
     uint32_t* __restrict__ vol_mark, Ctrl* __restrict__ ctrl, uint64_t brick_cap, int parity_arg) {
     __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];  // per (round, wave): count, then first position
     __shared__ uint32_t s_base, s_total;
+    __shared__ uint32_t s_lw[kCompactWaves], s_lbase;
     __shared__ uint32_t s_bb[kMaxBatch];  // first brick of each volume
     __shared__ uint32_t s_item[SCREEN ? kCompactPer * kPrepThreads : 1];  // the run's candidates, in list order
     for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_bb[v] = vols[v].brick_base;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     const uint64_t gb0 = (uint64_t)blockIdx.x * (kPrepThreads * kCompactPer) + threadIdx.x;
     uint32_t flags = 0;  // bit r: brick gb0 + r * kPrepThreads is active
+    uint32_t lw[kCompactPer], lsum = 0;  // pool entries of each brick (0 for a spilled list)
 #pragma unroll
     for (int r = 0; r < kCompactPer; ++r) {
         const uint64_t gb = gb0 + (uint64_t)(r * kPrepThreads);
-        const bool act = gb < nbricks && brick_count[gb] != 0u;
-        flags |= (uint32_t)act << r;
+        const uint32_t c = gb < nbricks ? brick_count[gb] : 0u;
+        flags |= (uint32_t)(c != 0u) << r;
+        lw[r] = c <= (uint32_t)kCap ? c : 0u;
+        lsum += lw[r];
+    }
+    {
+        uint32_t off = block_alloc(lsum, &ctrl->list_total, s_lw, &s_lbase);
+#pragma unroll
+        for (int r = 0; r < kCompactPer; ++r) {
+            // a spilled list has no place: its entries are dropped
+            if ((flags >> r) & 1u) brick_off[gb0 + (uint64_t)(r * kPrepThreads)] = lw[r] ? off : kNone;
+            off += lw[r];
+        }
     }
     const uint32_t base = compact_run(flags, SCREEN ? &ctrl->cand_count[parity] : &ctrl->active_count[parity],
                                       s_cnt, &s_base, &s_total);
@@ -739,6 +791,38 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
     }
 }
 
+// The node lists: every (node, brick) entry of k_prep_bin into the brick's
+// place in the pool, at its slot (a brick whose list spilled has none).
+// Four entries a thread at a time: their loads, then their offsets' loads,
+// then the stores (two dependent loads in flight for all four).
+__device__ __forceinline__ void fill_lists(const uint2* __restrict__ items, uint32_t total,
+                                           const uint32_t* __restrict__ brick_off, uint32_t* __restrict__ brick_nodes,
+                                           uint32_t gid, uint32_t nthreads) {
+    constexpr int kU = 4;
+    for (uint64_t i0 = gid; i0 < total; i0 += (uint64_t)kU * nthreads) {
+        uint2 e[kU];
+        uint32_t off[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * nthreads;
+            e[u] = i < total ? items[i] : make_uint2(kNone, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) off[u] = e[u].x != kNone ? brick_off[e[u].x] : kNone;
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (off[u] != kNone) brick_nodes[off[u] + (e[u].y & kSlotMask)] = e[u].y >> kSlotBits;
+    }
+}
+
+// The list fill alone, for frames without the brick screen (k_select does it
+// otherwise).
+__global__ __launch_bounds__(kPrepThreads) void k_fill_lists(const uint2* __restrict__ items, uint32_t total,
+                                                             const uint32_t* __restrict__ brick_off,
+                                                             uint32_t* __restrict__ brick_nodes) {
+    fill_lists(items, total, brick_off, brick_nodes, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // The active list from the candidates: a brick stays when the screen kept it
 // or when its target tiles meet a tile a pending-capable brick marked (or its
 // volume is marked whole).  Dropped bricks get their node-list count cleared
@@ -746,7 +830,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
 // cleared for the next frame.  A group of kGrp lanes takes one candidate
 // (lane q tests tiles q, q + 8, ...); a workgroup takes kSelPerBlock
 // candidates at a time, in list order, so the active list keeps the
-// candidates' order.
+// candidates' order.  The workgroups also fill the node lists.
 constexpr uint32_t kSelPerRound = kPrepThreads / kGrp;
 constexpr uint32_t kSelPerBlock = kCompactPer * kSelPerRound;
 
@@ -754,12 +838,15 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
     const VolDesc* __restrict__ vols, const uint32_t* __restrict__ cand, const uint4* __restrict__ cand_info,
     uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_disp, const uint32_t* __restrict__ tile_mark,
     const uint32_t* __restrict__ vol_mark, uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
-    uint64_t brick_cap, int parity_arg, unsigned long long* __restrict__ stat,
+    uint64_t brick_cap, int parity_arg, const uint2* __restrict__ items, uint32_t total_items,
+    const uint32_t* __restrict__ brick_off, uint32_t* __restrict__ brick_nodes, unsigned long long* __restrict__ stat,
     unsigned long long* __restrict__ totals) {
     __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];
     __shared__ uint32_t s_base;
     __shared__ unsigned long long s_drop;
     if (threadIdx.x == 0) s_drop = 0ull;
+    fill_lists(items, total_items, brick_off, brick_nodes, blockIdx.x * blockDim.x + threadIdx.x,
+               gridDim.x * blockDim.x);
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     const uint32_t n = ctrl->cand_count[parity];
     const uint32_t tag = ctrl->frame + 1u;
@@ -861,13 +948,24 @@ __device__ __forceinline__ bool cube_covers(uint32_t lpk, uint32_t apk, uint32_t
 // Five waves per SIMD (at most 96 VGPRs): with k = 8 the compiler keeps one
 // value of the sample path in scratch memory, which costs less than the
 // occupancy of four waves (measured 267 us against 303 us per 512^3 frame).
-template <int K, bool KNN_ONLY, class S>
+//
+// REDO: the redo pass.  Bricks whose pending records fit neither the
+// workgroup's arena nor the shared overflow arena are appended to the redo
+// list (their records are not stored); once pass A has finished, and with it
+// the collision bitmap, this instantiation recomputes them and writes each
+// pending voxel whose target was reached once straight into the volume, as
+// k_pass_b would have.  A voxel's new value depends only on its own tsdf and
+// weight, which no other kernel of the frame writes, so the result is the
+// same whichever way a brick's records go.
+template <int K, bool KNN_ONLY, class S, bool REDO = false>
 __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_pass_a(
     const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
-    const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
+    const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ brick_off,
+    const uint32_t* __restrict__ active2, uint32_t* __restrict__ redo, Ctrl* __restrict__ ctrl,
     uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint64_t brick_cap, int parity_arg,
     Arena arena, unsigned long long* __restrict__ stat,
     unsigned long long* __restrict__ totals, KnnOut kout) {
+    static_assert(!(KNN_ONLY && REDO), "the knn-only frame has no records");
     constexpr int kPassAThreads = S::kThreads;
     constexpr int kBX = S::BX, kBY = S::BY, kBZ = S::BZ;
     constexpr int kWX = S::WX, kWY = S::WY, kWZ = S::WZ;
@@ -880,29 +978,29 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     __shared__ float2 s_w[kCap];
     __shared__ double s_exp2[32];
     __shared__ unsigned long long s_keys[K * kPassAThreads];
-    __shared__ uint32_t s_bm[KNN_ONLY ? 1 : kWinTiles * 32];
+    __shared__ uint32_t s_bm[KNN_ONLY || REDO ? 1 : kWinTiles * 32];
     __shared__ uint32_t s_wave_emit[kPassAThreads / 64];
-    __shared__ unsigned long long s_stat[4];
-    __shared__ uint32_t s_next, s_first;
+    __shared__ unsigned long long s_stat[5];
+    __shared__ uint32_t s_next, s_first, s_ovf;
 
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int wave = tid >> 6;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
-    const uint32_t* __restrict__ active = active2 + (uint64_t)parity * brick_cap;
-    if (tid < 4) s_stat[tid] = 0ull;
+    const uint32_t* __restrict__ active = REDO ? redo : active2 + (uint64_t)parity * brick_cap;
+    if (tid < 5) s_stat[tid] = 0ull;
     if (tid < 32) s_exp2[tid] = kExp2Table[tid];
-    if (!KNN_ONLY) {
+    if (!KNN_ONLY && !REDO) {
         for (int w = tid; w < kWinTiles * 32; w += kPassAThreads) s_bm[w] = 0u;
     }
-    if (blockIdx.x == 0 && tid == 0) {
+    if (!REDO && blockIdx.x == 0 && tid == 0) {
         // the other parity's list and counters belong to the previous frame,
         // whose kernels have finished
         ctrl->active_count[parity ^ 1u] = 0u;
         ctrl->cand_count[parity ^ 1u] = 0u;
         for (uint32_t q = 0; q < kQueues; ++q) ctrl->queue[parity ^ 1u][q * kQueueStride] = 0u;
     }
-    const uint32_t nactive = ctrl->active_count[parity];
+    const uint32_t nactive = REDO ? ctrl->redo_count : ctrl->active_count[parity];
 
     // Each wave takes a 4x4x4 block of the brick: neighbour counts vary
     // less inside a compact block than along a 64-voxel slab, and a wave's
@@ -927,14 +1025,20 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     // one brick ahead and read a brick later.  The brick after the current
     // one is fetched while the current one computes: its entry, its list
     // count and, for this thread's list slot, the node id and the node record.
+    // The redo pass takes its few bricks round-robin.
     const uint32_t qx = blockIdx.x % kQueues;
     uint32_t* const qctr = &ctrl->queue[parity][qx * kQueueStride];
     auto queue_pos = [qx](uint32_t j) -> uint32_t {
         return ((j / kQueueChunk) * kQueues + qx) * kQueueChunk + j % kQueueChunk;
     };
     if (tid == 0) {
-        s_first = queue_pos(atomicAdd(qctr, 1u));
-        s_next = queue_pos(atomicAdd(qctr, 1u));
+        if (REDO) {
+            s_first = blockIdx.x;
+            s_next = blockIdx.x + gridDim.x;
+        } else {
+            s_first = queue_pos(atomicAdd(qctr, 1u));
+            s_next = queue_pos(atomicAdd(qctr, 1u));
+        }
     }
     __syncthreads();
     uint32_t a = s_first;
@@ -947,7 +1051,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         const uint32_t gb0 = v0.brick_base + (item & 0xffffffu);
         cnt = brick_count[gb0];
         if (cnt <= (uint32_t)kCap && (uint32_t)tid < cnt) {
-            pre_id = brick_nodes[(uint64_t)gb0 * kCap + tid];
+            pre_id = brick_nodes[brick_off[gb0] + tid];
             const float4* r = (const float4*)(nrec + v0.node_base + pre_id);
             pm = r[0];
             plo = r[1];
@@ -982,18 +1086,18 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         // voxels beyond the volume's last (x, y or z) are covered by no cube
         const uint32_t vidx = ((uint32_t)z * (uint32_t)dimy + (uint32_t)y) * (uint32_t)dimx + (uint32_t)x;
 
-        // next brick: entry, count and list ids (all kCap slots, in bounds);
-        // the grab of the one after it
+        // next brick: entry, count and this thread's list id; the grab of the
+        // one after it
         const uint32_t item_n = a_n < nactive ? active[a_n] : kNone;
         uint32_t a_nn = 0;
-        if (tid == 0 && item_n != kNone) a_nn = queue_pos(atomicAdd(qctr, 1u));
+        if (tid == 0 && item_n != kNone) a_nn = REDO ? a_n + gridDim.x : queue_pos(atomicAdd(qctr, 1u));
         uint32_t cnt_n = 0, id_n = 0, nbase_n = 0;
         if (item_n != kNone) {
             const VolDesc& vn = vols[item_n >> 24];
             const uint32_t gbn = vn.brick_base + (item_n & 0xffffffu);
             nbase_n = vn.node_base;
             cnt_n = brick_count[gbn];
-            if (tid < kCap) id_n = brick_nodes[(uint64_t)gbn * kCap + tid];
+            if (cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) id_n = brick_nodes[brick_off[gbn] + tid];
         }
 
         float p[3];
@@ -1065,9 +1169,11 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 }
             }
         }
+        // the list count is cleared for the next frame; pass A clears it
+        // after the record step, as the brick may go to the redo pass
         if (tid == 0) {
-            brick_count[gb] = 0u;
-            if (spill) s_stat[3] += 1ull;
+            if (KNN_ONLY || REDO) brick_count[gb] = 0u;
+            if (spill && !REDO) s_stat[3] += 1ull;
         }
 
         const int n = ks.n;
@@ -1220,7 +1326,9 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 ucode = ((vd.tile_base + tloc) << 9) | bit;
                 const uint32_t mask = 1u << (bit & 31u);
                 const uint32_t lx_t = (uint32_t)(tx - wx0), ly_t = (uint32_t)(ty - wy0), lz_t = (uint32_t)(tz - wz0);
-                if (lx_t < (uint32_t)kWX && ly_t < (uint32_t)kWY && lz_t < (uint32_t)kWZ) {
+                if (REDO) {
+                    // pass A has counted this brick's targets
+                } else if (lx_t < (uint32_t)kWX && ly_t < (uint32_t)kWY && lz_t < (uint32_t)kWZ) {
                     uint32_t* blk = s_bm + ((lz_t * kWY + ly_t) * kWX + lx_t) * 32u;
                     const uint32_t old = atomicOr(blk + (bit >> 5), mask);
                     if (old & mask) atomicOr(blk + 16 + (bit >> 5), mask);
@@ -1243,6 +1351,26 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         // next brick's node record for this thread's list slot, loaded once
         // this brick's knn state is no longer live
         VF_FETCH_NEXT_RECORD();
+        if (REDO) {
+            // k_pass_b's step for this voxel: write it unless its target
+            // was reached twice
+            bool coll = false, upd = false;
+            if (emit) {
+                const uint32_t word = bitmap[(uint64_t)(ucode >> 9) * 32u + 16u + ((ucode & 511u) >> 5)];
+                coll = ((word >> (ucode & 31u)) & 1u) != 0u;
+                upd = !coll;
+                if (upd) {
+                    vd.tsdf[vidx] = tnew;
+                    vd.wt[vidx] = wnew;
+                }
+            }
+            const unsigned long long mc = __ballot(coll);
+            const unsigned long long mu = __ballot(upd);
+            if (lane == 0) {
+                if (mc) atomicAdd(&s_stat[0], (unsigned long long)__popcll(mc));
+                if (mu) atomicAdd(&s_stat[1], (unsigned long long)__popcll(mu));
+            }
+        } else {
         const unsigned long long msup = __ballot(n > 0);
         const unsigned long long minb = __ballot(inb);
         const unsigned long long memit = __ballot(emit);
@@ -1253,7 +1381,10 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
             if (memit) atomicAdd(&s_stat[2], (unsigned long long)__popcll(memit));
         }
         __syncthreads();
-        // pending records into this workgroup's arena (in lane order)
+        // The brick's pending records (in lane order), all in one place: the
+        // workgroup's arena if they fit, else the shared overflow arena if
+        // they fit there, else none, and the brick goes to the redo list
+        // (its reserved overflow slots get null records).
         {
             uint32_t before = 0, total = 0;
 #pragma unroll
@@ -1262,19 +1393,35 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 before += wv < wave ? c : 0u;
                 total += c;
             }
-            if (emit) {
-                const uint32_t rank = before + (uint32_t)__popcll(memit & ((1ull << lane) - 1ull));
-                const uint4 rec = make_uint4(vd.voxel_base + (uint32_t)vidx, ucode, __float_as_uint(tnew),
-                                             __float_as_uint(wnew));
-                const uint32_t pos = fill + rank;
-                if (pos < arena.cap) {
-                    arena.records[(uint64_t)blockIdx.x * arena.cap + pos] = rec;
+            bool to_redo = false;
+            if (total) {  // uniform, as are fill and the branches below
+                uint4* dst;
+                uint64_t room;  // slots of dst that may be written
+                if ((uint64_t)fill + total <= arena.cap) {
+                    dst = arena.records + (uint64_t)blockIdx.x * arena.cap + fill;
+                    room = total;
+                    fill += total;
                 } else {
-                    const uint32_t o = atomicAdd(&ctrl->overflow_records, 1u);
-                    if (o < arena.overflow_cap) arena.overflow[o] = rec;
+                    if (tid == 0) s_ovf = atomicAdd(&ctrl->overflow_records, total);
+                    __syncthreads();
+                    const uint32_t o = s_ovf;
+                    dst = arena.overflow + o;
+                    room = o < arena.overflow_cap ? arena.overflow_cap - o : 0u;
+                    to_redo = room < total;
+                    if (to_redo && tid == 0) {
+                        redo[atomicAdd(&ctrl->redo_count, 1u)] = item;
+                        s_stat[4] += 1ull;
+                    }
+                }
+                if (emit) {
+                    const uint32_t rank = before + (uint32_t)__popcll(memit & ((1ull << lane) - 1ull));
+                    const uint4 rec = to_redo ? make_uint4(kNone, 0u, 0u, 0u)
+                                              : make_uint4(vd.voxel_base + (uint32_t)vidx, ucode,
+                                                           __float_as_uint(tnew), __float_as_uint(wnew));
+                    if (rank < room) dst[rank] = rec;
                 }
             }
-            fill += total;
+            if (tid == 0 && !to_redo) brick_count[gb] = 0u;
         }
         // merge the window into the global bitmap, one word at a time
         for (int w = tid; w < kWinTiles * 16; w += kPassAThreads) {
@@ -1297,6 +1444,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
                 blk[16 + wi] = 0u;
             }
         }
+        }  // !REDO
         }  // !KNN_ONLY
 #undef VF_FETCH_NEXT_RECORD
         // every thread read s_next before this brick's first barrier
@@ -1315,12 +1463,16 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
         pw = pw_n;
     }
     __syncthreads();
-    if (!KNN_ONLY && tid == 0) {
-        arena.counts[blockIdx.x] = fill < arena.cap ? fill : arena.cap;
+    if (REDO && tid == 0) {
+        stat_add(stat, totals, kStCollided, s_stat[0]);
+        stat_add(stat, totals, kStUpdated, s_stat[1]);
+    } else if (!KNN_ONLY && tid == 0) {
+        arena.counts[blockIdx.x] = fill;
         stat_add(stat, totals, kStSupported, s_stat[0]);
         stat_add(stat, totals, kStInBounds, s_stat[1]);
         stat_add(stat, totals, kStPending, s_stat[2]);
         stat_add(stat, totals, kStSpilled, s_stat[3]);
+        stat_add(stat, totals, kStRedo, s_stat[4]);
         if (blockIdx.x == 0) {
             stat[kStActive * kStatSlots] = nactive;
             atomicAdd(totals + kStActive * kStatSlots, (unsigned long long)nactive);
@@ -1403,11 +1555,13 @@ __global__ __launch_bounds__(256) void k_pass_b(const VolPtr* __restrict__ vptr,
             }
         }
     }
-    // the shared overflow arena
+    // the shared overflow arena (null records: slots of bricks that went to
+    // the redo pass)
     uint32_t no = ctrl->overflow_records;
     if (no > arena.overflow_cap) no = arena.overflow_cap;
     for (uint32_t r = blockIdx.x * blockDim.x + tid; r < no; r += gridDim.x * blockDim.x) {
         const uint4 rec = arena.overflow[r];
+        if (rec.x == kNone) continue;
         if (apply_record(rec, record_word(rec, bitmap), s_vp, nvol))
             ++updated;
         else
@@ -1456,8 +1610,10 @@ struct Buffers {
     Ctrl* ctrl = nullptr;
     NodeRec* nrec = nullptr;
     uint32_t* brick_count = nullptr;
-    uint32_t* brick_nodes = nullptr;
-    uint32_t* active = nullptr;  // 2 x brick_cap (frame parity)
+    uint32_t* brick_off = nullptr;    // per brick: first entry of its node list in brick_nodes
+    uint32_t* brick_nodes = nullptr;  // node-list pool: item_cap entries
+    uint2* items = nullptr;           // per (node, brick) item: its list entry (brick, node << 7 | slot)
+    uint32_t* active = nullptr;  // 2 x brick_cap (frame parity); cand doubles as the redo list
     uint32_t* bitmap = nullptr;
     uint32_t* tile_flag = nullptr;
     uint4* records = nullptr;     // arena_slots x arena_cap, then the overflow arena
@@ -1473,7 +1629,8 @@ struct Buffers {
     uint32_t* tile_mark = nullptr;   // per 8^3 tile: tag of the last frame that may have a pending target there
     uint32_t* vol_mark = nullptr;    // per volume: tag of the last frame that keeps every brick
     float* dmax = nullptr;           // depth-tile maxima
-    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0, arena_slots = 0, pyr_cap = 0;
+    uint64_t node_cap = 0, brick_cap = 0, tile_cap = 0, record_cap = 0, vol_cap = 0, arena_slots = 0, pyr_cap = 0,
+             item_cap = 0;
     uint64_t bytes = 0;
 };
 
@@ -1503,6 +1660,9 @@ struct vf_workspace {
     std::vector<hipEvent_t> events;
     int timed_frames = 0;
     vf_tuning tuning = {0, 0, 0.0f, 0};
+    // pending-record budget in records per voxel (0: the worst case); with a
+    // budget the buffers grow to exactly what a frame needs
+    float record_ratio = 0.0f;
     void* scratch = nullptr;  // general scratch for other entry points (point path)
     uint64_t scratch_bytes = 0;
 };
@@ -1514,10 +1674,10 @@ constexpr int kMaxTimed = 4096;
 constexpr int kTimedEvents = 6;  // per timed frame
 
 void free_buffers(Buffers& b) {
-    void* ptrs[] = {b.ctrl,      b.nrec,         b.brick_count, b.brick_nodes, b.active,    b.bitmap,
-                    b.tile_flag, b.records,      b.arena_counts, b.stat,       b.totals,    b.vols,
-                    b.vptr,      b.brick_disp,   b.cand,        b.cand_info,   b.tile_mark, b.vol_mark,
-                    b.dmax};
+    void* ptrs[] = {b.ctrl,      b.nrec,         b.brick_count,  b.brick_nodes, b.active,    b.bitmap,
+                    b.tile_flag, b.records,      b.arena_counts, b.stat,        b.totals,    b.vols,
+                    b.vptr,      b.brick_disp,   b.cand,         b.cand_info,   b.tile_mark, b.vol_mark,
+                    b.dmax,      b.brick_off,    b.items};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = Buffers();
@@ -1541,7 +1701,7 @@ int dev_alloc(T** p, uint64_t count, uint64_t* bytes) {
 }
 
 struct Need {
-    uint64_t nodes = 0, bricks = 0, tiles = 0, records = 0, nvol = 0, arena_slots = 0, pyr = 0;
+    uint64_t nodes = 0, bricks = 0, tiles = 0, records = 0, nvol = 0, arena_slots = 0, pyr = 0, items = 0;
 };
 
 // (Re)allocates to at least the requested capacities; new buffers are zeroed.
@@ -1549,12 +1709,15 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     Buffers& b = ws->buf;
     if (b.ctrl && need.nodes <= b.node_cap && need.bricks <= b.brick_cap && need.tiles <= b.tile_cap &&
         need.records <= b.record_cap && need.nvol <= b.vol_cap && need.arena_slots <= b.arena_slots &&
-        need.pyr <= b.pyr_cap)
+        need.pyr <= b.pyr_cap && need.items <= b.item_cap)
         return VF_OK;
     if (b.ctrl) VF_HIP(hipStreamSynchronize(stream));
     // a capacity that has to grow gets half again as much headroom, so that
-    // batches of varying size settle after a few frames
-    const auto grow = [](uint64_t want, uint64_t have) {
+    // batches of varying size settle after a few frames; a workspace with a
+    // record budget grows to exactly what is asked (its owner bounds it)
+    const bool exact = ws->record_ratio > 0.0f;
+    const auto grow = [exact](uint64_t want, uint64_t have) {
+        if (exact) return std::max<uint64_t>(std::max(want, have), 1);
         return std::max<uint64_t>(want <= have ? have : std::max(want, have + have / 2), 1);
     };
     Buffers nbuf;
@@ -1565,6 +1728,7 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     nbuf.vol_cap = grow(need.nvol, b.vol_cap);
     nbuf.arena_slots = grow(need.arena_slots, b.arena_slots);
     nbuf.pyr_cap = grow(need.pyr, b.pyr_cap);
+    nbuf.item_cap = grow(need.items, b.item_cap);
     // the running totals survive the reallocation
     unsigned long long* totals = b.totals;
     b.totals = nullptr;
@@ -1575,7 +1739,9 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     if (!st) st = dev_alloc(&nbuf.ctrl, 1, &bytes);
     if (!st) st = dev_alloc(&nbuf.nrec, nbuf.node_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.brick_count, nbuf.brick_cap, &bytes);
-    if (!st) st = dev_alloc(&nbuf.brick_nodes, nbuf.brick_cap * kCap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.brick_off, nbuf.brick_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.brick_nodes, nbuf.item_cap, &bytes);
+    if (!st) st = dev_alloc(&nbuf.items, nbuf.item_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.active, nbuf.brick_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.bitmap, nbuf.tile_cap * 32, &bytes);
     if (!st) st = dev_alloc(&nbuf.tile_flag, nbuf.tile_cap, &bytes);
@@ -1711,6 +1877,7 @@ struct FrameSetup {
     uint64_t blocks_pyr = 0;    // k_prep_bin workgroups over depth blocks
     uint32_t k1 = 0;
     uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
+    bool redo = false;         // launch the redo pass (a record budget: some bricks may not fit)
     uint64_t tiles = 0;        // collision tiles of this batch
     uint64_t nodes = 0;        // motion-graph nodes of this batch
     uint64_t clear_tiles = 0;  // tiles whose flags prep clears: every tile a previous frame or a replay may flag
@@ -1770,6 +1937,44 @@ int check_tuning(const vf_tuning& t) {
     if (t.flags & ~VF_TUNE_NO_SCREEN) return fail(VF_ERR_INVALID_ARGUMENT, "tuning flags 0x%x: unknown bits", t.flags);
     return VF_OK;
 }
+
+}  // namespace
+
+// The groups follow setup_frame and ensure: Need.nodes (NodeRec), Need.bricks
+// (brick_count, brick_off, active x 2, brick_disp, cand, cand_info) with
+// Need.arena_slots (at most one pass-A workgroup per brick), Need.tiles
+// (bitmap, tile_flag, tile_mark), Need.records (the budget, at least one),
+// Need.items (items, brick_nodes), Need.nvol (descriptor sets, vol_mark) and
+// Need.pyr (the screen's depth tiles, whether or not the volume is screened).
+void volume_ws_bytes(vf_int3 dim, uint64_t nodes, vf_int3 cube, int width, int height, int brick_shape,
+                     float records_per_voxel, uint64_t out[kWsGroups]) {
+    const int32_t* bs = kShapeDims[brick_shape >= 0 && brick_shape < kNumShapes ? brick_shape : 0];
+    const int32_t d[3] = {std::max(dim.x, 0), std::max(dim.y, 0), std::max(dim.z, 0)};
+    const int32_t c[3] = {cube.x, cube.y, cube.z};
+    uint64_t n = 1, bricks = 1, tiles = 1, maxb = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int32_t bd = ceil_div(d[a], bs[a]);
+        n *= (uint64_t)d[a];
+        bricks *= (uint64_t)bd;
+        tiles *= (uint64_t)ceil_div(d[a], 8);
+        maxb *= (uint64_t)std::min(ceil_div(std::max(c[a], 1) - 1, bs[a]) + 1, bd);
+    }
+    const bool no_cube = c[0] < 1 || c[1] < 1 || c[2] < 1;
+    const uint64_t k1 = no_cube ? 0 : nodes * maxb;
+    const uint64_t w = (uint64_t)std::max(width, 0), h = (uint64_t)std::max(height, 0);
+    const uint64_t pyr = (w + kTile0 - 1) / kTile0 * ((h + kTile0 - 1) / kTile0) +
+                         (w + kTile1 - 1) / kTile1 * ((h + kTile1 - 1) / kTile1);
+    const uint64_t fixed = sizeof(Ctrl) + 2ull * kStCount * kStatSlots * 8 + sizeof(uint4) + sizeof(uint2) + 4;
+    out[0] = nodes * sizeof(NodeRec);
+    out[1] = bricks * (4 + 4 + 8 + 4 + 4 + 16 + 4);
+    out[2] = tiles * (32 * 4 + 4 + 4);
+    out[3] = (uint64_t)std::ceil((double)n * (double)records_per_voxel) * sizeof(uint4);
+    out[4] = k1 * (sizeof(uint2) + 4);
+    out[5] = (sizeof(VolDesc) + sizeof(VolPtr)) * kDescSlots + 4 + fixed;
+    out[6] = pyr * sizeof(float);
+}
+
+namespace {
 
 // Builds the descriptors of a batch, sizes the launches, grows the buffers and
 // uploads the descriptors on `stream`.
@@ -1847,12 +2052,25 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     const int wg = tuning.wg_per_cu > 0 ? std::min(tuning.wg_per_cu, per_cu) : per_cu;
     const uint64_t blocks_a = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * wg, need.bricks));
     const int threads_a = kShapeDims[shape][0] * kShapeDims[shape][1] * kShapeDims[shape][2];
-    // records: one arena per pass-A workgroup sized for an even share, plus a
-    // shared overflow arena holding the whole bound
+    // records: one arena per pass-A workgroup, plus a shared overflow arena.
+    // Without a budget the arenas take an even share of the bound and the
+    // overflow arena the whole bound, so every brick's records fit.  With a
+    // budget of R records the arenas hold R / 2 between them and the overflow
+    // arena the rest; the bricks that fit neither go to the redo pass.
     uint64_t arena_cap = 0, overflow_cap = 0;
+    bool redo = false;
     if (!knn_only) {
-        arena_cap = (rec_bound + blocks_a - 1) / blocks_a + (uint64_t)threads_a;
-        overflow_cap = rec_bound;
+        if (ws->record_ratio > 0.0f) {
+            const uint64_t budget =
+                std::max<uint64_t>(1, (uint64_t)std::ceil((double)voxels * (double)ws->record_ratio));
+            const uint64_t r = std::min(rec_bound, budget);
+            arena_cap = r / (2 * blocks_a);
+            overflow_cap = std::max<uint64_t>(r - blocks_a * arena_cap, 1);
+            redo = true;
+        } else {
+            arena_cap = (rec_bound + blocks_a - 1) / blocks_a + (uint64_t)threads_a;
+            overflow_cap = rec_bound;
+        }
         if (arena_cap > 0xffffffffull || overflow_cap > 0xffffffffull)
             return fail(VF_ERR_CAPACITY, "record bound %llu exceeds 2^32", (unsigned long long)rec_bound);
         need.records = blocks_a * arena_cap + overflow_cap;
@@ -1861,6 +2079,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
         need.records = 1;
         need.arena_slots = 1;
     }
+    need.items = std::max<uint64_t>(k1, 1);
     st = ensure(ws, need, stream);
     if (st) return st;
     Buffers& B = ws->buf;
@@ -1906,6 +2125,7 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
     fs->arena_cap = arena_cap;
     fs->overflow_cap = overflow_cap;
+    fs->redo = redo;
     return VF_OK;
 }
 
@@ -1917,26 +2137,30 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
         for (int i = 0; i < 6; ++i) e[i] = (fs.timing == 2 && (i < 2 || i > 3)) ? nullptr : ev[i];
     const VolDesc* const vols = fs.vols;
     launch(k_prep_bin<S>, dim3((unsigned)fs.blocks_prep), dim3(kPrepThreads), stream, e[0], nullptr,
-           vols, fs.nvol, fs.k1, (uint32_t)fs.blocks_pair, B.nrec, B.brick_count, B.brick_nodes, B.brick_disp,
+           vols, fs.nvol, fs.k1, (uint32_t)fs.blocks_pair, B.nrec, B.brick_count, B.items, B.brick_disp,
            B.dmax, B.ctrl, B.bitmap, B.tile_flag, (uint32_t)fs.clear_tiles, B.stat);
     VF_LAUNCH_CHECK("k_prep_bin");
     const dim3 grid_c((unsigned)((fs.bricks + kPrepThreads * kCompactPer - 1) / (kPrepThreads * kCompactPer)));
     if (fs.screen) {
         launch(k_compact<S, true>, grid_c, dim3(kPrepThreads), stream, nullptr, nullptr, vols, fs.nvol,
-               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, (const uint32_t*)B.brick_disp,
+               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.brick_off, (const uint32_t*)B.brick_disp,
                (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
                parity_arg);
         VF_LAUNCH_CHECK("k_compact");
         launch(k_select, grid_c, dim3(kPrepThreads), stream, nullptr, e[1], vols, (const uint32_t*)B.cand,
                (const uint4*)B.cand_info, B.brick_count, B.brick_disp, (const uint32_t*)B.tile_mark,
-               (const uint32_t*)B.vol_mark, B.active, B.ctrl, B.brick_cap, parity_arg, B.stat, B.totals);
+               (const uint32_t*)B.vol_mark, B.active, B.ctrl, B.brick_cap, parity_arg, (const uint2*)B.items,
+               fs.k1, (const uint32_t*)B.brick_off, B.brick_nodes, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_select");
     } else {
-        launch(k_compact<S, false>, grid_c, dim3(kPrepThreads), stream, nullptr, e[1], vols, fs.nvol,
-               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, (const uint32_t*)B.brick_disp,
+        launch(k_compact<S, false>, grid_c, dim3(kPrepThreads), stream, nullptr, nullptr, vols, fs.nvol,
+               (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.brick_off, (const uint32_t*)B.brick_disp,
                (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
                parity_arg);
         VF_LAUNCH_CHECK("k_compact");
+        launch(k_fill_lists, dim3((unsigned)fs.blocks_pair), dim3(kPrepThreads), stream, nullptr, e[1],
+               (const uint2*)B.items, fs.k1, (const uint32_t*)B.brick_off, B.brick_nodes);
+        VF_LAUNCH_CHECK("k_fill_lists");
     }
     Arena arena;
     arena.records = B.records;
@@ -1945,17 +2169,23 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     arena.n_arenas = (uint32_t)fs.blocks_a;
     arena.overflow = B.records + fs.blocks_a * fs.arena_cap;
     arena.overflow_cap = (uint32_t)fs.overflow_cap;
-#define VF_PASS_A(KK)                                                                                                 \
-    if (fs.knn_only)                                                                                                  \
-        launch(k_pass_a<KK, true, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],             \
-               vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes,          \
-               (const uint32_t*)B.active, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat,      \
-               B.totals, kout);                                                                                       \
-    else                                                                                                              \
-        launch(k_pass_a<KK, false, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],            \
-               vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes,          \
-               (const uint32_t*)B.active, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat,      \
-               B.totals, kout)
+#define VF_PASS_A_ARGS                                                                                    \
+    vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes, (const uint32_t*)B.brick_off, \
+        (const uint32_t*)B.active, B.cand, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat, \
+        B.totals, kout
+#define VF_PASS_A(KK)                                                                                       \
+    if (fs.knn_only) {                                                                                      \
+        launch(k_pass_a<KK, true, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],   \
+               VF_PASS_A_ARGS);                                                                             \
+    } else {                                                                                                \
+        launch(k_pass_a<KK, false, S>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream, e[2], e[3],  \
+               VF_PASS_A_ARGS);                                                                             \
+        if (fs.redo) {                                                                                      \
+            VF_LAUNCH_CHECK("k_pass_a");                                                                    \
+            launch(k_pass_a<KK, false, S, true>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream,    \
+                   nullptr, nullptr, VF_PASS_A_ARGS);                                                       \
+        }                                                                                                   \
+    }
     switch (fs.k) {
         case 1: VF_PASS_A(1); break;
         case 2: VF_PASS_A(2); break;
@@ -1967,7 +2197,8 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
         case 8: VF_PASS_A(8); break;
     }
 #undef VF_PASS_A
-    VF_LAUNCH_CHECK("k_pass_a");
+#undef VF_PASS_A_ARGS
+    VF_LAUNCH_CHECK(fs.redo ? "k_pass_a (redo)" : "k_pass_a");
     if (!fs.knn_only) {
         launch(k_pass_b, dim3((unsigned)fs.blocks_b), dim3(256), stream, e[4], e[5], fs.vptr, fs.nvol,
                (uint32_t)fs.nodes, arena, B.ctrl,
@@ -2042,6 +2273,7 @@ int read_stats(vf_workspace* ws, bool totals, vf_frame_stats* out, hipStream_t s
     out->active_bricks = s[kStActive];
     out->overflow_bricks = s[kStSpilled];
     out->screened_bricks = s[kStScreened];
+    out->redo_bricks = s[kStRedo];
     return VF_OK;
 }
 
@@ -2361,6 +2593,15 @@ int vf_workspace_set_tuning(vf_workspace* ws, const vf_tuning* tuning) {
     int st = check_tuning(*tuning);
     if (st) return st;
     ws->tuning = *tuning;
+    return VF_OK;
+}
+
+int vf_workspace_set_record_budget(vf_workspace* ws, float records_per_voxel) {
+    if (!ws) return fail(VF_ERR_INVALID_ARGUMENT, "vf_workspace_set_record_budget: NULL workspace");
+    if (!(records_per_voxel >= 0.0f && records_per_voxel <= 1.0f))
+        return fail(VF_ERR_INVALID_ARGUMENT, "record budget %g per voxel: supported 0 (the worst case) .. 1",
+                    (double)records_per_voxel);
+    ws->record_ratio = records_per_voxel;
     return VF_OK;
 }
 

@@ -63,6 +63,17 @@ inline int check_k(int k) {
 // Device scratch owned by a workspace (defined with the workspace).
 int workspace_scratch(vf_workspace* ws, uint64_t bytes, void** ptr, hipStream_t stream);
 
+// What one volume adds, in bytes, to each buffer group of a workspace with a
+// record budget of records_per_voxel when a batch holds it: node records,
+// per-brick state, collision tiles, pending records, node-list entries,
+// descriptors (with the workspace's fixed part) and depth-tile maxima.  A
+// batch needs at most the sum over its volumes in each group, and such a
+// workspace grows to exactly what its batches need, so no group of it
+// exceeds the largest such sum (defined with the workspace).
+constexpr int kWsGroups = 7;
+void volume_ws_bytes(vf_int3 dim, uint64_t nodes, vf_int3 cube, int width, int height, int brick_shape,
+                     float records_per_voxel, uint64_t out[kWsGroups]);
+
 inline Camera make_camera(const vf_mat3& K, int W, int H, float delta, float maxw) {
     Camera c;
     c.fx = K.m[0];

@@ -19,6 +19,7 @@
 // a kernel reads them; batches on different lanes touch disjoint volumes and
 // may run concurrently.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -69,7 +70,7 @@ struct StreamState {
     vf_mat4* T = nullptr;
     float* depth = nullptr;
     uint64_t own = 0;  // its buffers
-    uint64_t ws = 0;   // what it adds to a workspace whose batch holds it
+    std::array<uint64_t, kWsGroups> ws{};  // what it adds to each buffer group of a lane workspace
     std::deque<Pending> queue;
     bool busy = false;  // a frame in flight
 };
@@ -118,7 +119,7 @@ struct vf_server {
     uint64_t submitted = 0, completed = 0, batches = 0;
     uint64_t bytes_in_use = 0;            // the admission charge of the open (and opening) streams
     uint64_t own_total = 0;               // their buffers
-    std::multiset<uint64_t> ws_open;      // their workspace shares
+    std::array<std::multiset<uint64_t>, kWsGroups> ws_open;  // their workspace shares, by buffer group
     int max_batch_seen = 0;
     int inflight = 0;
     bool stop = false;
@@ -308,34 +309,50 @@ uint64_t stream_own_bytes(const vf_stream_desc& d) {
     return 8 * n + (uint64_t)d.nodes * (12 + 4 + 64) + (uint64_t)std::max(d.width, 0) * std::max(d.height, 0) * 4;
 }
 
-// What a stream adds to the workspace of a batch that holds it (workspace
-// needs add up over a batch's volumes): brick lists (<= 4 B x 128 per 256
-// voxels), collision bitmap (1/4 B per voxel), node records (96 B), pending
-// records and the overflow arena (2 x 16 B per possible update), plus the
-// arenas' per-workgroup slack (<= 1024 workgroups x 512 records x 16 B); and
-// half again for the workspace's geometric growth.
+// The lanes' pending-record budget, in records per voxel.  On the first
+// frames of the 32-stream workload (129^3..491^3 volumes) 1.5 % to 3.2 % of
+// the voxels had a pending update; bricks beyond the budget are recomputed
+// by the redo pass, so the figure sets speed and memory, not results.
+constexpr float kServeRecordsPerVoxel = 1.0f / 24.0f;
+
+// What a stream adds to each buffer group of a lane workspace whose batch
+// holds it (lane workspaces use the default brick shape).
+std::array<uint64_t, kWsGroups> stream_ws_groups(const vf_stream_desc& d) {
+    std::array<uint64_t, kWsGroups> g{};
+    volume_ws_bytes(d.dim, d.nodes, d.voxels_per_node, d.width, d.height, 0, kServeRecordsPerVoxel, g.data());
+    return g;
+}
+
 uint64_t stream_ws_bytes(const vf_stream_desc& d) {
-    const uint64_t n = voxel_count(d.dim);
-    const uint64_t g = d.nodes;
-    const uint64_t cube = (uint64_t)std::max(d.voxels_per_node.x, 0) * std::max(d.voxels_per_node.y, 0) *
-                          std::max(d.voxels_per_node.z, 0);
-    const uint64_t records = std::min<uint64_t>(n, g * cube);
-    const uint64_t ws = 2 * n + n / 4 + g * 96 + 32 * records + (16u << 20);
-    return ws + ws / 2;
+    uint64_t t = 0;
+    for (uint64_t x : stream_ws_groups(d)) t += x;
+    return t;
 }
 
 uint64_t stream_bytes(const vf_stream_desc& d, int lanes) {
     return stream_own_bytes(d) + (uint64_t)std::max(lanes, 1) * stream_ws_bytes(d);
 }
 
-// The server's charge: every open stream's buffers, and `lanes` workspaces
-// each large enough for the batch of the max_batch largest workspace shares
-// (no batch holds more).  Caller holds sv->mu.
-uint64_t server_charge(const vf_server* sv, uint64_t own_total, const std::multiset<uint64_t>& ws) {
+// The server's charge: every open stream's buffers, and `lanes` workspaces.
+// A lane workspace grows to exactly what its batches need, and a batch holds
+// at most max_batch streams, so each of its buffer groups stays within the
+// sum of the max_batch largest shares of that group.  Caller holds sv->mu.
+uint64_t server_charge(const vf_server* sv, uint64_t own_total,
+                       const std::array<std::multiset<uint64_t>, kWsGroups>& ws) {
     uint64_t top = 0;
-    int k = 0;
-    for (auto it = ws.rbegin(); it != ws.rend() && k < sv->cfg.max_batch; ++it, ++k) top += *it;
+    for (const auto& grp : ws) {
+        int k = 0;
+        for (auto it = grp.rbegin(); it != grp.rend() && k < sv->cfg.max_batch; ++it, ++k) top += *it;
+    }
     return own_total + (uint64_t)std::max(sv->cfg.lanes, 1) * top;
+}
+
+void ws_insert(std::array<std::multiset<uint64_t>, kWsGroups>& ws, const std::array<uint64_t, kWsGroups>& g) {
+    for (int i = 0; i < kWsGroups; ++i) ws[i].insert(g[i]);
+}
+
+void ws_erase(std::array<std::multiset<uint64_t>, kWsGroups>& ws, const std::array<uint64_t, kWsGroups>& g) {
+    for (int i = 0; i < kWsGroups; ++i) ws[i].erase(ws[i].find(g[i]));
 }
 
 }  // namespace
@@ -370,6 +387,7 @@ int vf_server_create(vf_server** out, const vf_server_config* cfg) {
             return fail(VF_ERR_HIP, "server lane setup: %s", hipGetErrorString(e));
         }
         st = vf_workspace_create(&l.ws, cfg->max_batch, 0, 0, l.stream);
+        if (!st) st = vf_workspace_set_record_budget(l.ws, kServeRecordsPerVoxel);
         if (st) {
             for (Lane& m : sv->lanes) {
                 if (m.done) (void)hipEventDestroy(m.done);
@@ -425,13 +443,14 @@ int vf_server_open(vf_server* sv, const vf_stream_desc* d, int wait_ms, int32_t*
     if (d->width < 0 || d->height < 0) return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_open: negative frame size");
     if (d->nodes > 0 && (!d->points || !d->radii))
         return fail(VF_ERR_INVALID_ARGUMENT, "vf_server_open: node positions and radii must not be NULL");
-    const uint64_t own = stream_own_bytes(*d), wsb = stream_ws_bytes(*d);
+    const uint64_t own = stream_own_bytes(*d);
+    const std::array<uint64_t, kWsGroups> wsb = stream_ws_groups(*d);
     {
         std::unique_lock<std::mutex> lk(sv->mu);
         uint64_t charge = 0;
         const auto fits = [&] {
-            std::multiset<uint64_t> with = sv->ws_open;
-            with.insert(wsb);
+            auto with = sv->ws_open;
+            ws_insert(with, wsb);
             charge = server_charge(sv, sv->own_total + own, with);
             return charge <= sv->cfg.hbm_budget;
         };
@@ -444,13 +463,13 @@ int vf_server_open(vf_server* sv, const vf_stream_desc* d, int wait_ms, int32_t*
         }
         // reserved while allocating
         sv->own_total += own;
-        sv->ws_open.insert(wsb);
+        ws_insert(sv->ws_open, wsb);
         sv->bytes_in_use = charge;
     }
     const auto unreserve = [&] {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->own_total -= own;
-        sv->ws_open.erase(sv->ws_open.find(wsb));
+        ws_erase(sv->ws_open, wsb);
         sv->bytes_in_use = server_charge(sv, sv->own_total, sv->ws_open);
         sv->cv_done.notify_all();
     };
@@ -512,9 +531,10 @@ int vf_server_close(vf_server* sv, int32_t sid) {
     (void)hipSetDevice(sv->cfg.device);
     free_stream(s);
     sv->own_total -= s.own;
-    sv->ws_open.erase(sv->ws_open.find(s.ws));
+    ws_erase(sv->ws_open, s.ws);
     sv->bytes_in_use = server_charge(sv, sv->own_total, sv->ws_open);
-    s.own = s.ws = 0;
+    s.own = 0;
+    s.ws = {};
     sv->cv_done.notify_all();
     return VF_OK;
 }
@@ -703,6 +723,7 @@ int vf_server_get_stats(vf_server* sv, vf_server_stats* out) {
         out->totals.updated += t.updated;
         out->totals.active_bricks += t.active_bricks;
         out->totals.overflow_bricks += t.overflow_bricks;
+        out->totals.redo_bricks += t.redo_bricks;
         out->totals.screened_bricks += t.screened_bricks;
     }
     return VF_OK;

@@ -385,6 +385,12 @@ class Workspace:
         t = L.Tuning(int(brick_shape), int(wg_per_cu), 0.0, 0 if screen else L.TUNE_NO_SCREEN)
         L.call("vf_workspace_set_tuning", self.handle, C.byref(t))
 
+    def set_record_budget(self, records_per_voxel: float) -> None:
+        """Room for ceil(records_per_voxel x voxels) pending updates per frame
+        (0: the worst case); bricks beyond it are recomputed by a further
+        pass with the same results."""
+        L.call("vf_workspace_set_record_budget", self.handle, float(records_per_voxel))
+
     def close(self) -> None:
         if self.handle:
             L.load().vf_workspace_destroy(self.handle)
