@@ -35,6 +35,24 @@ def test_sphere(vf, oracle, n, full):
         np.testing.assert_array_equal(f, fo)
 
 
+@pytest.mark.parametrize("full", [False, True])
+def test_special_values_and_near_iso_corners(vf, oracle, full):
+    """NaN, +-inf and near-iso values (within 1e-7, and tiny next to the
+    other corners) on a sphere: the count sweep's screen must send these
+    cubes to the exact count, which the emit sweep must agree with."""
+    tsdf, wt, dims, vs, size = sphere(48, 0.8 / 48, 0.27)
+    rng = np.random.default_rng(7)
+    idx = rng.choice(tsdf.size, 3000, replace=False)
+    vals = np.array([np.nan, np.inf, -np.inf, 5e-8, -5e-8, 1e-6, -3e-7, 0.0], F)
+    tsdf[idx] = vals[rng.integers(0, vals.size, idx.size)]
+    v, f = ops.marching_cubes(ops.Workspace(), dev(tsdf), dev(wt), dims, vs, size, full=full)
+    vo, fo = oracle.marching_cubes(tsdf, wt, dims, vs, size, full=full)
+    assert v.shape[0] > 100
+    np.testing.assert_array_equal(v, vo)
+    if full:
+        np.testing.assert_array_equal(f, fo)
+
+
 @pytest.mark.parametrize("iso", [0.0, 0.5])
 def test_degenerate_triangles(vf, oracle, iso):
     """TSDF values on a coarse grid with the iso value among them: edge points

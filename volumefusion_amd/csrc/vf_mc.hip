@@ -6,7 +6,9 @@
 // the output is compacted in a canonical order (cubes by linear index, edges
 // by number, triangles in table order) with two sweeps:
 //   k_mc_count   per cube: case and output count; per run of 63 cubes of a
-//                row: sum
+//                row: sum.  With faces, a cube whose triangles may be
+//                degenerate flags its run for the next kernel.
+//   k_mc_exact   (faces only) the flagged runs' exact sums
 //   (scan)       exclusive scan of the run sums (rocPRIM)
 //   k_mc_emit    per cube again, only in runs with output: wave-local
 //                prefix, then the writes
@@ -64,6 +66,10 @@ struct McGeom {
     uint64_t ncubes;
     int32_t cz;  // cube layers (Z - 2)
     int32_t nr;  // runs of kRun cubes per row
+    // the count sweep's snap screen over the whole volume: lmin <= every
+    // cube's edge length L and pscale >= 2^-19 times every cube's largest
+    // corner coordinate P, with margins of 2^-30 (see screen_ok)
+    double lmin, pscale;
 };
 
 // interpolate, src/MarchingCubes.cu:294-312: ratio in float, widened,
@@ -190,6 +196,16 @@ __device__ __forceinline__ bool no_snap(const McGeom& g, const Cube& cb) {
     return (double)m >= 0x1p-16 * (double)M && (double)m * L >= 0x1p-19 * P * (double)M;
 }
 
+// no_snap over the whole volume, for the count sweep: with m and M the
+// smallest and largest |t - iso| over a cube's corners, m >= 2e-7,
+// m >= 2^-16 M and m lmin >= pscale M imply no_snap's m L >= 2^-19 P M for
+// every cube of the volume (lmin and pscale carry 2^-30 margins, far above
+// the binary64 rounding of the two products).  Only m and M of the cube are
+// needed, not its coordinates.
+__device__ __forceinline__ bool screen_ok(const McGeom& g, float m, float M) {
+    return m >= 2.0e-7f && (double)m >= 0x1p-16 * (double)M && (double)m * g.lmin >= g.pscale * (double)M;
+}
+
 // Output vertex count of a cube: 3 per triangle of its case (FULL, dropping
 // triangles with two equal points, src/MarchingCubes.cu:400-410) or 1 per
 // crossed edge.  With FULL, bit q of keep marks the kept triangles.
@@ -311,10 +327,23 @@ struct RunPos {
 
 // A workgroup's four waves take four consecutive rows j of the same run and
 // z segment, so the voxel row two waves share (j + 1 of one, j of the next)
-// is fetched from HBM once and read again from the CU's caches.
+// is fetched from HBM once and read again from the CU's caches.  Workgroups
+// are dispatched to the eight XCDs round-robin, and each XCD has its own L2:
+// tile numbers b go to the XCDs in chunks of kMcChunk row groups (every run
+// of each), so neighbouring runs of a row (whose 252 B spans share a 128 B
+// line) and neighbouring row groups (which share a row) mostly meet in one
+// L2, while the chunks' round-robin keeps the surface work spread over the
+// XCDs.  Tiles beyond the last whole round of chunks keep their number.
+constexpr uint32_t kMcChunk = 8;
 __device__ __forceinline__ RunPos run_pos(const McGeom& g) {
     const uint32_t groups = ((uint32_t)g.cy + (kMcBlock / 64) - 1) / (kMcBlock / 64);
-    const uint32_t b = blockIdx.x;
+    const uint32_t chunk = kMcChunk * (uint32_t)g.nr;
+    const uint32_t whole = gridDim.x / (8u * chunk) * (8u * chunk);
+    uint32_t b = blockIdx.x;
+    if (b < whole) {
+        const uint32_t i = b >> 3;  // this workgroup's place among its XCD's
+        b = ((i / chunk) * 8u + (blockIdx.x & 7u)) * chunk + i % chunk;
+    }
     RunPos p;
     p.r = (int32_t)(b % (uint32_t)g.nr);
     p.j = (int32_t)(((b / (uint32_t)g.nr) % groups) * (kMcBlock / 64) + (threadIdx.x >> 6));
@@ -329,14 +358,14 @@ __device__ __forceinline__ uint32_t slice_bits(const Slice& s, float iso) {
     return (s.t0 > iso ? 1u : 0u) | (s.t1 > iso ? 2u : 0u) | ((s.w0 == 0.0f || s.w1 == 0.0f) ? 4u : 0u);
 }
 
-// FULL needs about 120 VGPRs for the exact degenerate-triangle test of the
-// cubes no_snap cannot clear.  With the screen that path is rare, and six
-// waves per SIMD measured best for the full 512^3 extraction: 0.572 ms,
-// against 0.638 ms at four waves (no spills) and 0.617 ms at eight (168 B of
-// scratch per lane).
+// With FULL the sweep counts 3 per triangle where screen_ok rules out
+// degenerate triangles and leaves the other runs to k_mc_exact: the exact
+// test's registers (about 120 VGPRs, which held the sweep at six waves per
+// SIMD with spills) stay out of the sweep.
 template <bool FULL>
-__global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_mc_count(
-    McGeom g, const float* __restrict__ tsdf, const float* __restrict__ wt, uint32_t* __restrict__ sums) {
+__global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __restrict__ tsdf,
+                                                        const float* __restrict__ wt, uint32_t* __restrict__ sums,
+                                                        uint32_t* __restrict__ exact_list) {
     __shared__ uint32_t s_lut[256];
     static_assert(kMcBlock == 256, "one lookup entry per thread");
     s_lut[threadIdx.x] = kMcCaseLut.v[threadIdx.x];
@@ -368,12 +397,21 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(6))) v
         if (!FULL) {
             n = own && !zero ? (e >> 8) & 255u : 0u;
         } else {
+            // 3 per triangle where the screen rules out snapped edge points
+            // (then no triangle is degenerate); else the run is flagged
+            // and k_mc_exact counts it
+            const float d0 = fabsf(lo.t0 - g.iso), d1 = fabsf(lo.t1 - g.iso);
+            const float d2 = fabsf(hi.t0 - g.iso), d3 = fabsf(hi.t1 - g.iso);
+            const bool bad = !(d0 <= 3.0e38f) || !(d1 <= 3.0e38f) || !(d2 <= 3.0e38f) || !(d3 <= 3.0e38f);
+            const float mo = fminf(fminf(d0, d1), fminf(d2, d3)), Mo = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
+            const float m = fminf(mo, from_next_lane(mo)), M = fmaxf(Mo, from_next_lane(Mo));
+            const bool bad2 = bad || __builtin_amdgcn_update_dpp(0, (int)bad, 0x130, 0xf, 0xf, false) != 0;
             const bool surface = own && cs != 0u && cs != 255u;
-            if (__ballot(surface)) {
-                Cube cb;
-                make_cube(g, x, rp.j, k, lo, hi, &cb);
-                unsigned keep;
-                if (surface) n = (uint32_t)cube_outputs<true>(g, cb, keep);
+            const bool exact = surface && (bad2 || !screen_ok(g, m, M));
+            n = surface && !exact ? 3u * ((e >> 16) & 255u) : 0u;
+            if (__ballot(exact) && lane == 0) {
+                const uint64_t q = ((uint64_t)k * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
+                exact_list[1u + atomicAdd(exact_list, 1u)] = (uint32_t)q;
             }
         }
         // two steps per wave reduction: a run's count is below 2^16
@@ -398,6 +436,33 @@ __global__ __launch_bounds__(kMcBlock) __attribute__((amdgpu_waves_per_eu(6))) v
         n1 = n2;
         n2 = n3;
         blo = bhi;
+    }
+}
+
+// The exact sums of the runs k_mc_count flagged (list: count, then run
+// numbers q), a wave per run.
+__global__ __launch_bounds__(kMcBlock) void k_mc_exact(McGeom g, const float* __restrict__ tsdf,
+                                                        const float* __restrict__ wt, uint32_t* __restrict__ sums,
+                                                        const uint32_t* __restrict__ exact_list) {
+    const uint32_t n = exact_list[0];
+    const int lane = __lane_id();
+    const uint32_t waves = gridDim.x * (kMcBlock / 64);
+    for (uint32_t w = blockIdx.x * (kMcBlock / 64) + (threadIdx.x >> 6); w < n; w += waves) {
+        const uint32_t q = exact_list[1u + w];
+        const int32_t r = (int32_t)(q % (uint32_t)g.nr);
+        const uint32_t jk = q / (uint32_t)g.nr;
+        const int32_t j = (int32_t)(jk % (uint32_t)g.cy);
+        const int32_t k = (int32_t)(jk / (uint32_t)g.cy);
+        const int32_t x = r * kRun + lane;
+        const bool own = lane < kRun && x < g.cx;
+        const Slice lo = load_slice(g, tsdf, wt, x, j, k);
+        const Slice hi = load_slice(g, tsdf, wt, x, j, k + 1);
+        Cube cb;
+        make_cube(g, x, j, k, lo, hi, &cb);
+        unsigned keep;
+        const uint32_t c = own ? (uint32_t)cube_outputs<true>(g, cb, keep) : 0u;
+        const uint32_t total = wave_sum(c);
+        if (lane == 0) sums[q] = total;
     }
 }
 
@@ -530,6 +595,24 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     g.ncubes = (uint64_t)g.cx * (uint64_t)g.cy * (uint64_t)(dims.z - 2);
     g.cz = dims.z - 2;
     g.nr = (g.cx + kRun - 1) / kRun;
+    {
+        // the smallest edge length and the largest |coordinate| over the
+        // cubes' corners 0 .. c per axis, in the kernels' float arithmetic
+        double lmin = 3.0e38, pmax = 0.0;
+        const int32_t cmax[3] = {g.cx, g.cy, g.cz};
+        for (int a = 0; a < 3; ++a) {
+            float prev = (float)0 * g.vs[a] - g.halfs[a];
+            pmax = std::max(pmax, std::fabs((double)prev));
+            for (int32_t i = 1; i <= cmax[a]; ++i) {
+                const float cur = (float)i * g.vs[a] - g.halfs[a];
+                lmin = std::min(lmin, (double)(float)(cur - prev));
+                pmax = std::max(pmax, std::fabs((double)cur));
+                prev = cur;
+            }
+        }
+        g.lmin = lmin > 0.0 ? lmin * (1.0 - 0x1p-30) : 0.0;
+        g.pscale = 0x1p-19 * pmax * (1.0 + 0x1p-30);
+    }
     const uint64_t nq = (uint64_t)g.cz * (uint64_t)g.cy * (uint64_t)g.nr;  // runs
     const uint64_t groups = ((uint64_t)g.cy + kMcBlock / 64 - 1) / (kMcBlock / 64);
     const uint64_t nb64 = (uint64_t)g.nr * groups * (uint64_t)((g.cz + kMarch - 1) / kMarch);
@@ -539,20 +622,31 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     size_t scan_bytes = 0;
     VF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nq,
                                             stream));
+    // scratch: run sums, their offsets, (faces) the flagged-run list (a
+    // count, then at most one entry per run), the scan's temporary storage
     const uint64_t sums_bytes = nq * 4;
-    const uint64_t scan_off = (2 * sums_bytes + 255) & ~255ull;
+    const uint64_t list_off = 2 * sums_bytes;
+    const uint64_t scan_off = (list_off + (full ? (nq + 1) * 4 : 0) + 255) & ~255ull;
     void* scratch = nullptr;
     st = workspace_scratch(ws, scan_off + scan_bytes, &scratch, stream);
     if (st) return st;
     uint32_t* sums = (uint32_t*)scratch;
     uint32_t* offsets = sums + nq;
+    uint32_t* exact_list = (uint32_t*)((char*)scratch + list_off);
     void* scan_tmp = (char*)scratch + scan_off;
     if (full) {
-        hipLaunchKernelGGL(k_mc_count<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums);
+        VF_HIP(hipMemsetAsync(exact_list, 0, sizeof(uint32_t), stream));
+        hipLaunchKernelGGL(k_mc_count<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums, exact_list);
+        VF_LAUNCH_CHECK("k_mc_count");
+        const uint32_t nb_exact = (uint32_t)std::min<uint64_t>(nb, 512);
+        hipLaunchKernelGGL(k_mc_exact, dim3(nb_exact), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                           (const uint32_t*)exact_list);
+        VF_LAUNCH_CHECK("k_mc_exact");
     } else {
-        hipLaunchKernelGGL(k_mc_count<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums);
+        hipLaunchKernelGGL(k_mc_count<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                           (uint32_t*)nullptr);
+        VF_LAUNCH_CHECK("k_mc_count");
     }
-    VF_LAUNCH_CHECK("k_mc_count");
     VF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, sums, offsets, (int)nq, stream));
     if (full) {
         hipLaunchKernelGGL(k_mc_emit<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums, offsets,
