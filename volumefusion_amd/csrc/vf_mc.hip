@@ -378,21 +378,31 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __
     const int32_t k1 = rp.k0 + kMarch < g.cz ? rp.k0 + kMarch : g.cz;
     // slices k .. k + 3 are loaded or in flight while step k computes; near
     // the segment's end the prefetch re-reads slice k1 (the last one used)
-    Slice lo = load_slice(g, tsdf, wt, x, rp.j, rp.k0);
-    Slice hi = load_slice(g, tsdf, wt, x, rp.j, rp.k0 + 1);
-    Slice n1 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 2, k1));
-    Slice n2 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 3, k1));
-    uint32_t blo = slice_bits(lo, g.iso);
+    // A ring of four slices, slice k0 + i in s[i mod 4], unrolled by four so
+    // that the ring stays in place: step k reads slices k and k + 1 and then
+    // reloads slice k's registers with slice k + 4 (past the segment's end
+    // the last slice used, k1, again), so three slices stay in flight while
+    // a step computes.  Every pass issues the same loads and no other memory
+    // operation (the run sums are kept in lanes and stored after the march),
+    // so the waits before each step cover only the slices it reads.  (A loop
+    // that moved the slices along, lo = hi, ..., with its stores inside,
+    // waited at every step for the load it had just issued.)
+    Slice s0 = load_slice(g, tsdf, wt, x, rp.j, rp.k0);
+    Slice s1 = load_slice(g, tsdf, wt, x, rp.j, rp.k0 + 1);
+    Slice s2 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 2, k1));
+    Slice s3 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 3, k1));
     uint32_t pending = 0;  // the previous step's count, 16 bits, reduced with this one
-    for (int32_t k = rp.k0; k < k1; ++k) {
-        const Slice n3 = load_slice(g, tsdf, wt, x, rp.j, min(k + 4, k1));
-        const uint32_t bhi = slice_bits(hi, g.iso);
+    uint32_t my_sum = 0;   // lane t: the sum of step k0 + t
+    uint32_t flagged = 0;  // bit t: step k0 + t has a cube for k_mc_exact (uniform)
+    const auto step = [&](int32_t k, const Slice& lo, const Slice& hi) {
+        const uint32_t blo = slice_bits(lo, g.iso), bhi = slice_bits(hi, g.iso);
         // this lane's corner flags and the next lane's (x + 1), one DPP move
         const uint32_t mine = (blo & 3u) | ((bhi & 3u) << 2) | (((blo | bhi) & 4u) << 2);
         const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mine, 0x130, 0xf, 0xf, false);
         const bool zero = ((mine | next) & 16u) != 0u;
         const uint32_t e = s_lut[(mine & 15u) | ((next & 15u) << 4)];
         const uint32_t cs = zero ? 0u : (e & 255u);
+        const int t = k - rp.k0;
         uint32_t n = 0;
         if (!FULL) {
             n = own && !zero ? (e >> 8) & 255u : 0u;
@@ -409,33 +419,44 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __
             const bool surface = own && cs != 0u && cs != 255u;
             const bool exact = surface && (bad2 || !screen_ok(g, m, M));
             n = surface && !exact ? 3u * ((e >> 16) & 255u) : 0u;
-            if (__ballot(exact) && lane == 0) {
-                const uint64_t q = ((uint64_t)k * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
-                exact_list[1u + atomicAdd(exact_list, 1u)] = (uint32_t)q;
-            }
+            if (__ballot(exact)) flagged |= 1u << t;
         }
         // two steps per wave reduction: a run's count is below 2^16
         // (63 cubes x at most 15 vertices)
-        const bool odd = ((k - rp.k0) & 1) != 0;
+        const bool odd = (t & 1) != 0;
         if (!odd && k + 1 < k1) {
             pending = n;
         } else {
             const uint32_t both = wave_sum(odd ? (pending | (n << 16)) : n);
-            if (lane == 0) {
-                const uint64_t q = ((uint64_t)k * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
-                if (odd) {
-                    sums[q - (uint64_t)g.cy * (uint64_t)g.nr] = both & 0xffffu;
-                    sums[q] = both >> 16;
-                } else {
-                    sums[q] = both;
-                }
+            if (odd) {
+                my_sum = lane == t - 1 ? (both & 0xffffu) : my_sum;
+                my_sum = lane == t ? (both >> 16) : my_sum;
+            } else {
+                my_sum = lane == t ? both : my_sum;
             }
         }
-        lo = hi;
-        hi = n1;
-        n1 = n2;
-        n2 = n3;
-        blo = bhi;
+    };
+    for (int32_t k = rp.k0; k < k1; k += 4) {
+        step(k, s0, s1);
+        s0 = load_slice(g, tsdf, wt, x, rp.j, min(k + 4, k1));
+        if (k + 1 < k1) step(k + 1, s1, s2);
+        s1 = load_slice(g, tsdf, wt, x, rp.j, min(k + 5, k1));
+        if (k + 2 < k1) step(k + 2, s2, s3);
+        s2 = load_slice(g, tsdf, wt, x, rp.j, min(k + 6, k1));
+        if (k + 3 < k1) step(k + 3, s3, s0);
+        s3 = load_slice(g, tsdf, wt, x, rp.j, min(k + 7, k1));
+    }
+    // run q of step k0 + t: q0 + t * cy * nr
+    const uint64_t q0 = ((uint64_t)rp.k0 * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
+    const uint64_t kstride = (uint64_t)g.cy * (uint64_t)g.nr;
+    if (lane < k1 - rp.k0) sums[q0 + (uint64_t)lane * kstride] = my_sum;
+    if (FULL && flagged) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(exact_list, (uint32_t)__popc(flagged));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (lane < 32 && ((flagged >> lane) & 1u))
+            exact_list[1u + base + (uint32_t)__popc(flagged & ((1u << lane) - 1u))] =
+                (uint32_t)(q0 + (uint64_t)lane * kstride);
     }
 }
 
