@@ -358,9 +358,10 @@ def bench_gauss_newton(stream, G: int = 4096, M: int = 300_000, iters: int = 5) 
     the gn4096 workload: 4096 nodes, 300k mesh points with 6 knn slots, 4
     regularisation neighbours, targets made by a smooth rigid motion of every
     node.  vf_gn_solver_create builds the normal equations from the sparse
-    rows and factorises them (rocSOLVER getrf); vf_gn_solver_iterate then
-    runs `iters` Gauss-Newton iterations (residuals, energy, JTr, getrs,
-    update, mesh warp).  The reference rebuilds and refactorises JTJ on every
+    rows and factorises them (rocSOLVER potrf, or getrf where the Cholesky
+    factorisation fails); vf_gn_solver_iterate then runs `iters` Gauss-Newton
+    iterations (residuals, energy, JTr, the two triangular solves, update,
+    mesh warp).  The reference rebuilds and refactorises JTJ on every
     iteration, i.e. one create per iteration."""
     from volumefusion_amd import _lib as L
     from volumefusion_amd import scenes
@@ -404,16 +405,18 @@ def bench_gauss_newton(stream, G: int = 4096, M: int = 300_000, iters: int = 5) 
         L.call("vf_gn_solver_destroy", solver)
     c_ms, ne_ms, it_ms = min(create), min(ne), min(per_iter)
     lu_ms = max(c_ms - ne_ms, 1e-6)
-    flops = 2.0 / 3.0 * float(n) ** 3
+    chol = bool(rep.cholesky)
+    flops = (1.0 if chol else 2.0) / 3.0 * float(n) ** 3  # Cholesky n^3/3, LU 2n^3/3
     out = {
         "workload": f"gn4096: {G} nodes, {M} mesh points ({p.corr.shape[0]} correspondences), k={K}, "
                     f"{R} regularisation neighbours, {n}x{n} normal matrix",
         "rows": int(p.corr.shape[0] + 3 * G * R),
         "normal_equations_ms": ne_ms,
         "solver_create_ms": c_ms,
-        "lu_factor_ms_est": lu_ms,
-        "lu_factor_tflops_est": flops / (lu_ms * 1e-3) / 1e12,
-        "lu_factor_frac_fp32_peak": flops / (lu_ms * 1e-3) / FP32_PEAK,
+        "factorisation": "cholesky (potrf)" if chol else "lu (getrf)",
+        "factor_ms_est": lu_ms,
+        "factor_tflops_est": flops / (lu_ms * 1e-3) / 1e12,
+        "factor_frac_fp32_peak": flops / (lu_ms * 1e-3) / FP32_PEAK,
         "gn_iteration_ms": it_ms,
         "iterations": iters,
         "icp_iteration_ms": c_ms + iters * it_ms,
@@ -878,7 +881,7 @@ def main() -> int:
         result["gauss_newton"] = bench_gauss_newton(stream)
         g = result["gauss_newton"]
         print(f"gauss-newton: normal equations {g['normal_equations_ms']:.2f} ms, solver create "
-              f"{g['solver_create_ms']:.2f} ms (LU ~{g['lu_factor_tflops_est']:.1f} TFLOP/s), "
+              f"{g['solver_create_ms']:.2f} ms ({g['factorisation']} ~{g['factor_tflops_est']:.1f} TFLOP/s), "
               f"{g['gn_iteration_ms']:.3f} ms per iteration", file=sys.stderr, flush=True)
     if not args.no_fp64:
         result["fp64_error"] = fp64_error((L.BRICK_SHAPES.index(tuple(tuning["brick"])), tuning["wg_per_cu"]),

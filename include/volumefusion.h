@@ -529,14 +529,18 @@ int vf_gn_normal_equations(vf_workspace* ws, float* JTJ, float* JTr, const float
                            vf_stream stream);
 
 /* A solver for one ICP iteration: builds the rows and JTJ and factorises
- * JTJ once (its value does not change during the Gauss-Newton iterations);
- * the problem's buffers must stay alive and unchanged while it exists. */
+ * JTJ once (its value does not change during the Gauss-Newton iterations):
+ * by Cholesky, JTJ being symmetric positive definite, or, where that finds a
+ * non-positive pivot, by the LU with partial pivoting the reference uses
+ * (VF_ERR_SINGULAR on a zero pivot).  The problem's buffers must stay alive
+ * and unchanged while it exists. */
 typedef struct vf_gn_solver vf_gn_solver;
 typedef struct vf_gn_report {
     int iterations;       /* updates applied                               */
     int stopped;          /* 1: the energy decrease fell below min_step    */
     double final_energy;  /* energy of the last accepted transforms        */
     double energy[32];    /* energy at the start of each iteration         */
+    int cholesky;         /* 1: JTJ was factorised by Cholesky, 0: by LU   */
 } vf_gn_report;
 int vf_gn_solver_create(vf_gn_solver** solver, const vf_gn_problem* problem, vf_stream stream);
 int vf_gn_solver_destroy(vf_gn_solver* solver);

@@ -148,10 +148,11 @@ def test_correspondence_pairs(vf, oracle):
     assert (got[:base] == -7).all()
 
 
-def _oracle_loop(oracle, p, T, iters, min_step, float_lu=False):
+def _oracle_loop(oracle, p, T, iters, min_step, float_lu=False, float_chol=False):
     """The Gauss-Newton loop of src/VolumeFusion.cu:541-691 on the oracle,
     with its LU in binary64 (float_lu: LAPACK's single-precision LU instead,
-    the yardstick for a float solver's rounding)."""
+    float_chol: its single-precision Cholesky; the yardsticks for a float
+    solver's rounding)."""
     G = p.graph.shape[0]
     J = np.concatenate([oracle.gn_j_depth(p.wD, p.corr, p.canon, p.knn, p.kw, p.dnormals, G),
                         oracle.gn_j_mreg(p.wR, p.graph, p.gknn)])
@@ -174,6 +175,10 @@ def _oracle_loop(oracle, p, T, iters, min_step, float_lu=False):
             import scipy.linalg as sl
 
             x = sl.lu_solve(sl.lu_factor(JTJ), JTr.reshape(-1)).astype(np.float64)
+        elif float_chol:
+            import scipy.linalg as sl
+
+            x = sl.cho_solve(sl.cho_factor(JTJ, lower=True), JTr.reshape(-1)).astype(np.float64)
         else:
             x, info = oracle.lu_solve(JTJ, JTr.reshape(-1))
             assert info == 0
@@ -187,18 +192,20 @@ def test_gn_solver_loop(vf, oracle):
     """The Gauss-Newton loop on the device (JTJ factorised once) against the
     oracle's loop: the same number of steps, energies within 1e-3, the energy
     falling twentyfold or more, and transforms as close to the binary64 loop
-    as a single-precision LAPACK LU gets (three times its deviation: JTJ's
-    condition number here is about 3e5, so a float LU moves the update by
-    up to about 1e-4)."""
+    as a single-precision LAPACK solver gets (three times the larger deviation
+    of its LU and its Cholesky factorisation, the one the device uses for
+    this positive definite JTJ: JTJ's condition number here is about 3e5, so
+    a float factorisation moves the update by up to about 1e-4)."""
     G = 48
     p = scenes.gn_problem(6, G=G, M=3000, warp_angle_deg=3.0, warp_translation=0.006)
     T0 = np.tile(np.eye(4, dtype=F).reshape(16), (G, 1))
     iters, min_step = 4, 1e-7
     T_or, pend_or, e_or, steps_or = _oracle_loop(oracle, p, T0, iters, min_step)
     T_f, pend_f, _, steps_f = _oracle_loop(oracle, p, T0, iters, min_step, float_lu=True)
-    assert steps_f == steps_or
-    tol_T = 3.0 * float(np.abs(T_f - T_or).max()) + 1e-6
-    tol_P = 3.0 * float(np.abs(pend_f - pend_or).max()) + 1e-6
+    T_c, pend_c, _, steps_c = _oracle_loop(oracle, p, T0, iters, min_step, float_chol=True)
+    assert steps_f == steps_or and steps_c == steps_or
+    tol_T = 3.0 * max(float(np.abs(T_f - T_or).max()), float(np.abs(T_c - T_or).max())) + 1e-6
+    tol_P = 3.0 * max(float(np.abs(pend_f - pend_or).max()), float(np.abs(pend_c - pend_or).max())) + 1e-6
     d = _Dev(p)
     solver = C.c_void_p()
     L.call("vf_gn_solver_create", C.byref(solver), C.byref(d.desc), None)
@@ -213,9 +220,10 @@ def test_gn_solver_loop(vf, oracle):
     e_dev = [rep.energy[i] for i in range(len(e_or))]
     print(f"energies oracle {e_or}, device {e_dev}, steps {steps_or}/{rep.iterations}", flush=True)
     assert rep.iterations == steps_or
+    assert rep.cholesky == 1, "a positive definite JTJ is factorised by Cholesky"
     np.testing.assert_allclose(e_dev, e_or, rtol=1e-3, atol=1e-9)
     assert e_or[-1] < 0.05 * e_or[0]
     print(f"transform deviation from the binary64 loop: device {np.abs(dT.download() - T_or).max():.3g}, "
-          f"float LAPACK LU {tol_T / 3:.3g}", flush=True)
+          f"float LAPACK LU or Cholesky {(tol_T - 1e-6) / 3:.3g}", flush=True)
     np.testing.assert_allclose(dT.download(), T_or, rtol=0, atol=tol_T)
     np.testing.assert_allclose(dP.download(), pend_or, rtol=0, atol=tol_P)

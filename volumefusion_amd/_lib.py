@@ -181,7 +181,7 @@ class GNProblem(C.Structure):
 
 class GNReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("stopped", C.c_int), ("final_energy", C.c_double),
-                ("energy", C.c_double * 32)]
+                ("energy", C.c_double * 32), ("cholesky", C.c_int)]
 
 
 vp = C.c_void_p

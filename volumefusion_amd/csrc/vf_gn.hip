@@ -17,9 +17,11 @@
 //   - each node's rows are sorted by row index, and one wave per node sums
 //     its 6x6 blocks and its JTr entry over those rows in ascending order
 //     (deterministic; the reference's order is an atomic race);
-//   - JTJ is factorised once (rocSOLVER getrf, the LU with partial pivoting
-//     the reference calls through cuSOLVER), and each Gauss-Newton iteration
-//     only forms JTr from the new residuals and back-substitutes (getrs);
+//   - JTJ is factorised once (rocSOLVER potrf: JTJ is symmetric positive
+//     definite, so its Cholesky factor solves the system the reference's LU
+//     with partial pivoting solves, in half the work; getrf where potrf
+//     finds a non-positive pivot), and each Gauss-Newton iteration only forms
+//     JTr from the new residuals and back-substitutes;
 //   - residuals, energy (binary64, fixed order), update and the mesh warp
 //     stay on the device; one read-back per iteration carries the energy the
 //     reference's stopping rule needs.
@@ -652,7 +654,9 @@ __global__ __launch_bounds__(256) void k_gather_perm(const float* __restrict__ b
 
 // Forward step k (k = -1: the first diagonal block alone).  Wave w takes row
 // block k + 1 + w; the wave of block k + 1 also solves its diagonal block.
-// LU is column-major with leading dimension n; L has a unit diagonal.
+// LU is column-major with leading dimension n; L has a unit diagonal (UNIT,
+// the LU factors) or its own (the Cholesky factor, divisions by it).
+template <bool UNIT>
 __global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict__ LU, int n, int k,
                                                         float* __restrict__ y) {
     const int lane = threadIdx.x;
@@ -680,7 +684,10 @@ __global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict_
             a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
 #pragma unroll
         for (int j = 0; j < kTB; ++j) {
-            const float yj = lane_value(r, j);  // final: lane j has subtracted columns 0..j-1
+            // final: lane j has subtracted columns 0..j-1 (and, with its own
+            // diagonal, divides by it: a[j] of lane j is L(j, j))
+            if (!UNIT && lane == j && j < m) r = r / a[j];
+            const float yj = lane_value(r, j);
             if (lane > j) r = r - a[j] * yj;
         }
     }
@@ -689,9 +696,15 @@ __global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict_
 
 // Backward step k (k = nb: the last diagonal block alone).  Wave w takes row
 // block k - 1 - w; the wave of block k - 1 also solves its diagonal block (U
-// upper, divisions by its diagonal).
+// upper, divisions by its diagonal).  U is the upper triangle of LU, or
+// (TRANS) the transpose of the Cholesky factor L in LU's lower triangle:
+// U(i, c) = L(c, i), which lane i reads from its column of L.
+template <bool TRANS>
 __global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict__ LU, int n, int k, int nb,
                                                         float* __restrict__ x) {
+    const auto u_at = [&](int i, int c) -> float {
+        return TRANS ? LU[(uint64_t)i * (uint64_t)n + (uint64_t)c] : LU[(uint64_t)c * (uint64_t)n + (uint64_t)i];
+    };
     const int lane = threadIdx.x;
     const int b = k - 1 - (int)blockIdx.x;
     const int i = b * kTB + lane;
@@ -703,8 +716,7 @@ __global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict_
         const float xk = lane < m ? x[c0 + lane] : 0.0f;
         float a[kTB];
 #pragma unroll
-        for (int j = 0; j < kTB; ++j)
-            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+        for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(xk, j);
@@ -715,8 +727,7 @@ __global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict_
         const int m = n - c0 < kTB ? n - c0 : kTB;
         float a[kTB];
 #pragma unroll
-        for (int j = 0; j < kTB; ++j)
-            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+        for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
 #pragma unroll
         for (int j = kTB - 1; j >= 0; --j) {
             if (j < m) {
@@ -746,7 +757,8 @@ struct vf_gn_solver {
     uint32_t* node8 = nullptr;  // row data: members of each row
     float* j48 = nullptr;       // row data: their Jacobian entries
     uint32_t* bad = nullptr;
-    float* JTJ = nullptr;  // LU factors after the factorisation
+    float* JTJ = nullptr;  // the factors after the factorisation: Cholesky (lower) or LU
+    bool chol = false;     // JTJ holds the Cholesky factor
     int* ipiv = nullptr;
     int* info = nullptr;
     float* r = nullptr;    // residuals, nrows
@@ -862,8 +874,9 @@ int read_bad(vf_gn_solver* s, hipStream_t st) {
     }
 }
 
-// y = JTJ^-1 x with the factors of vf_gn_solver_create (P, L, U): the gather,
-// then the forward and backward block steps.
+// y = JTJ^-1 x with the factors of vf_gn_solver_create: the gather (P, the
+// identity for Cholesky), then the forward and backward block steps (L and
+// U of the LU factors, or L and L^T of the Cholesky factor).
 int lu_solve_factored(vf_gn_solver* s, hipStream_t st) {
     const int n = (int)s->n;
     const int nb = (n + kTB - 1) / kTB;
@@ -871,13 +884,21 @@ int lu_solve_factored(vf_gn_solver* s, hipStream_t st) {
                        (const int*)s->perm, s->y, n);
     VF_LAUNCH_CHECK("k_gather_perm");
     for (int k = -1; k <= nb - 2; ++k) {
-        hipLaunchKernelGGL(k_trsv_lower_step, dim3(k < 0 ? 1u : (unsigned)(nb - 1 - k)), dim3(kTB), 0, st,
-                           (const float*)s->JTJ, n, k, s->y);
+        const dim3 grid(k < 0 ? 1u : (unsigned)(nb - 1 - k));
+        if (s->chol)
+            hipLaunchKernelGGL(k_trsv_lower_step<false>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, s->y);
+        else
+            hipLaunchKernelGGL(k_trsv_lower_step<true>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, s->y);
         VF_LAUNCH_CHECK("k_trsv_lower_step");
     }
     for (int k = nb; k >= 1; --k) {
-        hipLaunchKernelGGL(k_trsv_upper_step, dim3(k == nb ? 1u : (unsigned)k), dim3(kTB), 0, st,
-                           (const float*)s->JTJ, n, k, nb, s->y);
+        const dim3 grid(k == nb ? 1u : (unsigned)k);
+        if (s->chol)
+            hipLaunchKernelGGL(k_trsv_upper_step<true>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, nb,
+                               s->y);
+        else
+            hipLaunchKernelGGL(k_trsv_upper_step<false>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, nb,
+                               s->y);
         VF_LAUNCH_CHECK("k_trsv_upper_step");
     }
     return VF_OK;
@@ -1107,31 +1128,60 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
         st = fail(VF_ERR_HIP, "rocblas_create_handle");
     if (!st && rocblas_set_stream(s->blas, hs) != rocblas_status_success) st = fail(VF_ERR_HIP, "rocblas_set_stream");
     if (!st) st = build_rows(s, hs);
-    if (!st) {
+    const auto normal_matrix = [&]() -> int {
         hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, s->JTJ, s->n);
         hipLaunchKernelGGL(k_blocks, dim3(s->rows.G), dim3(64), 0, hs, s->rows.G, (const uint32_t*)s->cnt,
                            (const uint32_t*)s->rows_of, (const uint32_t*)s->node8, (const float*)s->j48,
                            (const float*)nullptr, s->JTJ, (float*)nullptr, s->bad);
-        if (hipGetLastError() != hipSuccess) st = fail(VF_ERR_HIP, "normal-equation kernels launch");
-    }
-    if (!st) st = read_bad(s, hs);
-    // JTJ does not change within the ICP iteration: factorise it once (the
-    // reference's LU with partial pivoting, linearSolverLUStable).  Partial
-    // pivoting does interchange rows on these matrices (the rotation
-    // columns' couplings exceed their diagonals), so the factorisation
-    // without pivot searches is not an option.
-    if (!st && rocsolver_sgetrf(s->blas, (int)s->n, (int)s->n, s->JTJ, (int)s->n, s->ipiv, s->info) !=
+        if (hipGetLastError() != hipSuccess) return fail(VF_ERR_HIP, "normal-equation kernels launch");
+        return read_bad(s, hs);
+    };
+    const auto read_info = [&](int* hinfo) -> int {
+        if (hipMemcpyAsync(hinfo, s->info, sizeof(int), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            return fail(VF_ERR_HIP, "gn: info read-back");
+        return VF_OK;
+    };
+    if (!st) st = normal_matrix();
+    // JTJ does not change within the ICP iteration: factorise it once.  JTJ
+    // (J^T J, with the identity where a node has no rows) is symmetric
+    // positive definite unless the problem is degenerate, so the Cholesky
+    // factorisation L L^T (rocSOLVER potrf, half the work of the LU and no
+    // pivot search) solves the same system; the reference calls an LU with
+    // partial pivoting (linearSolverLUStable, src/LinearSolver.cu:179-225).
+    // Where potrf finds a non-positive pivot, JTJ is rebuilt and factorised
+    // as the reference does (getrf), which then reports a zero pivot as the
+    // reference's solver would.
+    if (!st && rocsolver_spotrf(s->blas, rocblas_fill_lower, (int)s->n, s->JTJ, (int)s->n, s->info) !=
                    rocblas_status_success)
-        st = fail(VF_ERR_HIP, "rocsolver_sgetrf");
+        st = fail(VF_ERR_HIP, "rocsolver_spotrf");
     if (!st) {
         int hinfo = 0;
-        if (hipMemcpyAsync(&hinfo, s->info, sizeof(int), hipMemcpyDeviceToHost, hs) != hipSuccess ||
-            hipStreamSynchronize(hs) != hipSuccess)
-            st = fail(VF_ERR_HIP, "gn: info read-back");
-        else if (hinfo != 0)
-            st = fail(VF_ERR_SINGULAR, "gn: the normal matrix has a zero pivot at column %d", hinfo);
+        st = read_info(&hinfo);
+        s->chol = !st && hinfo == 0;
     }
-    if (!st) {
+    if (!st && !s->chol) {
+        st = normal_matrix();
+        // Partial pivoting does interchange rows on these matrices (the
+        // rotation columns' couplings exceed their diagonals), so the LU
+        // without pivot searches is not an option.
+        if (!st && rocsolver_sgetrf(s->blas, (int)s->n, (int)s->n, s->JTJ, (int)s->n, s->ipiv, s->info) !=
+                       rocblas_status_success)
+            st = fail(VF_ERR_HIP, "rocsolver_sgetrf");
+        if (!st) {
+            int hinfo = 0;
+            st = read_info(&hinfo);
+            if (!st && hinfo != 0)
+                st = fail(VF_ERR_SINGULAR, "gn: the normal matrix has a zero pivot at column %d", hinfo);
+        }
+    }
+    if (!st && s->chol) {
+        std::vector<int> perm(s->n);
+        for (uint64_t i = 0; i < s->n; ++i) perm[i] = (int)i;
+        if (hipMemcpyAsync(s->perm, perm.data(), sizeof(int) * s->n, hipMemcpyHostToDevice, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            st = fail(VF_ERR_HIP, "gn: permutation upload");
+    } else if (!st) {
         // the interchanges (row i with row ipiv[i] - 1, in order) as one gather
         std::vector<int> piv(s->n), perm(s->n);
         if (hipMemcpyAsync(piv.data(), s->ipiv, sizeof(int) * s->n, hipMemcpyDeviceToHost, hs) != hipSuccess ||
@@ -1177,6 +1227,7 @@ int vf_gn_solver_iterate(vf_gn_solver* s, vf_mat4* transforms, vf_mat4* pending,
     hipStream_t hs = (hipStream_t)stream;
     if (rocblas_set_stream(s->blas, hs) != rocblas_status_success) return fail(VF_ERR_HIP, "rocblas_set_stream");
     memset(report, 0, sizeof(*report));
+    report->cholesky = s->chol ? 1 : 0;
     const uint32_t G = s->rows.G;
     // the loop of src/VolumeFusion.cu:541-691: pending starts as the transforms
     VF_HIP(hipMemcpyAsync(pending, transforms, sizeof(vf_mat4) * G, hipMemcpyDeviceToDevice, hs));
