@@ -53,6 +53,9 @@ def test_stream_same_alone_and_in_mix(vf, oracle):
     mixed = [srv.read(sid) for sid in ids]
     st = srv.stats()
     assert st["frames_done"] == len(tickets) and st["streams_open"] == len(sc)
+    # these scenes have pending updates on 7-16 % of their voxels, above the
+    # lanes' budget of 1/24 per voxel: bricks went through the redo pass
+    assert st["totals"]["redo_bricks"] > 0
     srv.close_server()
     for x, y in zip(alone, mixed[0]):
         np.testing.assert_array_equal(x, y)
