@@ -619,11 +619,14 @@ typedef struct vf_server_stats {
 } vf_server_stats;
 
 /* The most a stream with this description adds to a server's admission
- * charge: its buffers plus `lanes` times what it adds to a workspace whose
- * batch holds it (with half again for the workspace's growth).  A server
- * charges its open streams' buffers plus `lanes` workspaces sized for the
- * batch of its max_batch largest workspace shares; a stream is admitted
- * when that charge, with it, fits the HBM budget. */
+ * charge: its buffers plus `lanes` times what it adds to a lane workspace
+ * whose batch holds it.  Lane workspaces carry a pending-record budget of
+ * 1/24 record per voxel (vf_workspace_set_record_budget) and grow to exactly
+ * what their batches need, so a server charges its open streams' buffers
+ * plus `lanes` times, for each of the workspace's buffer groups (node
+ * records, brick state, collision tiles, pending records, node lists,
+ * descriptors, depth tiles), the sum of its max_batch largest shares; a
+ * stream is admitted when that charge, with it, fits the HBM budget. */
 uint64_t vf_server_stream_bytes(const vf_stream_desc* desc, int lanes);
 int vf_server_create(vf_server** server, const vf_server_config* config);
 /* Finishes every submitted frame, then frees everything. */
