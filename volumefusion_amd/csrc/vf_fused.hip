@@ -1876,7 +1876,7 @@ struct FrameSetup {
     uint64_t blocks_pair = 0;   // k_prep_bin workgroups over (node, brick) pairs
     uint64_t blocks_pyr = 0;    // k_prep_bin workgroups over depth blocks
     uint32_t k1 = 0;
-    uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
+    uint64_t blocks_prep = 0, blocks_a = 0, blocks_b = 0, blocks_redo = 0, arena_cap = 0, overflow_cap = 0, bricks = 0;
     bool redo = false;         // launch the redo pass (a record budget: some bricks may not fit)
     uint64_t tiles = 0;        // collision tiles of this batch
     uint64_t nodes = 0;        // motion-graph nodes of this batch
@@ -2123,6 +2123,9 @@ int setup_frame(vf_workspace* ws, int nvol, const vf_volume* vols, const vf_grap
     fs->nodes = need.nodes;
     fs->clear_tiles = std::max<uint64_t>(ws->dirty_tiles, need.tiles);
     fs->blocks_b = std::min<uint64_t>(blocks_a, (uint64_t)cus * 8);
+    // the redo list is short or empty on most frames: one workgroup per CU
+    // starts and exits quickly, and takes the list round-robin when it is long
+    fs->blocks_redo = std::min<uint64_t>(blocks_a, (uint64_t)std::max(cus, 1));
     fs->arena_cap = arena_cap;
     fs->overflow_cap = overflow_cap;
     fs->redo = redo;
@@ -2182,7 +2185,7 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
                VF_PASS_A_ARGS);                                                                             \
         if (fs.redo) {                                                                                      \
             VF_LAUNCH_CHECK("k_pass_a");                                                                    \
-            launch(k_pass_a<KK, false, S, true>, dim3((unsigned)fs.blocks_a), dim3(S::kThreads), stream,    \
+            launch(k_pass_a<KK, false, S, true>, dim3((unsigned)fs.blocks_redo), dim3(S::kThreads), stream, \
                    nullptr, nullptr, VF_PASS_A_ARGS);                                                       \
         }                                                                                                   \
     }
