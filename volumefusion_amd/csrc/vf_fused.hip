@@ -708,7 +708,7 @@ template <class S, bool SCREEN>
 __global__ __launch_bounds__(kPrepThreads) void k_compact(
     const VolDesc* __restrict__ vols, int nvol, uint32_t nbricks, const uint32_t* __restrict__ brick_count,
     uint32_t* __restrict__ brick_off, const uint32_t* __restrict__ brick_disp, const float* __restrict__ dmax,
-    uint32_t* __restrict__ active2,
+    uint32_t* __restrict__ active2, uint2* __restrict__ active_oc2,
     uint32_t* __restrict__ cand, uint4* __restrict__ cand_info, uint32_t* __restrict__ tile_mark,
     uint32_t* __restrict__ vol_mark, Ctrl* __restrict__ ctrl, uint64_t brick_cap, int parity_arg) {
     __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];  // per (round, wave): count, then first position
@@ -721,6 +721,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
     const uint64_t gb0 = (uint64_t)blockIdx.x * (kPrepThreads * kCompactPer) + threadIdx.x;
     uint32_t flags = 0;  // bit r: brick gb0 + r * kPrepThreads is active
     uint32_t lw[kCompactPer], lsum = 0;  // pool entries of each brick (0 for a spilled list)
+    uint2 oc[kCompactPer];               // each brick's (first pool entry, list count)
 #pragma unroll
     for (int r = 0; r < kCompactPer; ++r) {
         const uint64_t gb = gb0 + (uint64_t)(r * kPrepThreads);
@@ -728,13 +729,15 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
         flags |= (uint32_t)(c != 0u) << r;
         lw[r] = c <= (uint32_t)kCap ? c : 0u;
         lsum += lw[r];
+        oc[r] = make_uint2(kNone, c);
     }
     {
         uint32_t off = block_alloc(lsum, &ctrl->list_total, s_lw, &s_lbase);
 #pragma unroll
         for (int r = 0; r < kCompactPer; ++r) {
             // a spilled list has no place: its entries are dropped
-            if ((flags >> r) & 1u) brick_off[gb0 + (uint64_t)(r * kPrepThreads)] = lw[r] ? off : kNone;
+            if (lw[r]) oc[r].x = off;
+            if ((flags >> r) & 1u) brick_off[gb0 + (uint64_t)(r * kPrepThreads)] = oc[r].x;
             off += lw[r];
         }
     }
@@ -755,10 +758,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
                     hi = mid - 1;
             }
             const uint32_t item = ((uint32_t)lo << 24) | (gb - s_bb[lo]);
-            if (!SCREEN)
+            if (!SCREEN) {
                 active2[(uint64_t)parity * brick_cap + pos] = item;
-            else
+                active_oc2[(uint64_t)parity * brick_cap + pos] = oc[r];
+            } else {
                 s_item[pos - base] = item;
+            }
         }
     }
     if (!SCREEN) return;
@@ -837,8 +842,8 @@ constexpr uint32_t kSelPerBlock = kCompactPer * kSelPerRound;
 __global__ __launch_bounds__(kPrepThreads) void k_select(
     const VolDesc* __restrict__ vols, const uint32_t* __restrict__ cand, const uint4* __restrict__ cand_info,
     uint32_t* __restrict__ brick_count, uint32_t* __restrict__ brick_disp, const uint32_t* __restrict__ tile_mark,
-    const uint32_t* __restrict__ vol_mark, uint32_t* __restrict__ active2, Ctrl* __restrict__ ctrl,
-    uint64_t brick_cap, int parity_arg, const uint2* __restrict__ items, uint32_t total_items,
+    const uint32_t* __restrict__ vol_mark, uint32_t* __restrict__ active2, uint2* __restrict__ active_oc2,
+    Ctrl* __restrict__ ctrl, uint64_t brick_cap, int parity_arg, const uint2* __restrict__ items, uint32_t total_items,
     const uint32_t* __restrict__ brick_off, uint32_t* __restrict__ brick_nodes, unsigned long long* __restrict__ stat,
     unsigned long long* __restrict__ totals) {
     __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];
@@ -855,10 +860,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
     // the loop bound is the same for every thread of the workgroup
     for (uint64_t c0 = (uint64_t)blockIdx.x * kSelPerBlock; c0 < n; c0 += (uint64_t)gridDim.x * kSelPerBlock) {
         uint32_t flags = 0, items[kCompactPer];
+        uint2 ocs[kCompactPer];
 #pragma unroll
         for (int r = 0; r < kCompactPer; ++r) {
             const uint64_t i = c0 + (uint64_t)r * kSelPerRound + threadIdx.x / kGrp;
             items[r] = 0u;
+            ocs[r] = make_uint2(0u, 0u);
             if (i >= n) continue;
             const uint32_t item = cand[i];
             const uint4 info = cand_info[i];
@@ -899,6 +906,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
                 if (!keep) {
                     brick_count[gb] = 0u;
                     ++dropped;
+                } else {
+                    ocs[r] = make_uint2(brick_off[gb], brick_count[gb]);
                 }
                 items[r] = item;
                 flags |= (uint32_t)keep << r;
@@ -909,7 +918,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
         for (int r = 0; r < kCompactPer; ++r) {
             const bool keep = (flags >> r) & 1u;
             const uint32_t pos = compact_pos(base, s_cnt, r, keep);
-            if (keep) active2[(uint64_t)parity * brick_cap + pos] = items[r];
+            if (keep) {
+                active2[(uint64_t)parity * brick_cap + pos] = items[r];
+                active_oc2[(uint64_t)parity * brick_cap + pos] = ocs[r];
+            }
         }
         __syncthreads();  // s_cnt and s_base are rewritten by the next round
     }
@@ -961,7 +973,8 @@ template <int K, bool KNN_ONLY, class S, bool REDO = false>
 __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_pass_a(
     const VolDesc* __restrict__ vols, const NodeRec* __restrict__ nrec, uint32_t* __restrict__ brick_count,
     const uint32_t* __restrict__ brick_nodes, const uint32_t* __restrict__ brick_off,
-    const uint32_t* __restrict__ active2, uint32_t* __restrict__ redo, Ctrl* __restrict__ ctrl,
+    const uint32_t* __restrict__ active2, const uint2* __restrict__ active_oc2, uint32_t* __restrict__ redo,
+    Ctrl* __restrict__ ctrl,
     uint32_t* __restrict__ bitmap, uint32_t* __restrict__ tile_flag, uint64_t brick_cap, int parity_arg,
     Arena arena, unsigned long long* __restrict__ stat,
     unsigned long long* __restrict__ totals, KnnOut kout) {
@@ -988,6 +1001,12 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     const int wave = tid >> 6;
     const uint32_t parity = frame_parity(ctrl, parity_arg);
     const uint32_t* __restrict__ active = REDO ? redo : active2 + (uint64_t)parity * brick_cap;
+    // each active brick's (first node-list entry, list count), read with the
+    // entry (the redo pass reads them from the brick arrays)
+    const uint2* __restrict__ active_oc = active_oc2 + (uint64_t)parity * brick_cap;
+    const auto brick_oc = [&](uint32_t a_i, uint32_t gbi) -> uint2 {
+        return REDO ? make_uint2(brick_off[gbi], brick_count[gbi]) : active_oc[a_i];
+    };
     if (tid < 5) s_stat[tid] = 0ull;
     if (tid < 32) s_exp2[tid] = kExp2Table[tid];
     if (!KNN_ONLY && !REDO) {
@@ -1049,9 +1068,10 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     if (item != kNone) {
         const VolDesc& v0 = vols[item >> 24];
         const uint32_t gb0 = v0.brick_base + (item & 0xffffffu);
-        cnt = brick_count[gb0];
+        const uint2 oc0 = brick_oc(a, gb0);
+        cnt = oc0.y;
         if (cnt <= (uint32_t)kCap && (uint32_t)tid < cnt) {
-            pre_id = brick_nodes[brick_off[gb0] + tid];
+            pre_id = brick_nodes[oc0.x + tid];
             const float4* r = (const float4*)(nrec + v0.node_base + pre_id);
             pm = r[0];
             plo = r[1];
@@ -1096,8 +1116,9 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
             const VolDesc& vn = vols[item_n >> 24];
             const uint32_t gbn = vn.brick_base + (item_n & 0xffffffu);
             nbase_n = vn.node_base;
-            cnt_n = brick_count[gbn];
-            if (cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) id_n = brick_nodes[brick_off[gbn] + tid];
+            const uint2 ocn = brick_oc(a_n, gbn);
+            cnt_n = ocn.y;
+            if (cnt_n <= (uint32_t)kCap && (uint32_t)tid < cnt_n) id_n = brick_nodes[ocn.x + tid];
         }
 
         float p[3];
@@ -1614,6 +1635,7 @@ struct Buffers {
     uint32_t* brick_nodes = nullptr;  // node-list pool: item_cap entries
     uint2* items = nullptr;           // per (node, brick) item: its list entry (brick, node << 7 | slot)
     uint32_t* active = nullptr;  // 2 x brick_cap (frame parity); cand doubles as the redo list
+    uint2* active_oc = nullptr;  // 2 x brick_cap: each active brick's (first node-list entry, list count)
     uint32_t* bitmap = nullptr;
     uint32_t* tile_flag = nullptr;
     uint4* records = nullptr;     // arena_slots x arena_cap, then the overflow arena
@@ -1677,7 +1699,7 @@ void free_buffers(Buffers& b) {
     void* ptrs[] = {b.ctrl,      b.nrec,         b.brick_count,  b.brick_nodes, b.active,    b.bitmap,
                     b.tile_flag, b.records,      b.arena_counts, b.stat,        b.totals,    b.vols,
                     b.vptr,      b.brick_disp,   b.cand,         b.cand_info,   b.tile_mark, b.vol_mark,
-                    b.dmax,      b.brick_off,    b.items};
+                    b.dmax,      b.brick_off,    b.items,        b.active_oc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = Buffers();
@@ -1743,6 +1765,7 @@ int ensure(vf_workspace* ws, const Need& need, hipStream_t stream) {
     if (!st) st = dev_alloc(&nbuf.brick_nodes, nbuf.item_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.items, nbuf.item_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.active, nbuf.brick_cap * 2, &bytes);
+    if (!st) st = dev_alloc(&nbuf.active_oc, nbuf.brick_cap * 2, &bytes);
     if (!st) st = dev_alloc(&nbuf.bitmap, nbuf.tile_cap * 32, &bytes);
     if (!st) st = dev_alloc(&nbuf.tile_flag, nbuf.tile_cap, &bytes);
     if (!st) st = dev_alloc(&nbuf.records, nbuf.record_cap, &bytes);
@@ -1941,7 +1964,8 @@ int check_tuning(const vf_tuning& t) {
 }  // namespace
 
 // The groups follow setup_frame and ensure: Need.nodes (NodeRec), Need.bricks
-// (brick_count, brick_off, active x 2, brick_disp, cand, cand_info) with
+// (brick_count, brick_off, active x 2, active_oc x 2, brick_disp, cand,
+// cand_info) with
 // Need.arena_slots (at most one pass-A workgroup per brick), Need.tiles
 // (bitmap, tile_flag, tile_mark), Need.records (the budget, at least one),
 // Need.items (items, brick_nodes), Need.nvol (descriptor sets, vol_mark) and
@@ -1966,7 +1990,7 @@ void volume_ws_bytes(vf_int3 dim, uint64_t nodes, vf_int3 cube, int width, int h
                          (w + kTile1 - 1) / kTile1 * ((h + kTile1 - 1) / kTile1);
     const uint64_t fixed = sizeof(Ctrl) + 2ull * kStCount * kStatSlots * 8 + sizeof(uint4) + sizeof(uint2) + 4;
     out[0] = nodes * sizeof(NodeRec);
-    out[1] = bricks * (4 + 4 + 8 + 4 + 4 + 16 + 4);
+    out[1] = bricks * (4 + 4 + 8 + 16 + 4 + 4 + 16 + 4);
     out[2] = tiles * (32 * 4 + 4 + 4);
     out[3] = (uint64_t)std::ceil((double)n * (double)records_per_voxel) * sizeof(uint4);
     out[4] = k1 * (sizeof(uint2) + 4);
@@ -2147,19 +2171,20 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     if (fs.screen) {
         launch(k_compact<S, true>, grid_c, dim3(kPrepThreads), stream, nullptr, nullptr, vols, fs.nvol,
                (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.brick_off, (const uint32_t*)B.brick_disp,
-               (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
-               parity_arg);
+               (const float*)B.dmax, B.active, B.active_oc, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl,
+               B.brick_cap, parity_arg);
         VF_LAUNCH_CHECK("k_compact");
         launch(k_select, grid_c, dim3(kPrepThreads), stream, nullptr, e[1], vols, (const uint32_t*)B.cand,
                (const uint4*)B.cand_info, B.brick_count, B.brick_disp, (const uint32_t*)B.tile_mark,
-               (const uint32_t*)B.vol_mark, B.active, B.ctrl, B.brick_cap, parity_arg, (const uint2*)B.items,
+               (const uint32_t*)B.vol_mark, B.active, B.active_oc, B.ctrl, B.brick_cap, parity_arg,
+               (const uint2*)B.items,
                fs.k1, (const uint32_t*)B.brick_off, B.brick_nodes, B.stat, B.totals);
         VF_LAUNCH_CHECK("k_select");
     } else {
         launch(k_compact<S, false>, grid_c, dim3(kPrepThreads), stream, nullptr, nullptr, vols, fs.nvol,
                (uint32_t)fs.bricks, (const uint32_t*)B.brick_count, B.brick_off, (const uint32_t*)B.brick_disp,
-               (const float*)B.dmax, B.active, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl, B.brick_cap,
-               parity_arg);
+               (const float*)B.dmax, B.active, B.active_oc, B.cand, B.cand_info, B.tile_mark, B.vol_mark, B.ctrl,
+               B.brick_cap, parity_arg);
         VF_LAUNCH_CHECK("k_compact");
         launch(k_fill_lists, dim3((unsigned)fs.blocks_pair), dim3(kPrepThreads), stream, nullptr, e[1],
                (const uint2*)B.items, fs.k1, (const uint32_t*)B.brick_off, B.brick_nodes);
@@ -2174,7 +2199,8 @@ int launch_kernels_shape(Buffers& B, const FrameSetup& fs, int parity_arg, KnnOu
     arena.overflow_cap = (uint32_t)fs.overflow_cap;
 #define VF_PASS_A_ARGS                                                                                    \
     vols, (const NodeRec*)B.nrec, B.brick_count, (const uint32_t*)B.brick_nodes, (const uint32_t*)B.brick_off, \
-        (const uint32_t*)B.active, B.cand, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, parity_arg, arena, B.stat, \
+        (const uint32_t*)B.active, (const uint2*)B.active_oc, B.cand, B.ctrl, B.bitmap, B.tile_flag, B.brick_cap, \
+        parity_arg, arena, B.stat,                                                                                  \
         B.totals, kout
 #define VF_PASS_A(KK)                                                                                       \
     if (fs.knn_only) {                                                                                      \
