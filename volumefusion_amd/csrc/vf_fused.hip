@@ -161,13 +161,16 @@ constexpr uint32_t kQueues = 8;       // pass-A brick queues, one per XCD
 constexpr uint32_t kQueueChunk = 16;  // list entries per queue chunk
 constexpr uint32_t kQueueStride = 64; // counters 256 bytes apart
 
+// A list's length (low half) and, when k_compact appends to it, the node-list
+// pool entries handed out with it (high half): one 64-bit atomic reserves both.
+__device__ __forceinline__ uint32_t run_length(unsigned long long run) { return (uint32_t)run; }
+
 struct Ctrl {
-    uint32_t active_count[2];  // by frame parity
-    uint32_t cand_count[2];    // bricks with a node list before the screen, by frame parity
+    unsigned long long active_run[2];  // active list, by frame parity
+    unsigned long long cand_run[2];    // bricks with a node list before the screen, by frame parity
     uint32_t queue[2][kQueues * kQueueStride];  // pass-A queue counters by frame parity
     uint32_t overflow_records;  // reserved in the shared overflow arena (may exceed its capacity)
     uint32_t redo_count;        // bricks whose records fit no arena, recomputed by the redo pass
-    uint32_t list_total;        // node-list pool entries handed out by k_compact
     uint32_t frame;  // frames completed (k_pass_b counts); its parity serves replayed launches
 };
 
@@ -370,7 +373,6 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_bin(
     if (gid == 0) {
         ctrl->overflow_records = 0u;
         ctrl->redo_count = 0u;
-        ctrl->list_total = 0u;
     }
     __syncthreads();
 
@@ -519,24 +521,53 @@ __device__ __forceinline__ uint32_t compact_run(uint32_t flags, uint32_t* counte
     return *s_base;
 }
 
-// Exclusive prefix of v over the workgroup's threads, plus a base taken from
-// *counter for the workgroup's total (the whole workgroup calls it).
-__device__ __forceinline__ uint32_t block_alloc(uint32_t v, uint32_t* counter, uint32_t* s_w, uint32_t* s_base) {
+// compact_run for k_compact, which also places the run's node lists in the
+// pool: v is this thread's pool entries, *voff receives their first entry
+// (an exclusive prefix over the workgroup plus the run's pool base).  The run
+// and its pool entries are reserved together by one 64-bit atomic on *run
+// (low half: list positions, high half: pool entries; neither half can carry
+// into the other, both stay below 2^32), so the workgroup waits for one
+// contended atomic instead of two in a row.
+__device__ __forceinline__ uint32_t compact_alloc(uint32_t flags, uint32_t v, unsigned long long* run,
+                                                  uint32_t* s_cnt, uint32_t* s_w, uint2* s_base,
+                                                  uint32_t* s_total, uint32_t* voff) {
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < kCompactPer; ++r) {
+        const unsigned long long m = __ballot((flags >> r) & 1u);
+        if (lane == 0) s_cnt[r * kCompactWaves + wave] = (uint32_t)__popcll(m);
+    }
     const uint32_t incl = wave_inclusive_sum(v);
     if (lane == 63) s_w[wave] = incl;
     __syncthreads();
     if (wave == 0) {
+        static_assert(kCompactPer * kCompactWaves <= 128, "two table entries per lane of one wave");
         static_assert(kCompactWaves <= 64, "one wave total a lane");
+        const int e0 = 2 * lane;
+        const uint32_t c0 = e0 < kCompactPer * kCompactWaves ? s_cnt[e0] : 0u;
+        const uint32_t c1 = e0 + 1 < kCompactPer * kCompactWaves ? s_cnt[e0 + 1] : 0u;
+        const uint32_t ci = wave_inclusive_sum(c0 + c1);
+        const uint32_t ce = ci - (c0 + c1);
+        if (e0 < kCompactPer * kCompactWaves) s_cnt[e0] = ce;
+        if (e0 + 1 < kCompactPer * kCompactWaves) s_cnt[e0 + 1] = ce + c0;
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)ci, 63);
         const uint32_t x = lane < kCompactWaves ? s_w[lane] : 0u;
-        const uint32_t inc = wave_inclusive_sum(x);
-        if (lane < kCompactWaves) s_w[lane] = inc - x;
-        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-        if (lane == 0) *s_base = tot ? atomicAdd(counter, tot) : 0u;
+        const uint32_t xi = wave_inclusive_sum(x);
+        if (lane < kCompactWaves) s_w[lane] = xi - x;
+        const uint32_t ptot = (uint32_t)__builtin_amdgcn_readlane((int)xi, 63);
+        if (lane == 0) {
+            // a thread with pool entries has a listed brick, so ptot > 0 implies total > 0
+            const unsigned long long old =
+                total ? atomicAdd(run, ((unsigned long long)ptot << 32) | total) : 0ull;
+            *s_base = make_uint2((uint32_t)old, (uint32_t)(old >> 32));
+            *s_total = total;
+        }
     }
     __syncthreads();
-    return *s_base + s_w[wave] + (incl - v);
+    const uint2 b = *s_base;
+    *voff = b.y + s_w[wave] + (incl - v);
+    return b.x;
 }
 
 // List position of this thread's slot r after compact_run (whole waves call it).
@@ -712,8 +743,9 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
     uint32_t* __restrict__ cand, uint4* __restrict__ cand_info, uint32_t* __restrict__ tile_mark,
     uint32_t* __restrict__ vol_mark, Ctrl* __restrict__ ctrl, uint64_t brick_cap, int parity_arg) {
     __shared__ uint32_t s_cnt[kCompactPer * kCompactWaves];  // per (round, wave): count, then first position
-    __shared__ uint32_t s_base, s_total;
-    __shared__ uint32_t s_lw[kCompactWaves], s_lbase;
+    __shared__ uint32_t s_total;
+    __shared__ uint2 s_base;              // the run's first list position and first pool entry
+    __shared__ uint32_t s_lw[kCompactWaves];
     __shared__ uint32_t s_bb[kMaxBatch];  // first brick of each volume
     __shared__ uint32_t s_item[SCREEN ? kCompactPer * kPrepThreads : 1];  // the run's candidates, in list order
     for (int v = threadIdx.x; v < nvol; v += blockDim.x) s_bb[v] = vols[v].brick_base;
@@ -731,18 +763,16 @@ __global__ __launch_bounds__(kPrepThreads) void k_compact(
         lsum += lw[r];
         oc[r] = make_uint2(kNone, c);
     }
-    {
-        uint32_t off = block_alloc(lsum, &ctrl->list_total, s_lw, &s_lbase);
+    uint32_t off;
+    const uint32_t base = compact_alloc(flags, lsum, SCREEN ? &ctrl->cand_run[parity] : &ctrl->active_run[parity],
+                                        s_cnt, s_lw, &s_base, &s_total, &off);
 #pragma unroll
-        for (int r = 0; r < kCompactPer; ++r) {
-            // a spilled list has no place: its entries are dropped
-            if (lw[r]) oc[r].x = off;
-            if ((flags >> r) & 1u) brick_off[gb0 + (uint64_t)(r * kPrepThreads)] = oc[r].x;
-            off += lw[r];
-        }
+    for (int r = 0; r < kCompactPer; ++r) {
+        // a spilled list has no place: its entries are dropped
+        if (lw[r]) oc[r].x = off;
+        if ((flags >> r) & 1u) brick_off[gb0 + (uint64_t)(r * kPrepThreads)] = oc[r].x;
+        off += lw[r];
     }
-    const uint32_t base = compact_run(flags, SCREEN ? &ctrl->cand_count[parity] : &ctrl->active_count[parity],
-                                      s_cnt, &s_base, &s_total);
 #pragma unroll
     for (int r = 0; r < kCompactPer; ++r) {
         const bool act = (flags >> r) & 1u;
@@ -853,7 +883,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
     fill_lists(items, total_items, brick_off, brick_nodes, blockIdx.x * blockDim.x + threadIdx.x,
                gridDim.x * blockDim.x);
     const uint32_t parity = frame_parity(ctrl, parity_arg);
-    const uint32_t n = ctrl->cand_count[parity];
+    const uint32_t n = run_length(ctrl->cand_run[parity]);
     const uint32_t tag = ctrl->frame + 1u;
     const int q = (int)(threadIdx.x % kGrp);
     uint32_t dropped = 0;
@@ -913,7 +943,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_select(
                 flags |= (uint32_t)keep << r;
             }
         }
-        const uint32_t base = compact_run(flags, &ctrl->active_count[parity], s_cnt, &s_base);
+        // the low half of the 64-bit run counter (little-endian)
+        const uint32_t base = compact_run(flags, (uint32_t*)&ctrl->active_run[parity], s_cnt, &s_base);
 #pragma unroll
         for (int r = 0; r < kCompactPer; ++r) {
             const bool keep = (flags >> r) & 1u;
@@ -1015,11 +1046,11 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(5))
     if (!REDO && blockIdx.x == 0 && tid == 0) {
         // the other parity's list and counters belong to the previous frame,
         // whose kernels have finished
-        ctrl->active_count[parity ^ 1u] = 0u;
-        ctrl->cand_count[parity ^ 1u] = 0u;
+        ctrl->active_run[parity ^ 1u] = 0ull;
+        ctrl->cand_run[parity ^ 1u] = 0ull;
         for (uint32_t q = 0; q < kQueues; ++q) ctrl->queue[parity ^ 1u][q * kQueueStride] = 0u;
     }
-    const uint32_t nactive = REDO ? ctrl->redo_count : ctrl->active_count[parity];
+    const uint32_t nactive = REDO ? ctrl->redo_count : run_length(ctrl->active_run[parity]);
 
     // Each wave takes a 4x4x4 block of the brick: neighbour counts vary
     // less inside a compact block than along a 64-voxel slab, and a wave's
