@@ -358,8 +358,9 @@ def bench_gauss_newton(stream, G: int = 4096, M: int = 300_000, iters: int = 5) 
     the gn4096 workload: 4096 nodes, 300k mesh points with 6 knn slots, 4
     regularisation neighbours, targets made by a smooth rigid motion of every
     node.  vf_gn_solver_create builds the normal equations from the sparse
-    rows and factorises them (rocSOLVER potrf, or getrf where the Cholesky
-    factorisation fails); vf_gn_solver_iterate then runs `iters` Gauss-Newton
+    rows and factorises them (a blocked Cholesky factorisation: one-workgroup
+    diagonal blocks, rocBLAS trsm/syrk/gemm updates, look-ahead on a second
+    stream; rocSOLVER getrf where a pivot is not positive); vf_gn_solver_iterate then runs `iters` Gauss-Newton
     iterations (residuals, energy, JTr, the two triangular solves, update,
     mesh warp).  The reference rebuilds and refactorises JTJ on every
     iteration, i.e. one create per iteration."""
@@ -413,7 +414,7 @@ def bench_gauss_newton(stream, G: int = 4096, M: int = 300_000, iters: int = 5) 
         "rows": int(p.corr.shape[0] + 3 * G * R),
         "normal_equations_ms": ne_ms,
         "solver_create_ms": c_ms,
-        "factorisation": "cholesky (potrf)" if chol else "lu (getrf)",
+        "factorisation": "cholesky (blocked, look-ahead)" if chol else "lu (getrf)",
         "factor_ms_est": lu_ms,
         "factor_tflops_est": flops / (lu_ms * 1e-3) / 1e12,
         "factor_frac_fp32_peak": flops / (lu_ms * 1e-3) / FP32_PEAK,

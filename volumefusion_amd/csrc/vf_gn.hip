@@ -17,11 +17,11 @@
 //   - each node's rows are sorted by row index, and one wave per node sums
 //     its 6x6 blocks and its JTr entry over those rows in ascending order
 //     (deterministic; the reference's order is an atomic race);
-//   - JTJ is factorised once (rocSOLVER potrf: JTJ is symmetric positive
-//     definite, so its Cholesky factor solves the system the reference's LU
-//     with partial pivoting solves, in half the work; getrf where potrf
-//     finds a non-positive pivot), and each Gauss-Newton iteration only forms
-//     JTr from the new residuals and back-substitutes;
+//   - JTJ is factorised once (a blocked Cholesky factorisation: JTJ is
+//     symmetric positive definite, so its Cholesky factor solves the system
+//     the reference's LU with partial pivoting solves, in half the work;
+//     rocSOLVER getrf where a pivot is not positive), and each Gauss-Newton
+//     iteration only forms JTr from the new residuals and back-substitutes;
 //   - residuals, energy (binary64, fixed order), update and the mesh warp
 //     stay on the device; one read-back per iteration carries the energy the
 //     reference's stopping rule needs.
@@ -627,6 +627,158 @@ __global__ void k_energy_sum(const double* __restrict__ part, double* __restrict
     }
 }
 
+// ---- the Cholesky factorisation of JTJ -------------------------------------------
+//
+// A right-looking blocked L L^T (lower, column-major) in blocks of
+// kCholBlock columns.  Step k factorises the diagonal block k, solves the
+// rows below it against it (rocBLAS trsm), updates the next block column
+// (syrk on its diagonal block, gemm below it), and then, while the rest of
+// the trailing matrix is updated (syrk) on the main stream, factorises the
+// next diagonal block on a second stream (look-ahead).  rocBLAS keeps the
+// O(n^3) updates on the whole chip; the diagonal blocks, whose factorisation
+// is a chain of small dependent steps that rocSOLVER's potrf runs as many
+// tiny launches, take one workgroup each (k_chol_block).
+
+constexpr int kCholBlock = 256;                   // outer block (diagonal blocks)
+constexpr int kCholPanel = 32;                    // panel of k_chol_block
+constexpr int kCholThreads = 512;                 // k_chol_block workgroup
+constexpr int kCholRows = kCholBlock - kCholPanel;  // panel rows after the first panel
+constexpr int kCholLd = kCholPanel + 4;           // row stride of the panel factor in LDS
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Column J of the Cholesky factorisation of a pw x pw block (pw <= kCholPanel)
+// held by one wave, lane i holding row i in a[] (one instantiation per
+// column, so every index into a[] is a constant).  Returns the first column
+// whose pivot is not positive (or NaN), else -1.  Lanes at or beyond the
+// block hold zeros; entries above the diagonal are left as garbage.
+template <int J>
+__device__ __forceinline__ int chol_col(float (&a)[kCholPanel], int pw, int lane) {
+    if constexpr (J == kCholPanel) {
+        return -1;
+    } else {
+        if (J >= pw) return -1;
+        const float s = readlane_f(a[J], J);
+        if (!(s > 0.0f)) return J;
+        const float d = sqrtf(s);
+        const float l = a[J] / d;
+        a[J] = lane == J ? d : l;
+#pragma unroll
+        for (int k = J + 1; k < kCholPanel; ++k) a[k] = __builtin_fmaf(-l, readlane_f(l, k), a[k]);
+        return chol_col<J + 1>(a, pw, lane);
+    }
+}
+
+// L L^T of the kb x kb diagonal block at A (kb <= kCholBlock, leading
+// dimension lda), in panels of kCholPanel columns, left-looking: each panel
+// subtracts the products of the columns already factorised (staged in LDS),
+// one wave factorises its diagonal part in registers, and the rows below are
+// solved against it by substitution, one thread per row.  A pivot that is
+// not positive stores col0 + its column + 1 in *info (if still 0, as potrf's
+// info) and ends the kernel; a kernel that finds *info set does nothing.
+__global__ __launch_bounds__(kCholThreads) void k_chol_block(float* __restrict__ A, int lda, int kb, int col0,
+                                                             int* __restrict__ info) {
+    __shared__ float s_w[32 * kCholPanel];  // L rows of the panel's columns, t-major
+    __shared__ float s_r[32 * kCholRows];   // L rows of the panel's rows, t-major
+    __shared__ float s_l[kCholPanel * kCholLd];
+    __shared__ int s_fail;
+    const int tid = threadIdx.x;
+    if (*(volatile int*)info != 0) return;  // the same value for every thread
+    if (tid == 0) s_fail = -1;
+    for (int c0 = 0; c0 < kb; c0 += kCholPanel) {
+        const int pw = min(kCholPanel, kb - c0);
+        const int rows = kb - c0;  // <= kCholRows after the first panel
+        float* P = A + c0 + (size_t)c0 * lda;
+        if (c0 > 0) {
+            // P[r][c] -= sum_{t < c0} L[c0 + r][t] L[c0 + c][t], in chunks of 32 t
+            constexpr int kH = kCholPanel * 256 / kCholThreads;  // columns per thread
+            constexpr int kTS = kCholThreads / 256, kTN = 32 / kTS;
+            const int rr = tid & 255, cg = tid >> 8;
+            float acc[kH];
+#pragma unroll
+            for (int q = 0; q < kH; ++q) acc[q] = 0.0f;
+            for (int t0 = 0; t0 < c0; t0 += 32) {
+                for (int e = tid; e < 32 * kCholPanel; e += kCholThreads) {
+                    const int c = e % kCholPanel, t = e / kCholPanel;
+                    s_w[e] = c < pw ? A[(size_t)(c0 + c) + (size_t)(t0 + t) * lda] : 0.0f;
+                }
+                float v[kTN];
+#pragma unroll
+                for (int j = 0; j < kTN; ++j)
+                    v[j] = rr < rows ? A[(size_t)(c0 + rr) + (size_t)(t0 + cg + kTS * j) * lda] : 0.0f;
+                if (rr < kCholRows) {
+#pragma unroll
+                    for (int j = 0; j < kTN; ++j) s_r[(cg + kTS * j) * kCholRows + rr] = v[j];
+                }
+                __syncthreads();
+                if (rr < rows) {
+#pragma unroll 4
+                    for (int t = 0; t < 32; ++t) {
+                        const float l0 = s_r[t * kCholRows + rr];
+                        const float4* w4 = (const float4*)(s_w + t * kCholPanel + cg * kH);
+#pragma unroll
+                        for (int q = 0; q < kH / 4; ++q) {
+                            const float4 w = w4[q];
+                            acc[4 * q + 0] = __builtin_fmaf(l0, w.x, acc[4 * q + 0]);
+                            acc[4 * q + 1] = __builtin_fmaf(l0, w.y, acc[4 * q + 1]);
+                            acc[4 * q + 2] = __builtin_fmaf(l0, w.z, acc[4 * q + 2]);
+                            acc[4 * q + 3] = __builtin_fmaf(l0, w.w, acc[4 * q + 3]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int q = 0; q < kH; ++q) {
+                const int c = cg * kH + q;
+                if (c < pw && rr < rows && rr >= c) P[rr + (size_t)c * lda] -= acc[q];
+            }
+            __syncthreads();
+        }
+        if (tid < 64) {
+            const int i = tid;
+            float a[kCholPanel];
+#pragma unroll
+            for (int k = 0; k < kCholPanel; ++k) a[k] = (i < pw && k <= i) ? P[i + (size_t)k * lda] : 0.0f;
+            const int f = chol_col<0>(a, pw, i);
+            if (f >= 0) {
+                if (i == 0) s_fail = f;
+            } else if (i < kCholPanel) {
+#pragma unroll
+                for (int k = 0; k < kCholPanel; ++k) {
+                    const bool low = i < pw && k <= i;
+                    if (low) P[i + (size_t)k * lda] = a[k];
+                    s_l[i * kCholLd + k] = low ? a[k] : 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+        if (s_fail >= 0) {
+            if (tid == 0) atomicCAS(info, 0, col0 + c0 + s_fail + 1);
+            return;
+        }
+        // rows below the panel (pw == kCholPanel whenever there are any):
+        // x L11^T = b by forward substitution
+        const int nbelow = rows - pw;
+        if (tid < nbelow) {
+            float* B = P + pw + tid;
+            float x[kCholPanel];
+#pragma unroll
+            for (int c = 0; c < kCholPanel; ++c) {
+                float v = B[(size_t)c * lda];
+#pragma unroll
+                for (int t = 0; t < c; ++t) v = __builtin_fmaf(-x[t], s_l[c * kCholLd + t], v);
+                x[c] = v / s_l[c * kCholLd + c];
+            }
+#pragma unroll
+            for (int c = 0; c < kCholPanel; ++c) B[(size_t)c * lda] = x[c];
+        }
+        __syncthreads();
+    }
+}
+
 // ---- the solve with the factorised JTJ ---------------------------------------
 //
 // getrs (src/LinearSolver.cu:179-225 calls it through cuSOLVER) as blocked
@@ -768,6 +920,9 @@ struct vf_gn_solver {
     double* part = nullptr;
     double* energy = nullptr;
     rocblas_handle blas = nullptr;
+    hipStream_t aux = nullptr;          // the Cholesky look-ahead's stream
+    hipEvent_t ev_col = nullptr;        // next block column updated (main stream)
+    hipEvent_t ev_diag = nullptr;       // next diagonal block factorised (aux)
 };
 
 namespace vf {
@@ -779,6 +934,9 @@ void gn_free(vf_gn_solver* s) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (s->blas) (void)rocblas_destroy_handle(s->blas);
+    if (s->ev_col) (void)hipEventDestroy(s->ev_col);
+    if (s->ev_diag) (void)hipEventDestroy(s->ev_diag);
+    if (s->aux) (void)hipStreamDestroy(s->aux);
 }
 
 template <typename T>
@@ -872,6 +1030,53 @@ int read_bad(vf_gn_solver* s, hipStream_t st) {
         case 3: return fail(VF_ERR_CAPACITY, "gn: a node shares rows with more than %d nodes", kGnMaxNbr);
         default: return fail(VF_ERR_HIP, "gn: inconsistent row lists");
     }
+}
+
+// The blocked Cholesky factorisation of the n x n matrix A (lower triangle,
+// column-major), as described at k_chol_block; *info (device) as potrf's:
+// 0, or the first column (from 1) whose pivot is not positive.  The
+// rocBLAS calls run on the handle's stream hs, the diagonal blocks after the
+// first on the solver's second stream.
+int cholesky_lower(vf_gn_solver* s, hipStream_t hs, float* A, int n, int* info) {
+    if (hipMemsetAsync(info, 0, sizeof(int), hs) != hipSuccess) return fail(VF_ERR_HIP, "gn: info clear");
+    const float one = 1.0f, mone = -1.0f;
+    const auto diag = [&](hipStream_t st, int k, int kb) -> int {
+        hipLaunchKernelGGL(k_chol_block, dim3(1), dim3(kCholThreads), 0, st, A + k + (size_t)k * n, n, kb, k, info);
+        return hipGetLastError() == hipSuccess ? VF_OK : fail(VF_ERR_HIP, "k_chol_block launch");
+    };
+    const auto blas_ok = [](rocblas_status b) { return b == rocblas_status_success; };
+    int st = diag(hs, 0, std::min(kCholBlock, n));
+    for (int k = 0; !st && k < n; k += kCholBlock) {
+        const int kb = std::min(kCholBlock, n - k);
+        const int m = n - k - kb;  // rows below the diagonal block
+        if (k > 0 && hipStreamWaitEvent(hs, s->ev_diag, 0) != hipSuccess) st = fail(VF_ERR_HIP, "gn: stream wait");
+        if (st || m <= 0) break;
+        float* A11 = A + k + (size_t)k * n;
+        float* A21 = A11 + kb;
+        float* D1 = A + (k + kb) + (size_t)(k + kb) * n;  // the next diagonal block
+        const int nb1 = std::min(kCholBlock, m);
+        bool ok = blas_ok(rocblas_strsm(s->blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                        rocblas_diagonal_non_unit, m, kb, &one, A11, n, A21, n));
+        ok = ok && blas_ok(rocblas_ssyrk(s->blas, rocblas_fill_lower, rocblas_operation_none, nb1, kb, &mone, A21, n,
+                                         &one, D1, n));
+        if (ok && m > nb1)
+            ok = blas_ok(rocblas_sgemm(s->blas, rocblas_operation_none, rocblas_operation_transpose, m - nb1, nb1, kb,
+                                       &mone, A21 + nb1, n, A21, n, &one, D1 + nb1, n));
+        if (!ok) {
+            st = fail(VF_ERR_HIP, "gn: rocBLAS call in the Cholesky factorisation");
+            break;
+        }
+        if (hipEventRecord(s->ev_col, hs) != hipSuccess || hipStreamWaitEvent(s->aux, s->ev_col, 0) != hipSuccess)
+            st = fail(VF_ERR_HIP, "gn: look-ahead event");
+        if (!st) st = diag(s->aux, k + kb, nb1);
+        if (!st && hipEventRecord(s->ev_diag, s->aux) != hipSuccess) st = fail(VF_ERR_HIP, "gn: look-ahead event");
+        // the rest of the trailing matrix, beside the next diagonal block's factorisation
+        if (!st && m > nb1 &&
+            !blas_ok(rocblas_ssyrk(s->blas, rocblas_fill_lower, rocblas_operation_none, m - nb1, kb, &mone,
+                                   A21 + nb1, n, &one, D1 + nb1 + (size_t)nb1 * n, n)))
+            st = fail(VF_ERR_HIP, "gn: rocBLAS call in the Cholesky factorisation");
+    }
+    return st;
 }
 
 // y = JTJ^-1 x with the factors of vf_gn_solver_create: the gather (P, the
@@ -1127,6 +1332,10 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     if (!st && rocblas_create_handle(&s->blas) != rocblas_status_success)
         st = fail(VF_ERR_HIP, "rocblas_create_handle");
     if (!st && rocblas_set_stream(s->blas, hs) != rocblas_status_success) st = fail(VF_ERR_HIP, "rocblas_set_stream");
+    if (!st && (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&s->ev_col, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&s->ev_diag, hipEventDisableTiming) != hipSuccess))
+        st = fail(VF_ERR_HIP, "gn: look-ahead stream and events");
     if (!st) st = build_rows(s, hs);
     const auto normal_matrix = [&]() -> int {
         hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, s->JTJ, s->n);
@@ -1146,15 +1355,14 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     // JTJ does not change within the ICP iteration: factorise it once.  JTJ
     // (J^T J, with the identity where a node has no rows) is symmetric
     // positive definite unless the problem is degenerate, so the Cholesky
-    // factorisation L L^T (rocSOLVER potrf, half the work of the LU and no
-    // pivot search) solves the same system; the reference calls an LU with
-    // partial pivoting (linearSolverLUStable, src/LinearSolver.cu:179-225).
-    // Where potrf finds a non-positive pivot, JTJ is rebuilt and factorised
-    // as the reference does (getrf), which then reports a zero pivot as the
-    // reference's solver would.
-    if (!st && rocsolver_spotrf(s->blas, rocblas_fill_lower, (int)s->n, s->JTJ, (int)s->n, s->info) !=
-                   rocblas_status_success)
-        st = fail(VF_ERR_HIP, "rocsolver_spotrf");
+    // factorisation L L^T (half the work of the LU and no pivot search;
+    // blocked, cholesky_lower) solves the same system; the reference calls an
+    // LU with partial pivoting (linearSolverLUStable,
+    // src/LinearSolver.cu:179-225).  Where the Cholesky factorisation finds a
+    // non-positive pivot, JTJ is rebuilt and factorised as the reference
+    // does (getrf), which then reports a zero pivot as the reference's solver
+    // would.
+    if (!st) st = cholesky_lower(s, hs, s->JTJ, (int)s->n, s->info);
     if (!st) {
         int hinfo = 0;
         st = read_info(&hinfo);
