@@ -785,13 +785,21 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_block(float* __restrict__
 // substitutions for the solver loop, which solves with the same factors on
 // every Gauss-Newton iteration: the row interchanges as one gather (the
 // permutation composed once from ipiv), then L y = P b and U x = y in blocks
-// of 64 rows, one launch per block step.  Step k subtracts the block column
-// k of the factor times the solved block k from every block still to solve
-// (a GEMV over a column panel, coalesced along the rows; one wave per row
-// block), and the wave of the next block then solves its 64 x 64 diagonal
-// block by substitution in registers.  Rounding differs from rocSOLVER's
-// getrs (another summation order), not the method.
+// of 64 rows.  One launch per triangle: one wave per row block, the blocks
+// handed out in the order the waves start (a ticket), so a wave only ever
+// waits for blocks taken by waves already running.  Block b's wave loads
+// each block column of the factor it needs before that block's solution is
+// published, then subtracts the factor block times the solved block (in
+// block order), and finally solves its 64 x 64 diagonal block by
+// substitution in registers and publishes its solution (each value tagged
+// with the solve's epoch in one 64-bit atomic store).  The operations and
+// their order are those of a step-by-step blocked substitution (one launch
+// per block step, which the step-by-step version took); only the waiting is
+// per block instead of per launch.
+// Rounding differs from rocSOLVER's getrs (another summation order), not the
+// method.
 constexpr int kTB = 64;
+constexpr uint32_t kChainSpin = 1u << 22;  // polls of a block's flag before the solve gives up
 
 // Lane j's value of x, broadcast (j a constant of the unrolled loops).
 __device__ __forceinline__ float lane_value(float x, int j) {
@@ -804,30 +812,77 @@ __global__ __launch_bounds__(256) void k_gather_perm(const float* __restrict__ b
     if (i < n) y[i] = b[perm[i]];
 }
 
-// Forward step k (k = -1: the first diagonal block alone).  Wave w takes row
-// block k + 1 + w; the wave of block k + 1 also solves its diagonal block.
-// LU is column-major with leading dimension n; L has a unit diagonal (UNIT,
-// the LU factors) or its own (the Cholesky factor, divisions by it).
+// sync[0]: the ticket counter; sync[1]: a wait gave up (sticky until the
+// caller clears it).  tagged[i]: (epoch << 32) | bits of the solved value i,
+// written by one 64-bit atomic, so a reader that sees this solve's epoch sees
+// the value with it; no fence (an agent-scope fence writes back or
+// invalidates the whole L2 of the XCD) and no flags to clear between solves.
+
+// This wave's block: the ticket, in start order (one wave per workgroup).
+__device__ __forceinline__ int chain_ticket(int* sync) {
+    int t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(&sync[0], 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
+// Solved value i of this solve (lanes with need; others get 0): polls the
+// tagged word until it carries the epoch, bounded; on giving up, ok turns
+// false and the value is what memory holds, so that every wave still ends.
+__device__ __forceinline__ float chain_value(const unsigned long long* __restrict__ tagged, int i, bool need,
+                                             uint32_t epoch, bool& ok) {
+    unsigned long long v = 0;
+    if (need) {
+        uint32_t spin = 0;
+        for (;;) {
+            v = __hip_atomic_load(tagged + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(v >> 32) == epoch) break;
+            if (++spin > kChainSpin) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __uint_as_float((uint32_t)v);
+}
+
+// Stores this block's solution: the plain value for the kernels after this
+// launch, the tagged one for the waves of this launch.
+__device__ __forceinline__ void chain_publish(float* v, unsigned long long* __restrict__ tagged, int i, bool row,
+                                              float r, uint32_t epoch, int* sync, bool ok) {
+    if (row) {
+        v[i] = r;
+        __hip_atomic_store(tagged + i, ((unsigned long long)epoch << 32) | __float_as_uint(r), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (__ballot(!ok) != 0ull && threadIdx.x == 0) atomicOr(&sync[1], 1);
+}
+
+// L y = y in place.  LU is column-major with leading dimension n; L has a unit
+// diagonal (UNIT, the LU factors) or its own (the Cholesky factor, divisions
+// by it).  Grid: one 64-thread workgroup per row block.
 template <bool UNIT>
-__global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict__ LU, int n, int k,
-                                                        float* __restrict__ y) {
+__global__ __launch_bounds__(64) void k_trsv_lower_chain(const float* __restrict__ LU, int n, float* __restrict__ y,
+                                                         unsigned long long* __restrict__ tagged, uint32_t epoch,
+                                                         int* __restrict__ sync) {
     const int lane = threadIdx.x;
-    const int b = k + 1 + (int)blockIdx.x;
+    const int b = chain_ticket(sync);
     const int i = b * kTB + lane;
     const bool row = i < n;
     float r = row ? y[i] : 0.0f;
-    if (k >= 0) {
-        const int c0 = k * kTB;
-        const float yk = y[c0 + lane];  // block k is full: k < b <= last block
+    bool ok = true;
+    for (int k = 0; k < b; ++k) {
+        const int c0 = k * kTB;  // block k is full: k < b <= last block
         float a[kTB];
 #pragma unroll
         for (int j = 0; j < kTB; ++j) a[j] = row ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
+        const float yk = chain_value(tagged, c0 + lane, ok, epoch, ok);
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(yk, j);
         r = r - acc;
     }
-    if (b == k + 1) {
+    {
         const int c0 = b * kTB;
         const int m = n - c0 < kTB ? n - c0 : kTB;  // rows of this block
         float a[kTB];
@@ -843,38 +898,39 @@ __global__ __launch_bounds__(64) void k_trsv_lower_step(const float* __restrict_
             if (lane > j) r = r - a[j] * yj;
         }
     }
-    if (row) y[i] = r;
+    chain_publish(y, tagged, i, row, r, epoch, sync, ok);
 }
 
-// Backward step k (k = nb: the last diagonal block alone).  Wave w takes row
-// block k - 1 - w; the wave of block k - 1 also solves its diagonal block (U
-// upper, divisions by its diagonal).  U is the upper triangle of LU, or
+// U x = x in place, blocks from the last: U is the upper triangle of LU, or
 // (TRANS) the transpose of the Cholesky factor L in LU's lower triangle:
 // U(i, c) = L(c, i), which lane i reads from its column of L.
 template <bool TRANS>
-__global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict__ LU, int n, int k, int nb,
-                                                        float* __restrict__ x) {
+__global__ __launch_bounds__(64) void k_trsv_upper_chain(const float* __restrict__ LU, int n, float* __restrict__ x,
+                                                         unsigned long long* __restrict__ tagged, uint32_t epoch,
+                                                         int* __restrict__ sync) {
     const auto u_at = [&](int i, int c) -> float {
         return TRANS ? LU[(uint64_t)i * (uint64_t)n + (uint64_t)c] : LU[(uint64_t)c * (uint64_t)n + (uint64_t)i];
     };
     const int lane = threadIdx.x;
-    const int b = k - 1 - (int)blockIdx.x;
+    const int nb = (n + kTB - 1) / kTB;
+    const int b = nb - 1 - chain_ticket(sync);
     const int i = b * kTB + lane;
     const bool row = i < n;
     float r = row ? x[i] : 0.0f;
-    if (k < nb) {
+    bool ok = true;
+    for (int k = nb - 1; k > b; --k) {
         const int c0 = k * kTB;
         const int m = n - c0 < kTB ? n - c0 : kTB;
-        const float xk = lane < m ? x[c0 + lane] : 0.0f;
         float a[kTB];
 #pragma unroll
         for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
+        const float xk = chain_value(tagged, c0 + lane, ok && lane < m, epoch, ok);
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(xk, j);
         r = r - acc;
     }
-    if (b == k - 1) {
+    {
         const int c0 = b * kTB;
         const int m = n - c0 < kTB ? n - c0 : kTB;
         float a[kTB];
@@ -889,7 +945,7 @@ __global__ __launch_bounds__(64) void k_trsv_upper_step(const float* __restrict_
             }
         }
     }
-    if (row) x[i] = r;
+    chain_publish(x, tagged, i, row, r, epoch, sync, ok);
 }
 
 }  // namespace
@@ -917,6 +973,9 @@ struct vf_gn_solver {
     float* x = nullptr;    // JTr
     float* y = nullptr;    // the permuted JTr, solved in place into the update
     int* perm = nullptr;   // row interchanges of the factorisation as one permutation
+    int* chain = nullptr;  // the substitutions' ticket and failure flag
+    unsigned long long* tagged = nullptr;  // the substitutions' solved values with their solve's epoch
+    uint32_t epoch = 0;                    // solves launched (the tag of the next one is epoch + 1)
     double* part = nullptr;
     double* energy = nullptr;
     rocblas_handle blas = nullptr;
@@ -930,7 +989,7 @@ namespace {
 
 void gn_free(vf_gn_solver* s) {
     void* ptrs[] = {s->cnt, s->fill, s->rows_of, s->node8, s->j48,  s->bad,  s->JTJ,   s->ipiv,
-                    s->info, s->r,   s->x,       s->y,     s->perm, s->part, s->energy};
+                    s->info, s->r,   s->x,       s->y,     s->perm, s->part, s->energy, s->chain, s->tagged};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (s->blas) (void)rocblas_destroy_handle(s->blas);
@@ -1088,24 +1147,26 @@ int lu_solve_factored(vf_gn_solver* s, hipStream_t st) {
     hipLaunchKernelGGL(k_gather_perm, dim3(blocks_for((uint64_t)n, 256)), dim3(256), 0, st, (const float*)s->x,
                        (const int*)s->perm, s->y, n);
     VF_LAUNCH_CHECK("k_gather_perm");
-    for (int k = -1; k <= nb - 2; ++k) {
-        const dim3 grid(k < 0 ? 1u : (unsigned)(nb - 1 - k));
-        if (s->chol)
-            hipLaunchKernelGGL(k_trsv_lower_step<false>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, s->y);
-        else
-            hipLaunchKernelGGL(k_trsv_lower_step<true>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, s->y);
-        VF_LAUNCH_CHECK("k_trsv_lower_step");
-    }
-    for (int k = nb; k >= 1; --k) {
-        const dim3 grid(k == nb ? 1u : (unsigned)k);
-        if (s->chol)
-            hipLaunchKernelGGL(k_trsv_upper_step<true>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, nb,
-                               s->y);
-        else
-            hipLaunchKernelGGL(k_trsv_upper_step<false>, grid, dim3(kTB), 0, st, (const float*)s->JTJ, n, k, nb,
-                               s->y);
-        VF_LAUNCH_CHECK("k_trsv_upper_step");
-    }
+    // the ticket cleared before each triangle (not the failure flag); a new tag
+    // per triangle (0 is never one: the tags start cleared)
+    VF_HIP(hipMemsetAsync(s->chain, 0, sizeof(int), st));
+    if (++s->epoch == 0) ++s->epoch;
+    if (s->chol)
+        hipLaunchKernelGGL(k_trsv_lower_chain<false>, dim3(nb), dim3(kTB), 0, st, (const float*)s->JTJ, n, s->y,
+                           s->tagged, s->epoch, s->chain);
+    else
+        hipLaunchKernelGGL(k_trsv_lower_chain<true>, dim3(nb), dim3(kTB), 0, st, (const float*)s->JTJ, n, s->y,
+                           s->tagged, s->epoch, s->chain);
+    VF_LAUNCH_CHECK("k_trsv_lower_chain");
+    VF_HIP(hipMemsetAsync(s->chain, 0, sizeof(int), st));
+    if (++s->epoch == 0) ++s->epoch;
+    if (s->chol)
+        hipLaunchKernelGGL(k_trsv_upper_chain<true>, dim3(nb), dim3(kTB), 0, st, (const float*)s->JTJ, n, s->y,
+                           s->tagged, s->epoch, s->chain);
+    else
+        hipLaunchKernelGGL(k_trsv_upper_chain<false>, dim3(nb), dim3(kTB), 0, st, (const float*)s->JTJ, n, s->y,
+                           s->tagged, s->epoch, s->chain);
+    VF_LAUNCH_CHECK("k_trsv_upper_chain");
     return VF_OK;
 }
 
@@ -1327,6 +1388,11 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     if (!st) st = gn_alloc(&s->x, s->n);
     if (!st) st = gn_alloc(&s->y, s->n);
     if (!st) st = gn_alloc(&s->perm, s->n);
+    if (!st) st = gn_alloc(&s->chain, 2);
+    if (!st) st = gn_alloc(&s->tagged, s->n);
+    if (!st && (hipMemsetAsync(s->chain, 0, 2 * sizeof(int), hs) != hipSuccess ||
+                hipMemsetAsync(s->tagged, 0, sizeof(unsigned long long) * s->n, hs) != hipSuccess))
+        st = fail(VF_ERR_HIP, "gn: substitution state clear");
     if (!st) st = gn_alloc(&s->part, kEnergyBlocks);
     if (!st) st = gn_alloc(&s->energy, 1);
     if (!st && rocblas_create_handle(&s->blas) != rocblas_status_success)
@@ -1439,6 +1505,7 @@ int vf_gn_solver_iterate(vf_gn_solver* s, vf_mat4* transforms, vf_mat4* pending,
     const uint32_t G = s->rows.G;
     // the loop of src/VolumeFusion.cu:541-691: pending starts as the transforms
     VF_HIP(hipMemcpyAsync(pending, transforms, sizeof(vf_mat4) * G, hipMemcpyDeviceToDevice, hs));
+    VF_HIP(hipMemsetAsync(s->chain + 1, 0, sizeof(int), hs));  // the substitutions' failure flag
     double energy = 1.0e300;
     for (int it = 0; it < max_iters; ++it) {
         int st = launch_residuals(s, (const float*)warped, depth_mesh, pending, hs);
@@ -1469,7 +1536,10 @@ int vf_gn_solver_iterate(vf_gn_solver* s, vf_mat4* transforms, vf_mat4* pending,
         if (st) return st;
         report->iterations = it + 1;
     }
+    int gave_up = 0;
+    VF_HIP(hipMemcpyAsync(&gave_up, s->chain + 1, sizeof(int), hipMemcpyDeviceToHost, hs));
     VF_HIP(hipStreamSynchronize(hs));
+    if (gave_up) return fail(VF_ERR_HIP, "gn: a triangular solve's wait for a block gave up");
     return VF_OK;
 }
 
