@@ -551,7 +551,9 @@ int vf_gn_solver_destroy(vf_gn_solver* solver);
  * the reference sums in float in thrust's order); stops when the energy drops
  * by less than min_step; else commits pending to transforms (after the first
  * iteration), solves JTJ x = JTr, updates pending and re-warps the mesh.
- * Blocks; reads back one double per iteration. */
+ * Blocks; reads back one double per iteration and one int at the end.
+ * VF_ERR_HIP if a triangular solve's wait for a solved block gave up (a
+ * bounded poll; the solve's waves end regardless). */
 int vf_gn_solver_iterate(vf_gn_solver* solver, vf_mat4* transforms, vf_mat4* pending, vf_float3* warped,
                          const vf_float3* canon, size_t mesh_size, const vf_float3* depth_mesh, int max_iters,
                          float min_step, vf_gn_report* report, vf_stream stream);
