@@ -188,7 +188,7 @@ def _oracle_loop(oracle, p, T, iters, min_step, float_lu=False, float_chol=False
     return T, pending, energies, steps
 
 
-@pytest.mark.parametrize("G,M", [(48, 3000), (50, 3000), (200, 12000)])
+@pytest.mark.parametrize("G,M", [(48, 3000), (49, 3000), (50, 3000), (200, 12000)])
 def test_gn_solver_loop(vf, oracle, G, M):
     """The Gauss-Newton loop on the device (JTJ factorised once) against the
     oracle's loop: the same number of steps, energies within 1e-3, the energy
@@ -199,7 +199,8 @@ def test_gn_solver_loop(vf, oracle, G, M):
     a float factorisation moves the update by up to about 1e-4).  The sizes
     cover the factorisation's blocking: 288 unknowns (a 256-column block and
     a 32-column one), 300 (a last panel of 12 columns) and 1200 (four
-    look-ahead steps and a 176-column last block)."""
+    look-ahead steps and a 176-column last block); 294 leaves the columns of
+    the factor unaligned to 16 bytes (the transposed solve's scalar loads)."""
     p = scenes.gn_problem(6, G=G, M=M, warp_angle_deg=3.0, warp_translation=0.006)
     T0 = np.tile(np.eye(4, dtype=F).reshape(16), (G, 1))
     iters, min_step = 4, 1e-7

@@ -918,12 +918,27 @@ __global__ __launch_bounds__(64) void k_trsv_upper_chain(const float* __restrict
     const bool row = i < n;
     float r = row ? x[i] : 0.0f;
     bool ok = true;
+    // (TRANS) a lane's entries of a block are 64 consecutive floats of its
+    // column of L: 16-byte loads where the column starts 16-byte aligned
+    const bool vec = TRANS && (n & 3) == 0;
     for (int k = nb - 1; k > b; --k) {
         const int c0 = k * kTB;
         const int m = n - c0 < kTB ? n - c0 : kTB;
         float a[kTB];
+        if (vec && m == kTB) {
+            const float4* col = (const float4*)(LU + (uint64_t)(row ? i : 0) * (uint64_t)n + (uint64_t)c0);
 #pragma unroll
-        for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
+            for (int j = 0; j < kTB / 4; ++j) {
+                const float4 v = row ? col[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a[4 * j] = v.x;
+                a[4 * j + 1] = v.y;
+                a[4 * j + 2] = v.z;
+                a[4 * j + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
+        }
         const float xk = chain_value(tagged, c0 + lane, ok && lane < m, epoch, ok);
         float acc = 0.0f;
 #pragma unroll
