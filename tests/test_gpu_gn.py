@@ -231,3 +231,23 @@ def test_gn_solver_loop(vf, oracle, G, M):
           f"float LAPACK LU or Cholesky {(tol_T - 1e-6) / 3:.3g}", flush=True)
     np.testing.assert_allclose(dT.download(), T_or, rtol=0, atol=tol_T)
     np.testing.assert_allclose(dP.download(), pend_or, rtol=0, atol=tol_P)
+
+
+def test_gn_solver_singular_normal_matrix(vf):
+    """A normal matrix that is not positive definite: with zero normals and a
+    zero regularisation weight every node's rows have zero Jacobians, so JTJ
+    is zero wherever a node has rows.  The Cholesky factorisation meets a
+    zero pivot in its first diagonal block and leaves the later ones alone,
+    the solver falls back to the LU with partial pivoting, which finds a zero
+    pivot too: solver creation reports VF_ERR_SINGULAR (the reference's LU
+    would fail on the same matrix).  522 unknowns: three diagonal blocks."""
+    import dataclasses
+    G = 87
+    p = scenes.gn_problem(6, G=G, M=3000, warp_angle_deg=3.0, warp_translation=0.006)
+    p = dataclasses.replace(p, dnormals=np.zeros_like(p.dnormals), wR=0.0)
+    d = _Dev(p)
+    solver = C.c_void_p()
+    rc = L.load().vf_gn_solver_create(C.byref(solver), C.byref(d.desc), None)
+    if rc == 0:
+        L.load().vf_gn_solver_destroy(solver)
+    assert rc == 7  # VF_ERR_SINGULAR
