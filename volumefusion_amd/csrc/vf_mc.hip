@@ -514,13 +514,25 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_emit(McGeom g, const float* __r
         my_off = offsets[q0 + (uint64_t)lane * kstride];
     }
     unsigned long long steps = __ballot(my_sum != 0u);
+    // the next step with output: its slices are loaded while this one
+    // computes and writes (a step right after the last reuses its upper slice)
+    Slice nlo{}, nhi{};
+    if (steps) {
+        const int32_t k = rp.k0 + __builtin_ctzll(steps);
+        nlo = load_slice(g, tsdf, wt, x, rp.j, k);
+        nhi = load_slice(g, tsdf, wt, x, rp.j, k + 1);
+    }
     while (steps) {
         const int kk = __builtin_ctzll(steps);
         steps &= steps - 1ull;
         const int32_t k = rp.k0 + kk;
         const uint32_t run_off = __shfl(my_off, kk);
-        const Slice lo = load_slice(g, tsdf, wt, x, rp.j, k);
-        const Slice hi = load_slice(g, tsdf, wt, x, rp.j, k + 1);
+        const Slice lo = nlo, hi = nhi;
+        if (steps) {
+            const int kn = __builtin_ctzll(steps);
+            nlo = kn == kk + 1 ? hi : load_slice(g, tsdf, wt, x, rp.j, rp.k0 + kn);
+            nhi = load_slice(g, tsdf, wt, x, rp.j, rp.k0 + kn + 1);
+        }
         Cube cb;
         make_cube(g, x, rp.j, k, lo, hi, &cb);
         unsigned keep = 0;
