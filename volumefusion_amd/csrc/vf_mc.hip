@@ -409,17 +409,20 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __
         } else {
             // 3 per triangle where the screen rules out snapped edge points
             // (then no triangle is degenerate); else the run is flagged
-            // and k_mc_exact counts it
-            const float d0 = fabsf(lo.t0 - g.iso), d1 = fabsf(lo.t1 - g.iso);
-            const float d2 = fabsf(hi.t0 - g.iso), d3 = fabsf(hi.t1 - g.iso);
-            const bool bad = !(d0 <= 3.0e38f) || !(d1 <= 3.0e38f) || !(d2 <= 3.0e38f) || !(d3 <= 3.0e38f);
-            const float mo = fminf(fminf(d0, d1), fminf(d2, d3)), Mo = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
-            const float m = fminf(mo, from_next_lane(mo)), M = fmaxf(Mo, from_next_lane(Mo));
-            const bool bad2 = bad || __builtin_amdgcn_update_dpp(0, (int)bad, 0x130, 0xf, 0xf, false) != 0;
+            // and k_mc_exact counts it.  Most steps of most waves hold no
+            // surface cube: the screen is skipped for the whole wave.
             const bool surface = own && cs != 0u && cs != 255u;
-            const bool exact = surface && (bad2 || !screen_ok(g, m, M));
-            n = surface && !exact ? 3u * ((e >> 16) & 255u) : 0u;
-            if (__ballot(exact)) flagged |= 1u << t;
+            if (__ballot(surface) != 0ull) {
+                const float d0 = fabsf(lo.t0 - g.iso), d1 = fabsf(lo.t1 - g.iso);
+                const float d2 = fabsf(hi.t0 - g.iso), d3 = fabsf(hi.t1 - g.iso);
+                const bool bad = !(d0 <= 3.0e38f) || !(d1 <= 3.0e38f) || !(d2 <= 3.0e38f) || !(d3 <= 3.0e38f);
+                const float mo = fminf(fminf(d0, d1), fminf(d2, d3)), Mo = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
+                const float m = fminf(mo, from_next_lane(mo)), M = fmaxf(Mo, from_next_lane(Mo));
+                const bool bad2 = bad || __builtin_amdgcn_update_dpp(0, (int)bad, 0x130, 0xf, 0xf, false) != 0;
+                const bool exact = surface && (bad2 || !screen_ok(g, m, M));
+                n = surface && !exact ? 3u * ((e >> 16) & 255u) : 0u;
+                if (__ballot(exact)) flagged |= 1u << t;
+            }
         }
         // two steps per wave reduction: a run's count is below 2^16
         // (63 cubes x at most 15 vertices)
