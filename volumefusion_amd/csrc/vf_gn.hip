@@ -870,6 +870,12 @@ __global__ __launch_bounds__(64) void k_trsv_lower_chain(const float* __restrict
     const int i = b * kTB + lane;
     const bool row = i < n;
     float r = row ? y[i] : 0.0f;
+    // the diagonal block first: its loads are then off the chain of waits
+    const int cb = b * kTB;
+    const int mb = n - cb < kTB ? n - cb : kTB;  // rows of this block
+    float d[kTB];
+#pragma unroll
+    for (int j = 0; j < kTB; ++j) d[j] = (row && j < mb) ? LU[(uint64_t)(cb + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
     bool ok = true;
     for (int k = 0; k < b; ++k) {
         const int c0 = k * kTB;  // block k is full: k < b <= last block
@@ -882,21 +888,13 @@ __global__ __launch_bounds__(64) void k_trsv_lower_chain(const float* __restrict
         for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(yk, j);
         r = r - acc;
     }
-    {
-        const int c0 = b * kTB;
-        const int m = n - c0 < kTB ? n - c0 : kTB;  // rows of this block
-        float a[kTB];
 #pragma unroll
-        for (int j = 0; j < kTB; ++j)
-            a[j] = (row && j < m) ? LU[(uint64_t)(c0 + j) * (uint64_t)n + (uint64_t)i] : 0.0f;
-#pragma unroll
-        for (int j = 0; j < kTB; ++j) {
-            // final: lane j has subtracted columns 0..j-1 (and, with its own
-            // diagonal, divides by it: a[j] of lane j is L(j, j))
-            if (!UNIT && lane == j && j < m) r = r / a[j];
-            const float yj = lane_value(r, j);
-            if (lane > j) r = r - a[j] * yj;
-        }
+    for (int j = 0; j < kTB; ++j) {
+        // final: lane j has subtracted columns 0..j-1 (and, with its own
+        // diagonal, divides by it: d[j] of lane j is L(j, j))
+        if (!UNIT && lane == j && j < mb) r = r / d[j];
+        const float yj = lane_value(r, j);
+        if (lane > j) r = r - d[j] * yj;
     }
     chain_publish(y, tagged, i, row, r, epoch, sync, ok);
 }
@@ -917,6 +915,12 @@ __global__ __launch_bounds__(64) void k_trsv_upper_chain(const float* __restrict
     const int i = b * kTB + lane;
     const bool row = i < n;
     float r = row ? x[i] : 0.0f;
+    // the diagonal block first: its loads are then off the chain of waits
+    const int cb = b * kTB;
+    const int mb = n - cb < kTB ? n - cb : kTB;
+    float d[kTB];
+#pragma unroll
+    for (int j = 0; j < kTB; ++j) d[j] = (row && j < mb) ? u_at(i, cb + j) : 0.0f;
     bool ok = true;
     // (TRANS) a lane's entries of a block are 64 consecutive floats of its
     // column of L: 16-byte loads where the column starts 16-byte aligned
@@ -945,19 +949,12 @@ __global__ __launch_bounds__(64) void k_trsv_upper_chain(const float* __restrict
         for (int j = 0; j < kTB; ++j) acc = acc + a[j] * lane_value(xk, j);
         r = r - acc;
     }
-    {
-        const int c0 = b * kTB;
-        const int m = n - c0 < kTB ? n - c0 : kTB;
-        float a[kTB];
 #pragma unroll
-        for (int j = 0; j < kTB; ++j) a[j] = (row && j < m) ? u_at(i, c0 + j) : 0.0f;
-#pragma unroll
-        for (int j = kTB - 1; j >= 0; --j) {
-            if (j < m) {
-                if (lane == j) r = r / a[j];  // a[j] of lane j is the diagonal U(j, j)
-                const float xj = lane_value(r, j);
-                if (lane < j) r = r - a[j] * xj;
-            }
+    for (int j = kTB - 1; j >= 0; --j) {
+        if (j < mb) {
+            if (lane == j) r = r / d[j];  // d[j] of lane j is the diagonal U(j, j)
+            const float xj = lane_value(r, j);
+            if (lane < j) r = r - d[j] * xj;
         }
     }
     chain_publish(x, tagged, i, row, r, epoch, sync, ok);
