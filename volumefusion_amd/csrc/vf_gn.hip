@@ -630,7 +630,9 @@ __global__ void k_energy_sum(const double* __restrict__ part, double* __restrict
 // ---- the Cholesky factorisation of JTJ -------------------------------------------
 //
 // A right-looking blocked L L^T (lower, column-major) in blocks of
-// kCholBlock columns.  Step k factorises the diagonal block k, solves the
+// kCholOuter columns.  Step k factorises the diagonal block k (two halves of
+// at most kCholBlock columns by k_chol_block, with a trsm and a syrk
+// between them), solves the
 // rows below it against it (rocBLAS trsm), updates the next block column
 // (syrk on its diagonal block, gemm below it), and then, while the rest of
 // the trailing matrix is updated (syrk) on the main stream, factorises the
@@ -639,7 +641,8 @@ __global__ void k_energy_sum(const double* __restrict__ part, double* __restrict
 // is a chain of small dependent steps that rocSOLVER's potrf runs as many
 // tiny launches, take one workgroup each (k_chol_block).
 
-constexpr int kCholBlock = 256;                   // outer block (diagonal blocks)
+constexpr int kCholOuter = 512;                   // outer block (rank of the trailing updates)
+constexpr int kCholBlock = 256;                   // largest block of k_chol_block
 constexpr int kCholPanel = 32;                    // panel of k_chol_block
 constexpr int kCholThreads = 512;                 // k_chol_block workgroup
 constexpr int kCholRows = kCholBlock - kCholPanel;  // panel rows after the first panel
@@ -991,6 +994,7 @@ struct vf_gn_solver {
     double* part = nullptr;
     double* energy = nullptr;
     rocblas_handle blas = nullptr;
+    rocblas_handle blas_aux = nullptr;  // rocBLAS on the look-ahead stream
     hipStream_t aux = nullptr;          // the Cholesky look-ahead's stream
     hipEvent_t ev_col = nullptr;        // next block column updated (main stream)
     hipEvent_t ev_diag = nullptr;       // next diagonal block factorised (aux)
@@ -1005,6 +1009,7 @@ void gn_free(vf_gn_solver* s) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (s->blas) (void)rocblas_destroy_handle(s->blas);
+    if (s->blas_aux) (void)rocblas_destroy_handle(s->blas_aux);
     if (s->ev_col) (void)hipEventDestroy(s->ev_col);
     if (s->ev_diag) (void)hipEventDestroy(s->ev_diag);
     if (s->aux) (void)hipStreamDestroy(s->aux);
@@ -1111,21 +1116,37 @@ int read_bad(vf_gn_solver* s, hipStream_t st) {
 int cholesky_lower(vf_gn_solver* s, hipStream_t hs, float* A, int n, int* info) {
     if (hipMemsetAsync(info, 0, sizeof(int), hs) != hipSuccess) return fail(VF_ERR_HIP, "gn: info clear");
     const float one = 1.0f, mone = -1.0f;
-    const auto diag = [&](hipStream_t st, int k, int kb) -> int {
-        hipLaunchKernelGGL(k_chol_block, dim3(1), dim3(kCholThreads), 0, st, A + k + (size_t)k * n, n, kb, k, info);
-        return hipGetLastError() == hipSuccess ? VF_OK : fail(VF_ERR_HIP, "k_chol_block launch");
-    };
     const auto blas_ok = [](rocblas_status b) { return b == rocblas_status_success; };
-    int st = diag(hs, 0, std::min(kCholBlock, n));
-    for (int k = 0; !st && k < n; k += kCholBlock) {
-        const int kb = std::min(kCholBlock, n - k);
+    // the kb x kb diagonal block at column k (kb <= kCholOuter) on stream st,
+    // whose rocBLAS handle is h
+    const auto diag = [&](rocblas_handle h, hipStream_t st, int k, int kb) -> int {
+        // right-looking over blocks of at most kCholBlock columns
+        for (int c = 0; c < kb; c += kCholBlock) {
+            float* D = A + (k + c) + (size_t)(k + c) * n;
+            const int h1 = std::min(kCholBlock, kb - c), h2 = kb - c - h1;
+            hipLaunchKernelGGL(k_chol_block, dim3(1), dim3(kCholThreads), 0, st, D, n, h1, k + c, info);
+            if (hipGetLastError() != hipSuccess) return fail(VF_ERR_HIP, "k_chol_block launch");
+            if (h2 == 0) break;
+            float* D21 = D + h1;
+            float* D22 = D21 + (size_t)h1 * n;
+            if (!blas_ok(rocblas_strsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                       rocblas_diagonal_non_unit, h2, h1, &one, D, n, D21, n)) ||
+                !blas_ok(rocblas_ssyrk(h, rocblas_fill_lower, rocblas_operation_none, h2, h1, &mone, D21, n, &one,
+                                       D22, n)))
+                return fail(VF_ERR_HIP, "gn: rocBLAS call in the Cholesky factorisation");
+        }
+        return VF_OK;
+    };
+    int st = diag(s->blas, hs, 0, std::min(kCholOuter, n));
+    for (int k = 0; !st && k < n; k += kCholOuter) {
+        const int kb = std::min(kCholOuter, n - k);
         const int m = n - k - kb;  // rows below the diagonal block
         if (k > 0 && hipStreamWaitEvent(hs, s->ev_diag, 0) != hipSuccess) st = fail(VF_ERR_HIP, "gn: stream wait");
         if (st || m <= 0) break;
         float* A11 = A + k + (size_t)k * n;
         float* A21 = A11 + kb;
         float* D1 = A + (k + kb) + (size_t)(k + kb) * n;  // the next diagonal block
-        const int nb1 = std::min(kCholBlock, m);
+        const int nb1 = std::min(kCholOuter, m);
         bool ok = blas_ok(rocblas_strsm(s->blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                                         rocblas_diagonal_non_unit, m, kb, &one, A11, n, A21, n));
         ok = ok && blas_ok(rocblas_ssyrk(s->blas, rocblas_fill_lower, rocblas_operation_none, nb1, kb, &mone, A21, n,
@@ -1139,7 +1160,7 @@ int cholesky_lower(vf_gn_solver* s, hipStream_t hs, float* A, int n, int* info) 
         }
         if (hipEventRecord(s->ev_col, hs) != hipSuccess || hipStreamWaitEvent(s->aux, s->ev_col, 0) != hipSuccess)
             st = fail(VF_ERR_HIP, "gn: look-ahead event");
-        if (!st) st = diag(s->aux, k + kb, nb1);
+        if (!st) st = diag(s->blas_aux, s->aux, k + kb, nb1);
         if (!st && hipEventRecord(s->ev_diag, s->aux) != hipSuccess) st = fail(VF_ERR_HIP, "gn: look-ahead event");
         // the rest of the trailing matrix, beside the next diagonal block's factorisation
         if (!st && m > nb1 &&
@@ -1414,6 +1435,9 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
                 hipEventCreateWithFlags(&s->ev_col, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&s->ev_diag, hipEventDisableTiming) != hipSuccess))
         st = fail(VF_ERR_HIP, "gn: look-ahead stream and events");
+    if (!st && (rocblas_create_handle(&s->blas_aux) != rocblas_status_success ||
+                rocblas_set_stream(s->blas_aux, s->aux) != rocblas_status_success))
+        st = fail(VF_ERR_HIP, "gn: look-ahead rocBLAS handle");
     if (!st) st = build_rows(s, hs);
     const auto normal_matrix = [&]() -> int {
         hipLaunchKernelGGL(k_identity, dim3(2048), dim3(256), 0, hs, s->JTJ, s->n);
