@@ -531,7 +531,7 @@ def bench_batching(vols, rendered, steps: int, warmup: int) -> dict:
 
 def bench_serving(vols, rendered, seconds: float, fps: float = 30.0, seed: int = 5, max_batch: int = 8,
                   lanes: int = 8, rounds: int = 32,
-                  ramp_rates=(120.0, 240.0, 300.0, 360.0, 400.0, 440.0, 480.0, 500.0, 520.0)) -> dict:
+                  ramp_rates=(120.0, 240.0, 300.0, 360.0, 400.0, 440.0, 480.0, 500.0, 520.0, 540.0, 560.0)) -> dict:
     """Poisson arrivals into the serving loop at 30 fps per stream and at
     rising rates (one second each; the steps narrow where the p99 latency
     approaches 33 ms), the streams saturated, and plain per-volume calls
