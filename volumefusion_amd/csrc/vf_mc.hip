@@ -272,13 +272,18 @@ struct Slice {
 __device__ __forceinline__ Slice load_slice(const McGeom& g, const float* __restrict__ tsdf,
                                             const float* __restrict__ wt, int32_t x, int32_t j, int32_t k) {
     const int32_t xc = x < g.X ? x : g.X - 1;
-    const uint64_t o0 = ((uint64_t)k * (uint64_t)g.Y + (uint64_t)j) * (uint64_t)g.X + (uint64_t)xc;
-    const uint64_t o1 = o0 + (uint64_t)g.X;
+    // the slice's base (scalar where k is uniform, as in the sweeps) and the
+    // lane's 32-bit place in it (run_mc checks that a slice has < 2^31 voxels)
+    const uint64_t sk = (uint64_t)k * (uint64_t)g.X * (uint64_t)g.Y;
+    const float* __restrict__ tk = tsdf + sk;
+    const float* __restrict__ wk = wt + sk;
+    const uint32_t o0 = (uint32_t)j * (uint32_t)g.X + (uint32_t)xc;
+    const uint32_t o1 = o0 + (uint32_t)g.X;
     Slice s;
-    s.t0 = tsdf[o0];
-    s.t1 = tsdf[o1];
-    s.w0 = wt[o0];
-    s.w1 = wt[o1];
+    s.t0 = tk[o0];
+    s.t1 = tk[o1];
+    s.w0 = wk[o0];
+    s.w1 = wk[o1];
     return s;
 }
 
@@ -612,6 +617,9 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     uint64_t n;
     int st = check_volume(dims, vs, &n);
     if (st) return st;
+    if ((uint64_t)dims.x * (uint64_t)dims.y >= (1ull << 31))
+        return fail(VF_ERR_INVALID_ARGUMENT, "marching cubes: a z slice of %d x %d voxels (at most 2^31 - 1)", dims.x,
+                    dims.y);
     if (!mesh || !mesh_size || !tsdf || !wt || (full && (!faces || !faces_size)))
         return fail(VF_ERR_INVALID_ARGUMENT, "marching cubes: NULL argument");
     if (dims.x < 3 || dims.y < 3 || dims.z < 3) return VF_OK;  // no cube is visited
