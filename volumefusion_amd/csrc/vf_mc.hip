@@ -5,7 +5,7 @@
 // counter, so its output order is a race and the counter a hot spot.  Here
 // the output is compacted in a canonical order (cubes by linear index, edges
 // by number, triangles in table order) with two sweeps:
-//   k_mc_count   per cube: case and output count; per run of 63 cubes of a
+//   k_mc_count   per cube: case and output count; per run of 60 cubes of a
 //                row: sum.  With faces, a cube whose triangles may be
 //                degenerate flags its run for the next kernel.
 //   k_mc_exact   (faces only) the flagged runs' exact sums
@@ -257,7 +257,9 @@ __device__ __forceinline__ int cube_outputs(const McGeom& g, const Cube& cb, uns
 // weight of rows j and j + 1).  Runs are numbered q = (k * cy + j) * nr + r,
 // which is the cubes' linear order, so per-run sums scanned in q order give
 // every cube's output offset.
-constexpr int kRun = 63;
+constexpr int kRun = 60;          // 15 lanes x 4 cubes in the count sweep (one 16-lane row per run)
+constexpr int kRunLanes = kRun / 4;
+static_assert(kRunLanes == 15, "a run fills one 16-lane row but its last lane");
 constexpr int kMarch = 16;  // z steps per wave
 static_assert(kMarch <= 64, "one lane per step holds the run sums");
 
@@ -357,17 +359,86 @@ __device__ __forceinline__ RunPos run_pos(const McGeom& g) {
     return p;
 }
 
-// Above-iso flags of a lane's two rows (bits 0, 1) and whether either weight
-// is 0 (bit 2), for one slice.
-__device__ __forceinline__ uint32_t slice_bits(const Slice& s, float iso) {
-    return (s.t0 > iso ? 1u : 0u) | (s.t1 > iso ? 2u : 0u) | ((s.w0 == 0.0f || s.w1 == 0.0f) ? 4u : 0u);
+// The count sweep's mapping: a workgroup's four waves take four consecutive
+// rows j of the same four runs and z segment (rows two waves share are read
+// from HBM once, as in run_pos); in a wave, 16-lane row q takes run r + q,
+// its lane s the four columns x0 = (r + q) * kRun + 4 s (lane 15 only
+// supplies the corners at x0 of its row).  Tile numbers go to the XCDs in
+// chunks as in run_pos.
+__device__ __forceinline__ RunPos run_pos4(const McGeom& g) {
+    const uint32_t groups = ((uint32_t)g.cy + (kMcBlock / 64) - 1) / (kMcBlock / 64);
+    const uint32_t rg = ((uint32_t)g.nr + 3u) / 4u;  // groups of four runs per row
+    const uint32_t chunk = kMcChunk * rg;
+    const uint32_t whole = gridDim.x / (8u * chunk) * (8u * chunk);
+    uint32_t b = blockIdx.x;
+    if (b < whole) {
+        const uint32_t i = b >> 3;
+        b = ((i / chunk) * 8u + (blockIdx.x & 7u)) * chunk + i % chunk;
+    }
+    RunPos p;
+    p.r = (int32_t)(b % rg) * 4;
+    p.j = (int32_t)(((b / rg) % groups) * (kMcBlock / 64) + (threadIdx.x >> 6));
+    p.k0 = (int32_t)(b / (rg * groups)) * kMarch;
+    p.live = p.k0 < g.cz && p.j < g.cy;
+    return p;
+}
+
+// One slice of four columns: tsdf and weight of rows j and j + 1.
+struct Slice4 {
+    float t0[4], t1[4], w0[4], w1[4];
+};
+
+// VEC: 16-byte loads (rows 16-byte aligned); a lane whose four columns start
+// past the row's end reads the row's last four instead.  Else one load per
+// column, columns beyond the row reading the last one (as load_slice).
+// Either way only cubes no lane owns see columns past the row's end (their
+// x + 1 corner of a lane's last cube lies past cx otherwise), and the loads
+// are the same on every pass (no branch around them), so the waits before a
+// step cover only the slices it reads.
+template <bool VEC>
+__device__ __forceinline__ Slice4 load_slice4(const McGeom& g, const float* __restrict__ tsdf,
+                                              const float* __restrict__ wt, int32_t x0, int32_t j, int32_t k) {
+    const uint64_t sk = (uint64_t)k * (uint64_t)g.X * (uint64_t)g.Y;
+    const float* __restrict__ tk = tsdf + sk;
+    const float* __restrict__ wk = wt + sk;
+    const uint32_t o0 = (uint32_t)j * (uint32_t)g.X, o1 = o0 + (uint32_t)g.X;
+    Slice4 s;
+    if (VEC) {
+        const uint32_t xs = (uint32_t)(x0 < g.X - 4 ? x0 : g.X - 4);
+        const float4 a = *(const float4*)(tk + o0 + xs), b = *(const float4*)(tk + o1 + xs);
+        const float4 c = *(const float4*)(wk + o0 + xs), d = *(const float4*)(wk + o1 + xs);
+        s.t0[0] = a.x, s.t0[1] = a.y, s.t0[2] = a.z, s.t0[3] = a.w;
+        s.t1[0] = b.x, s.t1[1] = b.y, s.t1[2] = b.z, s.t1[3] = b.w;
+        s.w0[0] = c.x, s.w0[1] = c.y, s.w0[2] = c.z, s.w0[3] = c.w;
+        s.w1[0] = d.x, s.w1[1] = d.y, s.w1[2] = d.z, s.w1[3] = d.w;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t xc = (uint32_t)(x0 + c < g.X ? x0 + c : g.X - 1);
+            s.t0[c] = tk[o0 + xc];
+            s.t1[c] = tk[o1 + xc];
+            s.w0[c] = wk[o0 + xc];
+            s.w1[c] = wk[o1 + xc];
+        }
+    }
+    return s;
+}
+
+// Inclusive sum within each 16-lane row (DPP row shifts; VALU only).
+__device__ __forceinline__ uint32_t row_inclusive_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    return x;
 }
 
 // With FULL the sweep counts 3 per triangle where screen_ok rules out
 // degenerate triangles and leaves the other runs to k_mc_exact: the exact
-// test's registers (about 120 VGPRs, which held the sweep at six waves per
-// SIMD with spills) stay out of the sweep.
-template <bool FULL>
+// test's registers stay out of the sweep.  Four cubes per lane: a wave's
+// step reads four 1 KB row segments (tsdf and weight of rows j, j + 1) and
+// the per-step work (addresses, sums, loop) is shared by 240 cubes.
+template <bool FULL, bool VEC>
 __global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __restrict__ tsdf,
                                                         const float* __restrict__ wt, uint32_t* __restrict__ sums,
                                                         uint32_t* __restrict__ exact_list) {
@@ -375,96 +446,130 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_count(McGeom g, const float* __
     static_assert(kMcBlock == 256, "one lookup entry per thread");
     s_lut[threadIdx.x] = kMcCaseLut.v[threadIdx.x];
     __syncthreads();
-    const RunPos rp = run_pos(g);
+    const RunPos rp = run_pos4(g);
     if (!rp.live) return;
     const int lane = __lane_id();
-    const int32_t x = rp.r * kRun + lane;
-    const bool own = lane < kRun && x < g.cx;
+    const int q = lane >> 4, sub = lane & 15;
+    const int32_t r = rp.r + q;  // this 16-lane row's run
+    const bool run_ok = r < g.nr;
+    const int32_t x0 = r * kRun + 4 * sub;
+    uint32_t ownm = 0;  // bit c: the cube at x0 + c is this lane's
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ownm |= (uint32_t)(run_ok && sub < kRunLanes && x0 + c < g.cx) << c;
     const int32_t k1 = rp.k0 + kMarch < g.cz ? rp.k0 + kMarch : g.cz;
-    // slices k .. k + 3 are loaded or in flight while step k computes; near
-    // the segment's end the prefetch re-reads slice k1 (the last one used)
-    // A ring of four slices, slice k0 + i in s[i mod 4], unrolled by four so
-    // that the ring stays in place: step k reads slices k and k + 1 and then
-    // reloads slice k's registers with slice k + 4 (past the segment's end
-    // the last slice used, k1, again), so three slices stay in flight while
-    // a step computes.  Every pass issues the same loads and no other memory
-    // operation (the run sums are kept in lanes and stored after the march),
-    // so the waits before each step cover only the slices it reads.  (A loop
-    // that moved the slices along, lo = hi, ..., with its stores inside,
-    // waited at every step for the load it had just issued.)
-    Slice s0 = load_slice(g, tsdf, wt, x, rp.j, rp.k0);
-    Slice s1 = load_slice(g, tsdf, wt, x, rp.j, rp.k0 + 1);
-    Slice s2 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 2, k1));
-    Slice s3 = load_slice(g, tsdf, wt, x, rp.j, min(rp.k0 + 3, k1));
-    uint32_t pending = 0;  // the previous step's count, 16 bits, reduced with this one
-    uint32_t my_sum = 0;   // lane t: the sum of step k0 + t
-    uint32_t flagged = 0;  // bit t: step k0 + t has a cube for k_mc_exact (uniform)
-    const auto step = [&](int32_t k, const Slice& lo, const Slice& hi) {
-        const uint32_t blo = slice_bits(lo, g.iso), bhi = slice_bits(hi, g.iso);
-        // this lane's corner flags and the next lane's (x + 1), one DPP move
-        const uint32_t mine = (blo & 3u) | ((bhi & 3u) << 2) | (((blo | bhi) & 4u) << 2);
-        const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mine, 0x130, 0xf, 0xf, false);
-        const bool zero = ((mine | next) & 16u) != 0u;
-        const uint32_t e = s_lut[(mine & 15u) | ((next & 15u) << 4)];
-        const uint32_t cs = zero ? 0u : (e & 255u);
+    // a ring of four slices as in the one-cube sweep: step k reads slices k
+    // and k + 1, then reloads slice k's registers with slice k + 4 (past the
+    // segment's end the last slice used, k1, again; a ring of three, at five
+    // waves per SIMD instead of three, was no faster)
+    Slice4 s0 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, rp.k0);
+    Slice4 s1 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, rp.k0 + 1);
+    Slice4 s2 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(rp.k0 + 2, k1));
+    Slice4 s3 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(rp.k0 + 3, k1));
+    uint32_t pending = 0;          // the previous step's count, 16 bits, reduced with this one
+    uint32_t my_sum = 0;           // lane 16 q + t: the sum of run r + q at step k0 + t
+    unsigned long long flagged = 0;  // bit 16 q + t: run r + q at step k0 + t has a cube for k_mc_exact
+    const auto step = [&](int32_t k, const Slice4& lo, const Slice4& hi) {
+        // per column: above-iso flags (rows j, j + 1 of slices k, k + 1) and a
+        // zero-weight bit; column 4 is the next lane's column 0
+        uint32_t code[5];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            code[c] = (lo.t0[c] > g.iso ? 1u : 0u) | (lo.t1[c] > g.iso ? 2u : 0u) | (hi.t0[c] > g.iso ? 4u : 0u) |
+                      (hi.t1[c] > g.iso ? 8u : 0u) |
+                      ((lo.w0[c] == 0.0f || lo.w1[c] == 0.0f || hi.w0[c] == 0.0f || hi.w1[c] == 0.0f) ? 16u : 0u);
+        }
+        code[4] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)code[0], 0x130, 0xf, 0xf, false);
+        uint32_t e[4], cs[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool zero = ((code[c] | code[c + 1]) & 16u) != 0u;
+            e[c] = s_lut[(code[c] & 15u) | ((code[c + 1] & 15u) << 4)];
+            cs[c] = zero ? 0u : (e[c] & 255u);
+        }
         const int t = k - rp.k0;
         uint32_t n = 0;
         if (!FULL) {
-            n = own && !zero ? (e >> 8) & 255u : 0u;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                n += ((ownm >> c) & 1u) && ((code[c] | code[c + 1]) & 16u) == 0u ? (e[c] >> 8) & 255u : 0u;
         } else {
-            // 3 per triangle where the screen rules out snapped edge points
-            // (then no triangle is degenerate); else the run is flagged
-            // and k_mc_exact counts it.  Most steps of most waves hold no
-            // surface cube: the screen is skipped for the whole wave.
-            const bool surface = own && cs != 0u && cs != 255u;
-            if (__ballot(surface) != 0ull) {
-                const float d0 = fabsf(lo.t0 - g.iso), d1 = fabsf(lo.t1 - g.iso);
-                const float d2 = fabsf(hi.t0 - g.iso), d3 = fabsf(hi.t1 - g.iso);
-                const bool bad = !(d0 <= 3.0e38f) || !(d1 <= 3.0e38f) || !(d2 <= 3.0e38f) || !(d3 <= 3.0e38f);
-                const float mo = fminf(fminf(d0, d1), fminf(d2, d3)), Mo = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
-                const float m = fminf(mo, from_next_lane(mo)), M = fmaxf(Mo, from_next_lane(Mo));
-                const bool bad2 = bad || __builtin_amdgcn_update_dpp(0, (int)bad, 0x130, 0xf, 0xf, false) != 0;
-                const bool exact = surface && (bad2 || !screen_ok(g, m, M));
-                n = surface && !exact ? 3u * ((e >> 16) & 255u) : 0u;
-                if (__ballot(exact)) flagged |= 1u << t;
+            uint32_t surf = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) surf |= (uint32_t)(((ownm >> c) & 1u) && cs[c] != 0u && cs[c] != 255u) << c;
+            // most steps of most waves hold no surface cube: the screen is
+            // skipped for the whole wave
+            if (__ballot(surf != 0u) != 0ull) {
+                // 3 per triangle where the screen rules out snapped edge
+                // points (then no triangle is degenerate); else the run is
+                // flagged and k_mc_exact counts it
+                float dmn[5], dmx[5];
+                uint32_t bad[5];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float d0 = fabsf(lo.t0[c] - g.iso), d1 = fabsf(lo.t1[c] - g.iso);
+                    const float d2 = fabsf(hi.t0[c] - g.iso), d3 = fabsf(hi.t1[c] - g.iso);
+                    bad[c] = (!(d0 <= 3.0e38f) || !(d1 <= 3.0e38f) || !(d2 <= 3.0e38f) || !(d3 <= 3.0e38f)) ? 1u : 0u;
+                    dmn[c] = fminf(fminf(d0, d1), fminf(d2, d3));
+                    dmx[c] = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3));
+                }
+                dmn[4] = from_next_lane(dmn[0]);
+                dmx[4] = from_next_lane(dmx[0]);
+                bad[4] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)bad[0], 0x130, 0xf, 0xf, false);
+                bool exact_any = false;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float m = fminf(dmn[c], dmn[c + 1]), M = fmaxf(dmx[c], dmx[c + 1]);
+                    const bool surface = ((surf >> c) & 1u) != 0u;
+                    const bool exact = surface && ((bad[c] | bad[c + 1]) != 0u || !screen_ok(g, m, M));
+                    n += surface && !exact ? 3u * ((e[c] >> 16) & 255u) : 0u;
+                    exact_any = exact_any || exact;
+                }
+                const unsigned long long em = __ballot(exact_any);
+#pragma unroll
+                for (int rq = 0; rq < 4; ++rq)
+                    if ((em >> (16 * rq)) & 0xffffull) flagged |= 1ull << (16 * rq + t);
             }
         }
-        // two steps per wave reduction: a run's count is below 2^16
-        // (63 cubes x at most 15 vertices)
+        // two steps per row reduction: a run's count is below 2^16 (60 cubes
+        // x at most 15 vertices)
         const bool odd = (t & 1) != 0;
         if (!odd && k + 1 < k1) {
             pending = n;
         } else {
-            const uint32_t both = wave_sum(odd ? (pending | (n << 16)) : n);
+            const uint32_t tot = row_inclusive_sum(odd ? (pending | (n << 16)) : n);
+            uint32_t both = (uint32_t)__builtin_amdgcn_readlane((int)tot, 15);
+            both = q == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)tot, 31) : both;
+            both = q == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)tot, 47) : both;
+            both = q == 3 ? (uint32_t)__builtin_amdgcn_readlane((int)tot, 63) : both;
             if (odd) {
-                my_sum = lane == t - 1 ? (both & 0xffffu) : my_sum;
-                my_sum = lane == t ? (both >> 16) : my_sum;
+                my_sum = sub == t - 1 ? (both & 0xffffu) : my_sum;
+                my_sum = sub == t ? (both >> 16) : my_sum;
             } else {
-                my_sum = lane == t ? both : my_sum;
+                my_sum = sub == t ? both : my_sum;
             }
         }
     };
     for (int32_t k = rp.k0; k < k1; k += 4) {
         step(k, s0, s1);
-        s0 = load_slice(g, tsdf, wt, x, rp.j, min(k + 4, k1));
+        s0 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(k + 4, k1));
         if (k + 1 < k1) step(k + 1, s1, s2);
-        s1 = load_slice(g, tsdf, wt, x, rp.j, min(k + 5, k1));
+        s1 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(k + 5, k1));
         if (k + 2 < k1) step(k + 2, s2, s3);
-        s2 = load_slice(g, tsdf, wt, x, rp.j, min(k + 6, k1));
+        s2 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(k + 6, k1));
         if (k + 3 < k1) step(k + 3, s3, s0);
-        s3 = load_slice(g, tsdf, wt, x, rp.j, min(k + 7, k1));
+        s3 = load_slice4<VEC>(g, tsdf, wt, x0, rp.j, min(k + 7, k1));
     }
     // run q of step k0 + t: q0 + t * cy * nr
-    const uint64_t q0 = ((uint64_t)rp.k0 * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)rp.r;
     const uint64_t kstride = (uint64_t)g.cy * (uint64_t)g.nr;
-    if (lane < k1 - rp.k0) sums[q0 + (uint64_t)lane * kstride] = my_sum;
+    const uint64_t q0 = ((uint64_t)rp.k0 * (uint64_t)g.cy + (uint64_t)rp.j) * (uint64_t)g.nr + (uint64_t)r;
+    if (run_ok && sub < k1 - rp.k0) sums[q0 + (uint64_t)sub * kstride] = my_sum;
     if (FULL && flagged) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(exact_list, (uint32_t)__popc(flagged));
+        if (lane == 0) base = atomicAdd(exact_list, (uint32_t)__popcll(flagged));
         base = (uint32_t)__shfl((int)base, 0);
-        if (lane < 32 && ((flagged >> lane) & 1u))
-            exact_list[1u + base + (uint32_t)__popc(flagged & ((1u << lane) - 1u))] =
-                (uint32_t)(q0 + (uint64_t)lane * kstride);
+        if ((flagged >> lane) & 1ull)
+            exact_list[1u + base + (uint32_t)__popcll(flagged & ((1ull << lane) - 1ull))] =
+                (uint32_t)(q0 + (uint64_t)sub * kstride);
     }
 }
 
@@ -663,6 +768,11 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     if (nb64 > 0x7fffffffull || nq > 0x7fffffffull)
         return fail(VF_ERR_CAPACITY, "marching cubes: %llu runs", (unsigned long long)nq);
     const uint32_t nb = (uint32_t)nb64;
+    // the count sweep: a workgroup per four runs (16-lane rows) of four rows
+    const uint32_t nb4 =
+        (uint32_t)(((uint64_t)g.nr + 3) / 4 * groups * (uint64_t)((g.cz + kMarch - 1) / kMarch));
+    // 16-byte loads where every row starts 16-byte aligned
+    const bool vec = (g.X & 3) == 0 && (((uintptr_t)tsdf | (uintptr_t)wt) & 15u) == 0;
     size_t scan_bytes = 0;
     VF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nq,
                                             stream));
@@ -680,15 +790,24 @@ int run_mc(vf_workspace* ws, bool full, float* mesh, uint32_t* mesh_size, uint64
     void* scan_tmp = (char*)scratch + scan_off;
     if (full) {
         VF_HIP(hipMemsetAsync(exact_list, 0, sizeof(uint32_t), stream));
-        hipLaunchKernelGGL(k_mc_count<true>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums, exact_list);
+        if (vec)
+            hipLaunchKernelGGL((k_mc_count<true, true>), dim3(nb4), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                               exact_list);
+        else
+            hipLaunchKernelGGL((k_mc_count<true, false>), dim3(nb4), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                               exact_list);
         VF_LAUNCH_CHECK("k_mc_count");
         const uint32_t nb_exact = (uint32_t)std::min<uint64_t>(nb, 512);
         hipLaunchKernelGGL(k_mc_exact, dim3(nb_exact), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
                            (const uint32_t*)exact_list);
         VF_LAUNCH_CHECK("k_mc_exact");
     } else {
-        hipLaunchKernelGGL(k_mc_count<false>, dim3(nb), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
-                           (uint32_t*)nullptr);
+        if (vec)
+            hipLaunchKernelGGL((k_mc_count<false, true>), dim3(nb4), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                               (uint32_t*)nullptr);
+        else
+            hipLaunchKernelGGL((k_mc_count<false, false>), dim3(nb4), dim3(kMcBlock), 0, stream, g, tsdf, wt, sums,
+                               (uint32_t*)nullptr);
         VF_LAUNCH_CHECK("k_mc_count");
     }
     VF_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, sums, offsets, (int)nq, stream));
