@@ -256,7 +256,8 @@ int vf_tsdf_load_host(const char* path, vf_tsdf_info* info, float* tsdf, float* 
  * number) where the reference's atomics give a racy order; *mesh_size
  * advances by the full count, and entries at or beyond mesh_capacity are not
  * written.  Uses the workspace's scratch (NULL: the device's default
- * workspace, for callers without one; such calls must not overlap). */
+ * workspace, for callers without one; such calls must not overlap).
+ * VF_ERR_INVALID_ARGUMENT if a z slice (X x Y) holds 2^31 voxels or more. */
 int vf_marching_cubes_mesh(vf_workspace* ws, vf_float3* mesh, uint32_t* mesh_size, uint64_t mesh_capacity,
                            const float* tsdf, const float* weights, vf_int3 dims, vf_float3 voxel_size,
                            vf_float3 volume_size, float iso, vf_stream stream);
