@@ -544,6 +544,14 @@ typedef struct vf_gn_report {
     int cholesky;         /* 1: JTJ was factorised by Cholesky, 0: by LU   */
 } vf_gn_report;
 int vf_gn_solver_create(vf_gn_solver** solver, const vf_gn_problem* problem, vf_stream stream);
+/* No counterpart in the reference (whose solver always runs its LU): the
+ * same with the factorisation chosen.  VF_GN_FACTOR_AUTO is
+ * vf_gn_solver_create's (Cholesky, the LU where a pivot is not positive);
+ * VF_GN_FACTOR_LU always takes the reference's LU with partial pivoting
+ * (twice the flops), for callers who want its pivoting. */
+enum { VF_GN_FACTOR_AUTO = 0, VF_GN_FACTOR_LU = 1 };
+int vf_gn_solver_create_ex(vf_gn_solver** solver, const vf_gn_problem* problem, int factorisation,
+                           vf_stream stream);
 int vf_gn_solver_destroy(vf_gn_solver* solver);
 /* The Gauss-Newton loop of src/VolumeFusion.cu:541-691: pending starts as a
  * copy of transforms; each iteration computes the residuals (depth rows from

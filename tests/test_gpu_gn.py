@@ -188,8 +188,9 @@ def _oracle_loop(oracle, p, T, iters, min_step, float_lu=False, float_chol=False
     return T, pending, energies, steps
 
 
+@pytest.mark.parametrize("factor", [0, 1], ids=["auto", "lu"])
 @pytest.mark.parametrize("G,M", [(48, 3000), (49, 3000), (50, 3000), (200, 12000)])
-def test_gn_solver_loop(vf, oracle, G, M):
+def test_gn_solver_loop(vf, oracle, G, M, factor):
     """The Gauss-Newton loop on the device (JTJ factorised once) against the
     oracle's loop: the same number of steps, energies within 1e-3, the energy
     falling twentyfold or more, and transforms as close to the binary64 loop
@@ -200,7 +201,10 @@ def test_gn_solver_loop(vf, oracle, G, M):
     cover the factorisation's blocking: 288 unknowns (a 256-column block and
     a 32-column one), 300 (a last panel of 12 columns) and 1200 (four
     look-ahead steps and a 176-column last block); 294 leaves the columns of
-    the factor unaligned to 16 bytes (the transposed solve's scalar loads)."""
+    the factor unaligned to 16 bytes (the transposed solve's scalar loads).
+    factor 1 (VF_GN_FACTOR_LU) takes the reference's LU with partial
+    pivoting instead: the permutation gather, the unit lower and the upper
+    substitutions."""
     p = scenes.gn_problem(6, G=G, M=M, warp_angle_deg=3.0, warp_translation=0.006)
     T0 = np.tile(np.eye(4, dtype=F).reshape(16), (G, 1))
     iters, min_step = 4, 1e-7
@@ -212,7 +216,7 @@ def test_gn_solver_loop(vf, oracle, G, M):
     tol_P = 3.0 * max(float(np.abs(pend_f - pend_or).max()), float(np.abs(pend_c - pend_or).max())) + 1e-6
     d = _Dev(p)
     solver = C.c_void_p()
-    L.call("vf_gn_solver_create", C.byref(solver), C.byref(d.desc), None)
+    L.call("vf_gn_solver_create_ex", C.byref(solver), C.byref(d.desc), factor, None)
     try:
         dT, dP = dev(T0), DeviceArray((G, 16), F)
         warped = dev(_warp(oracle, p, T0))
@@ -224,7 +228,10 @@ def test_gn_solver_loop(vf, oracle, G, M):
     e_dev = [rep.energy[i] for i in range(len(e_or))]
     print(f"energies oracle {e_or}, device {e_dev}, steps {steps_or}/{rep.iterations}", flush=True)
     assert rep.iterations == steps_or
-    assert rep.cholesky == 1, "a positive definite JTJ is factorised by Cholesky"
+    if factor == 0:
+        assert rep.cholesky == 1, "a positive definite JTJ is factorised by Cholesky"
+    else:
+        assert rep.cholesky == 0, "VF_GN_FACTOR_LU takes the LU"
     np.testing.assert_allclose(e_dev, e_or, rtol=1e-3, atol=1e-9)
     assert e_or[-1] < 0.05 * e_or[0]
     print(f"transform deviation from the binary64 loop: device {np.abs(dT.download() - T_or).max():.3g}, "

@@ -222,6 +222,7 @@ _SIGS = {
     "vf_lu_solve": [C.c_int, vp, C.c_int, vp, vp, vp],
     "vf_gn_normal_equations": [vp, vp, vp, vp, C.POINTER(GNProblem), vp],
     "vf_gn_solver_create": [C.POINTER(vp), C.POINTER(GNProblem), vp],
+    "vf_gn_solver_create_ex": [C.POINTER(vp), C.POINTER(GNProblem), C.c_int, vp],
     "vf_gn_solver_destroy": [vp],
     "vf_gn_solver_iterate": [vp, vp, vp, vp, vp, sz, vp, C.c_int, f32, C.POINTER(GNReport), vp],
     "vf_filter_depth": [vp, u32, f32, f32, vp],

@@ -1401,7 +1401,13 @@ int vf_gn_normal_equations(vf_workspace* ws, float* JTJ, float* JTr, const float
 }
 
 int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_stream stream) {
+    return vf_gn_solver_create_ex(out, problem, VF_GN_FACTOR_AUTO, stream);
+}
+
+int vf_gn_solver_create_ex(vf_gn_solver** out, const vf_gn_problem* problem, int factorisation, vf_stream stream) {
     if (!out) return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_create: out is NULL");
+    if (factorisation != VF_GN_FACTOR_AUTO && factorisation != VF_GN_FACTOR_LU)
+        return fail(VF_ERR_INVALID_ARGUMENT, "vf_gn_solver_create: factorisation %d", factorisation);
     *out = nullptr;
     int st = check_problem(problem);
     if (st) return st;
@@ -1464,14 +1470,15 @@ int vf_gn_solver_create(vf_gn_solver** out, const vf_gn_problem* problem, vf_str
     // non-positive pivot, JTJ is rebuilt and factorised as the reference
     // does (getrf), which then reports a zero pivot as the reference's solver
     // would.
-    if (!st) st = cholesky_lower(s, hs, s->JTJ, (int)s->n, s->info);
-    if (!st) {
+    if (!st && factorisation == VF_GN_FACTOR_AUTO) {
+        st = cholesky_lower(s, hs, s->JTJ, (int)s->n, s->info);
         int hinfo = 0;
-        st = read_info(&hinfo);
+        if (!st) st = read_info(&hinfo);
         s->chol = !st && hinfo == 0;
     }
     if (!st && !s->chol) {
-        st = normal_matrix();
+        // (after a Cholesky attempt JTJ holds its partial factor)
+        if (factorisation == VF_GN_FACTOR_AUTO) st = normal_matrix();
         // Partial pivoting does interchange rows on these matrices (the
         // rotation columns' couplings exceed their diagonals), so the LU
         // without pivot searches is not an option.
