@@ -198,9 +198,10 @@ def test_gn_solver_loop(vf, oracle, G, M, factor):
     of its LU and its Cholesky factorisation, the one the device uses for
     this positive definite JTJ: JTJ's condition number here is about 3e5, so
     a float factorisation moves the update by up to about 1e-4).  The sizes
-    cover the factorisation's blocking: 288 unknowns (a 256-column block and
-    a 32-column one), 300 (a last panel of 12 columns) and 1200 (four
-    look-ahead steps and a 176-column last block); 294 leaves the columns of
+    cover the factorisation's blocking: 288 unknowns (one diagonal block in
+    a 256-column half and a 32-column one), 300 (a last panel of 12
+    columns) and 1200 (two look-ahead steps of 512 columns and a 176-column
+    last block); 294 leaves the columns of
     the factor unaligned to 16 bytes (the transposed solve's scalar loads).
     factor 1 (VF_GN_FACTOR_LU) takes the reference's LU with partial
     pivoting instead: the permutation gather, the unit lower and the upper
