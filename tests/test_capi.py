@@ -65,6 +65,17 @@ def test_invalid_arguments_rejected_on_host():
     with pytest.raises(L.VFError):
         L.call("vf_integrate_tsdf", None, None, None, L.Int2(4, 4), 0.01, L.i3((4, 4, 4)), L.f3((0.01,) * 3),
                L.f3((0.04,) * 3), L.m3([1, 0, 0, 0, 1, 0, 0, 0, 1]), L.f3((0, 0, 0)), L.m3([1] * 9), 15.0, None)
+    # the Gauss-Newton solver's factorisation choice is checked first
+    import ctypes as C
+
+    solver = C.c_void_p()
+    with pytest.raises(L.VFError) as e:
+        L.call("vf_gn_solver_create_ex", C.byref(solver), None, 7, None)
+    assert e.value.status == 1 and "factorisation 7" in str(e.value)
+    with pytest.raises(L.VFError) as e:
+        L.call("vf_gn_solver_create_ex", C.byref(solver), None, 1, None)
+    assert e.value.status == 1 and "NULL problem" in str(e.value)
+    assert not solver.value
 
 
 def test_voxels_per_node_matches_reference_formula():
