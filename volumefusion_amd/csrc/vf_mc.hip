@@ -250,11 +250,12 @@ __device__ __forceinline__ int cube_outputs(const McGeom& g, const Cube& cb, uns
     return n;
 }
 
-// The sweeps march along z.  A wave owns a run of kRun cubes of one row
-// (j, k): lane l holds voxel column x = r * kRun + l, and the cube at x takes
-// its x + 1 corners from lane l + 1 (lane 63 only supplies corners).  Each
-// step down z loads one new slice: 4 coalesced loads per lane (tsdf and
-// weight of rows j and j + 1).  Runs are numbered q = (k * cy + j) * nr + r,
+// The sweeps march along z.  In the exact and emit sweeps a wave owns a run
+// of kRun cubes of one row (j, k): lane l holds voxel column x = r * kRun + l,
+// and the cube at x takes its x + 1 corners from lane l + 1 (lane kRun only
+// supplies corners, the lanes after it idle).  Each step down z loads one new
+// slice: 4 coalesced loads per lane (tsdf and weight of rows j and j + 1).
+// The count sweep takes four runs per wave, four cubes per lane (run_pos4).  Runs are numbered q = (k * cy + j) * nr + r,
 // which is the cubes' linear order, so per-run sums scanned in q order give
 // every cube's output offset.
 constexpr int kRun = 60;          // 15 lanes x 4 cubes in the count sweep (one 16-lane row per run)
